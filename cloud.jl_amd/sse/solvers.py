@@ -1,0 +1,307 @@
+"""Residual forms, ``Solver``, ``semidiscretize`` and ``semi_discrete_residual``.
+
+Mirrors ``src/Solvers/Solvers.jl`` (forms ``:90-115``, ``Solver`` ``:259-272``,
+``project_function``/``initialize``/``semidiscretize`` ``:388-452``,
+``semi_discrete_residual!`` ``:474-564``), ``src/Solvers/operators.jl:39-81``
+(``FluxDifferencingOperators``) and ``src/Solvers/mass_matrix.jl:19-24``
+(default weight-adjusted mass solver).
+
+The reference dispatches the residual on ``Solver``'s parallelism type
+(``Serial``/``Threaded``). Here the parallelism is ``MI355X()``: the residual
+runs in libsse_hip.so on the GPU, reached through its C-ABI
+(``include/sse_hip.h``). There is no CPU residual in this package.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Any, Optional, Tuple
+
+import numpy as np
+
+from . import abi
+from .conservation_laws import (BR1, CentralNumericalFlux, ConservativeFlux,
+                                EntropyConservativeFlux, EntropyConservativeNumericalFlux,
+                                EulerEquations, LaxFriedrichsNumericalFlux,
+                                LinearAdvectionDiffusionEquation, LinearAdvectionEquation)
+from .geometry import SpatialDiscretization
+from .reference_approximation import (LINE, TET, TRI, face_reference_normals,
+                                      flux_differencing_operators, num_faces)
+
+
+# ----------------------------------------------------------------------------- forms & strategies
+class SkewSymmetricMapping:
+    pass
+
+
+class StandardMapping:
+    pass
+
+
+@dataclass
+class StandardForm:
+    mapping_form: Any = field(default_factory=SkewSymmetricMapping)
+    inviscid_numerical_flux: Any = field(default_factory=LaxFriedrichsNumericalFlux)
+    viscous_numerical_flux: Any = field(default_factory=BR1)
+
+
+@dataclass
+class FluxDifferencingForm:
+    mapping_form: Any = field(default_factory=SkewSymmetricMapping)
+    inviscid_numerical_flux: Any = field(default_factory=LaxFriedrichsNumericalFlux)
+    viscous_numerical_flux: Any = field(default_factory=BR1)
+    two_point_flux: Any = field(default_factory=EntropyConservativeFlux)
+
+
+class ReferenceOperator:
+    pass
+
+
+class PhysicalOperator:
+    pass
+
+
+class MI355X:
+    """Parallelism: the residual runs as HIP kernels on one MI355X (gfx950)."""
+
+
+@dataclass
+class WeightAdjustedSolver:
+    """WADG with M⁻¹ = I for orthonormal bases (``mass_matrix.jl:280-296``)."""
+    Minv: Optional[np.ndarray] = None
+
+
+class DiagonalSolver:
+    """W J diagonal mass matrix for nodal V = I (``mass_matrix.jl:329-336``)."""
+
+
+def default_mass_matrix_solver(sd: SpatialDiscretization):
+    """(``mass_matrix.jl:19-24, 262-269``): diagonal for nodal V = I, else WADG."""
+    if sd.reference_approximation.V is None:
+        return DiagonalSolver()
+    return WeightAdjustedSolver()
+
+
+# ----------------------------------------------------------------------------- descriptor
+_ELEM = {LINE: abi.SSE_LINE, TRI: abi.SSE_TRI, TET: abi.SSE_TET}
+
+
+def _fptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _lptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class DescArrays:
+    """Host arrays behind one ``sse_desc`` (kept alive while the descriptor is used)."""
+
+    def __init__(self, law, sd: SpatialDiscretization, form, mass_solver):
+        ref = sd.reference_approximation
+        gf = sd.geometric_factors
+        d = ref.d
+        self.law = law
+        self.sd = sd
+        self.form = form
+        if not isinstance(form, FluxDifferencingForm):
+            raise NotImplementedError("the HIP path implements FluxDifferencingForm")
+        S, S_sparse, Cmat, C_kind = flux_differencing_operators(ref)
+        f64 = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64)
+        self.V = f64(ref.V)
+        self.R = f64(ref.R)
+        self.R_selection = (None if ref.R_selection is None
+                            else np.ascontiguousarray(ref.R_selection + 1, dtype=np.int64))
+        self.W = f64(ref.W)
+        self.B = f64(ref.B)
+        # S[:, :, m] column-major  <->  C-order array (d, N_q, N_q) holding S_m transposed
+        self.S = np.ascontiguousarray(np.stack([np.asfortranarray(S[m]).T for m in range(d)]))
+        self.C = f64(Cmat)
+        self.Minv = f64(mass_solver.Minv) if isinstance(mass_solver, WeightAdjustedSolver) else None
+        self.J_q = f64(gf.J_q)
+        self.Lambda_q = f64(gf.Lambda_q)
+        self.nJf = f64(gf.nJf)
+        self.J_f = f64(gf.J_f)
+        self.nJq = f64(gf.nJq)
+        self.face_normals = f64(face_reference_normals(ref))
+        self.mapP = np.asfortranarray(sd.mesh.mapP, dtype=np.int64)
+        desc = abi.SseDesc()
+        desc.abi_version = abi.SSE_ABI_VERSION
+        desc.element = _ELEM[ref.element_type]
+        desc.d = d
+        desc.N_p, desc.N_q, desc.N_f = ref.N_p, ref.N_q, ref.N_f
+        desc.N_fac = num_faces(ref.element_type)
+        desc.N_c = law.N_c
+        desc.N_e = sd.N_e
+        if isinstance(law, EulerEquations):
+            desc.law = abi.SSE_EULER
+            desc.gamma = law.gamma
+        elif isinstance(law, LinearAdvectionDiffusionEquation):
+            desc.law = abi.SSE_LINEAR_ADVECTION_DIFFUSION
+            desc.b = law.b
+        else:
+            desc.law = abi.SSE_LINEAR_ADVECTION
+        if hasattr(law, "a"):
+            for m, am in enumerate(law.a):
+                desc.a[m] = am
+        desc.form = abi.SSE_FLUX_DIFFERENCING
+        flux = form.inviscid_numerical_flux
+        if isinstance(flux, LaxFriedrichsNumericalFlux):
+            desc.inviscid_flux = abi.SSE_LAX_FRIEDRICHS
+            desc.half_lambda = flux.half_lambda
+        elif isinstance(flux, EntropyConservativeNumericalFlux):
+            desc.inviscid_flux = abi.SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX
+        else:
+            desc.inviscid_flux = abi.SSE_CENTRAL
+        desc.two_point_flux = (abi.SSE_TWO_POINT_CONSERVATIVE
+                               if isinstance(form.two_point_flux, ConservativeFlux)
+                               else abi.SSE_TWO_POINT_EC)
+        desc.mass_solver = (abi.SSE_MASS_DIAGONAL if isinstance(mass_solver, DiagonalSolver)
+                            else abi.SSE_MASS_WEIGHT_ADJUSTED)
+        desc.S_sparse = int(S_sparse)
+        desc.C_kind = C_kind
+        desc.V, desc.R, desc.R_selection = _fptr(self.V), _fptr(self.R), _lptr(self.R_selection)
+        desc.W, desc.B, desc.S, desc.C = _fptr(self.W), _fptr(self.B), _fptr(self.S), _fptr(self.C)
+        desc.Minv = _fptr(self.Minv)
+        desc.J_q, desc.Lambda_q, desc.nJf = _fptr(self.J_q), _fptr(self.Lambda_q), _fptr(self.nJf)
+        desc.J_f, desc.nJq = _fptr(self.J_f), _fptr(self.nJq)
+        desc.face_normals = _fptr(self.face_normals)
+        desc.mapP = _lptr(self.mapP)
+        self.desc = desc
+
+    @property
+    def shape(self) -> Tuple[int, int, int]:
+        return (self.desc.N_p, self.desc.N_c, self.desc.N_e)
+
+
+# ----------------------------------------------------------------------------- Solver
+class Solver:
+    """``Solver`` (``Solvers.jl:259-272``) whose residual runs on the GPU.
+
+    Holds the descriptor arrays and, on first use, a libsse_hip handle.
+    """
+
+    def __init__(self, conservation_law, spatial_discretization: SpatialDiscretization, form,
+                 strategy=None, mass_solver=None, parallelism=None):
+        self.conservation_law = conservation_law
+        self.spatial_discretization = spatial_discretization
+        self.form = form
+        self.strategy = strategy or ReferenceOperator()
+        self.mass_solver = mass_solver or default_mass_matrix_solver(spatial_discretization)
+        self.parallelism = parallelism or MI355X()
+        self.arrays = DescArrays(conservation_law, spatial_discretization, form, self.mass_solver)
+        self._handle = None
+        self._lib = None
+
+    @property
+    def connectivity(self) -> np.ndarray:
+        return self.spatial_discretization.mesh.mapP
+
+    def size(self) -> Tuple[int, int, int]:
+        """(N_p, N_c, N_e) (``Solvers.jl:276-285``)."""
+        return self.arrays.shape
+
+    # -- device handle
+    @property
+    def handle(self):
+        if self._handle is None:
+            from .hip import check, load_library
+            self._lib = load_library()
+            h = C.c_void_p()
+            code = self._lib.sse_create(C.byref(self.arrays.desc), C.byref(h))
+            check(self._lib, None, code)
+            self._handle = h
+        return self._handle
+
+    @property
+    def lib(self):
+        self.handle
+        return self._lib
+
+    def close(self):
+        if self._handle is not None:
+            self._lib.sse_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def semi_discrete_residual(dudt, u, solver: Solver, t: float = 0.0):
+    """``semi_discrete_residual!(dudt, u, solver, t)`` (``Solvers.jl:474-514``).
+
+    ``u``/``dudt`` are either torch CUDA tensors of shape (N_e, N_c, N_p) (the
+    reference's column-major (N_p, N_c, N_e) memory) or Fortran-ordered NumPy
+    arrays of shape (N_p, N_c, N_e) (copied through host memory).
+    """
+    from .hip import check
+    lib, h = solver.lib, solver.handle
+    if isinstance(u, np.ndarray):
+        if not (u.flags.f_contiguous and dudt.flags.f_contiguous):
+            raise ValueError("host arrays must be Fortran-ordered (N_p, N_c, N_e)")
+        if u.shape != solver.size() or dudt.shape != solver.size():
+            raise ValueError(f"expected shape {solver.size()}")
+        check(lib, h, lib.sse_residual(h, u.ctypes.data, dudt.ctypes.data, float(t), abi.SSE_HOST))
+        return dudt
+    _check_device_tensor(u, solver)
+    _check_device_tensor(dudt, solver)
+    check(lib, h, lib.sse_residual(h, u.data_ptr(), dudt.data_ptr(), float(t), abi.SSE_DEVICE))
+    return dudt
+
+
+def _check_device_tensor(x, solver: Solver):
+    import torch
+    N_p, N_c, N_e = solver.size()
+    if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64
+            and x.is_contiguous() and tuple(x.shape) == (N_e, N_c, N_p)):
+        raise ValueError(f"device state must be a contiguous float64 CUDA tensor of shape {(N_e, N_c, N_p)}")
+
+
+# ----------------------------------------------------------------------------- initial condition
+def project_function(initial_data, sd: SpatialDiscretization) -> np.ndarray:
+    """(``Solvers.jl:388-418``): nodal evaluation for V = I, else per-element
+    L2 projection with the Cholesky factor of V' W J V."""
+    ref = sd.reference_approximation
+    xq = sd.mesh.xyzq
+    uq = np.asarray(initial_data(tuple(xq), 0.0))  # N_q x N_e x N_c
+    uq = np.transpose(uq, (0, 2, 1))  # N_q x N_c x N_e
+    if ref.V is None:
+        return np.asfortranarray(uq)
+    V = ref.V
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q  # N_q x N_e
+    M = np.einsum("qa,qk,qb->kab", V, WJ, V)
+    rhs = np.einsum("qa,qk,qck->kac", V, WJ, uq)
+    L = np.linalg.cholesky(M)
+    y = np.linalg.solve(L, rhs)
+    u0 = np.linalg.solve(np.transpose(L, (0, 2, 1)), y)  # N_e x N_p x N_c
+    return np.asfortranarray(np.transpose(u0, (1, 2, 0)))
+
+
+def initialize(initial_data, sd: SpatialDiscretization) -> np.ndarray:
+    return project_function(initial_data, sd)
+
+
+@dataclass
+class ODEProblem:
+    """The ``ODEProblem(semi_discrete_residual!, u0, tspan, solver)`` of ``Solvers.jl:451``."""
+    f: Any
+    u0: np.ndarray
+    tspan: Tuple[float, float]
+    p: Solver
+
+
+def semidiscretize(conservation_law, spatial_discretization, initial_data, form, tspan,
+                   strategy=None, mass_matrix_solver=None, parallelism=None) -> ODEProblem:
+    """(``Solvers.jl:429-452``)."""
+    u0 = initialize(initial_data, spatial_discretization)
+    solver = Solver(conservation_law, spatial_discretization, form, strategy,
+                    mass_matrix_solver, parallelism)
+    return ODEProblem(semi_discrete_residual, u0, tuple(tspan), solver)

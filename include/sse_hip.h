@@ -1,0 +1,169 @@
+/*
+ * sse_hip.h -- C-ABI of libsse_hip.so, the MI355X (gfx950) semi-discrete
+ * residual of StableSpectralElements.jl (the reference), as a drop-in for the
+ * reference's `semi_discrete_residual!` path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns 0 on success or a nonzero sse_status; sse_last_error() then gives a
+ * message. A handle owns one HIP stream and its device buffers; like the
+ * reference's `Solver` (src/Solvers/Solvers.jl:259-272, which owns mutable
+ * scratch) a handle is not reentrant: calls on one handle must not overlap.
+ *
+ * Array layouts are the reference's column-major (Julia) layouts, so a Julia
+ * caller passes its arrays unchanged and a NumPy caller passes Fortran-ordered
+ * arrays with the reference's shapes.
+ */
+#ifndef SSE_HIP_H
+#define SSE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSE_ABI_VERSION 1
+
+typedef enum {
+    SSE_OK = 0,
+    SSE_ERR_INVALID_ARGUMENT = 1,
+    SSE_ERR_UNSUPPORTED = 2,
+    SSE_ERR_HIP = 3,
+    SSE_ERR_NONPHYSICAL_STATE = 4, /* rho <= 0 or p <= 0 met inside a kernel */
+    SSE_ERR_OUT_OF_MEMORY = 5
+} sse_status;
+
+/* element shapes (StartUpDG Line/Tri/Tet) */
+typedef enum { SSE_LINE = 1, SSE_TRI = 2, SSE_TET = 3 } sse_element;
+
+/* conservation laws (src/ConservationLaws/ in the reference) */
+typedef enum {
+    SSE_EULER = 1,                     /* EulerEquations{d}, N_c = d + 2 */
+    SSE_LINEAR_ADVECTION = 2,          /* LinearAdvectionEquation{d}, N_c = 1 */
+    SSE_LINEAR_ADVECTION_DIFFUSION = 3 /* LinearAdvectionDiffusionEquation{d} */
+} sse_law;
+
+/* residual forms (src/Solvers/Solvers.jl:90-115) */
+typedef enum { SSE_FLUX_DIFFERENCING = 1, SSE_STANDARD = 2 } sse_form;
+
+/* inviscid numerical fluxes (src/ConservationLaws/ConservationLaws.jl:51-62) */
+typedef enum {
+    SSE_LAX_FRIEDRICHS = 1,
+    SSE_CENTRAL = 2,
+    SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX = 3
+} sse_inviscid_flux;
+
+/* two-point fluxes (src/ConservationLaws/ConservationLaws.jl:69-73) */
+typedef enum { SSE_TWO_POINT_EC = 1, SSE_TWO_POINT_CONSERVATIVE = 2 } sse_two_point_flux;
+
+/* mass-matrix solvers (src/Solvers/mass_matrix.jl:1-17) */
+typedef enum { SSE_MASS_WEIGHT_ADJUSTED = 1, SSE_MASS_DIAGONAL = 2 } sse_mass_solver;
+
+/* facet-correction operator kind (src/Solvers/operators.jl:162-221) */
+typedef enum { SSE_C_NONE = 0, SSE_C_DENSE = 1, SSE_C_SPARSE = 2 } sse_c_kind;
+
+/* where the state arrays of a call live */
+typedef enum { SSE_DEVICE = 0, SSE_HOST = 1 } sse_memory;
+
+/*
+ * Everything the reference's `Solver{..., FluxDifferencingOperators, ...}`
+ * holds for one discretisation (src/Solvers/operators.jl:39-81,
+ * src/Solvers/mass_matrix.jl:280-296, src/Solvers/Solvers.jl:311-333).
+ * All pointers are host pointers; sse_create copies what it needs.
+ */
+typedef struct sse_desc {
+    int32_t abi_version;  /* SSE_ABI_VERSION */
+    int32_t element;      /* sse_element */
+    int32_t d;            /* spatial dimension */
+    int32_t N_p;          /* solution coefficients per element */
+    int32_t N_q;          /* volume quadrature nodes */
+    int32_t N_f;          /* facet quadrature nodes (all faces) */
+    int32_t N_fac;        /* faces per element */
+    int32_t N_c;          /* conservative variables */
+    int32_t N_e;          /* elements */
+    int32_t law;          /* sse_law */
+    int32_t form;         /* sse_form */
+    int32_t inviscid_flux;   /* sse_inviscid_flux */
+    int32_t two_point_flux;  /* sse_two_point_flux */
+    int32_t mass_solver;     /* sse_mass_solver */
+    int32_t S_sparse;        /* 1: S_m stored as SparseMatrixCSC in the reference */
+    int32_t C_kind;          /* sse_c_kind */
+    double gamma;            /* Euler: ratio of specific heats */
+    double half_lambda;      /* LaxFriedrichsNumericalFlux.halfλ */
+    double a[3];             /* advection velocity */
+    double b;                /* diffusion coefficient */
+
+    /* reference-element operators, column-major */
+    const double* V;            /* N_q x N_p Vandermonde; NULL = identity (nodal) */
+    const double* R;            /* N_f x N_q extrapolation */
+    const int64_t* R_selection; /* N_f, 1-based volume node of each facet node for
+                                   diagonal-E operators (SelectionMap), else NULL */
+    const double* W;            /* N_q volume weights */
+    const double* B;            /* N_f facet weights */
+    const double* S;            /* N_q x N_q x d, S[:, :, m] = 1/2 (W D_m - D_m' W) */
+    const double* C;            /* N_q x N_f, R' B (NULL when C_kind == SSE_C_NONE) */
+    const double* Minv;         /* N_p x N_p weight-adjusted M^{-1}; NULL = identity */
+
+    /* geometric factors (GeometricFactors, SpatialDiscretizations.jl:283-289) */
+    const double* J_q;       /* N_q x N_e */
+    const double* Lambda_q;  /* N_q x d x d x N_e, J dxi_l/dx_m */
+    const double* nJf;       /* d x N_f x N_e */
+    const double* J_f;       /* N_f x N_e */
+    const double* nJq;       /* d x N_fac x N_q x N_e; NULL = derived from Lambda_q */
+    const double* face_normals; /* d x N_fac reference normals nrstJ of each face */
+    const int64_t* mapP;     /* N_f x N_e, 1-based linear index into (N_f, N_e) */
+} sse_desc;
+
+typedef struct sse_handle sse_handle;
+
+/* Kernel slots reported by sse_profile_residual. */
+#define SSE_KERNEL_NODAL_VALUES 0   /* entropy projection + facet extrapolation */
+#define SSE_KERNEL_TIME_DERIVATIVE 1 /* interface flux, Hadamard sum, facet correction,
+                                        lift, V' and mass solve */
+#define SSE_NUM_KERNELS 2
+
+/* Replaces Solver construction (src/Solvers/Solvers.jl:311-333) and
+ * WeightAdjustedSolver (src/Solvers/mass_matrix.jl:280-296): validates the
+ * descriptor, copies operators and geometry to the device, builds the kernels'
+ * sparse tables. On failure *out is NULL and sse_last_error(NULL) explains. */
+int sse_create(const sse_desc* desc, sse_handle** out);
+
+/* Replaces semi_discrete_residual!(dudt, u, solver, t)
+ * (src/Solvers/Solvers.jl:474-514). u and dudt are (N_p, N_c, N_e) column-major;
+ * `where` says whether they are device or host pointers. dudt is overwritten.
+ * The call is asynchronous on the handle's stream for SSE_DEVICE. */
+int sse_residual(sse_handle* h, const double* u, double* dudt, double t, int32_t where);
+
+/* Advances u (N_p, N_c, N_e) by nsteps steps of the 5-stage, 4th-order
+ * low-storage Runge-Kutta method of Carpenter & Kennedy (1994), the
+ * `CarpenterKennedy2N54()` integrator the reference hands its ODEProblem to
+ * (test/test_driver.jl:221-227), with the stage update fused into the residual. */
+int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t where);
+
+/* Runs the residual nrep times on device buffers and reports the mean duration
+ * of each kernel slot in milliseconds (ms_out has SSE_NUM_KERNELS entries),
+ * timed with HIP events on the handle's stream. No counterpart in the
+ * reference (which times with TimerOutputs, src/Solvers/Solvers.jl:474). */
+int sse_profile_residual(sse_handle* h, const double* u_dev, double* dudt_dev, int32_t nrep,
+                         double* ms_out);
+
+/* Uses the given hipStream_t (0 = the handle's own stream) for later calls. */
+int sse_set_stream(sse_handle* h, void* hip_stream);
+
+/* Blocks until work queued on the handle's stream has finished. */
+int sse_synchronize(sse_handle* h);
+
+/* Message of the last failure on h (or of the last failed sse_create when h is NULL). */
+const char* sse_last_error(const sse_handle* h);
+
+/* Frees the handle's device memory and stream. */
+void sse_destroy(sse_handle* h);
+
+/* ABI version the library was built with. */
+int32_t sse_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSE_HIP_H */
