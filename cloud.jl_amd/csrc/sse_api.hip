@@ -1,0 +1,436 @@
+// sse_api.hip -- the C-ABI of libsse_hip.so (declared in include/sse_hip.h).
+//
+// sse_create validates the descriptor (the reference's Solver fields), turns
+// the dense reference operators into compressed-row tables, uploads operators
+// and geometry, and allocates the phase-1 intermediates. sse_residual and
+// sse_rk_steps launch the kernels on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sse_hip.h"
+#include "sse_kernels.h"
+
+namespace sse {
+hipError_t launch_generic_dispatch(const DevParams& p, const double* u, double* dudt, const RKStage& rk,
+                                   hipStream_t s, int which);
+size_t lds_nodal_values_generic(const DevParams& p);
+size_t lds_time_derivative_generic(const DevParams& p);
+}  // namespace sse
+
+using sse::DevParams;
+using sse::RKStage;
+
+struct sse_handle {
+    DevParams P{};
+    int element = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    double* rk_tmp = nullptr;
+    double* stage_u = nullptr;
+    double* stage_dudt = nullptr;
+    std::vector<void*> allocations;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+// Carpenter & Kennedy (1994), NASA TM-109112: five-stage fourth-order 2N-storage
+// Runge-Kutta, the coefficients of OrdinaryDiffEq's CarpenterKennedy2N54.
+const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                        -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double CK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                        1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                        2277821191437.0 / 14882151754819.0};
+
+int fail(sse_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define SSE_HIP_TRY(h, expr)                                                             \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return fail((h), SSE_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+template <typename T>
+int upload(sse_handle* h, const std::vector<T>& host, const T** dev) {
+    if (host.empty()) {
+        *dev = nullptr;
+        return 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, host.size() * sizeof(T));
+    if (e != hipSuccess) return fail(h, SSE_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocations.push_back(p);
+    e = hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    *dev = static_cast<const T*>(p);
+    return 0;
+}
+
+template <typename T>
+int alloc(sse_handle* h, size_t n, T** dev) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T) + 16);
+    if (e != hipSuccess) return fail(h, SSE_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocations.push_back(p);
+    *dev = static_cast<T*>(p);
+    return 0;
+}
+
+struct Rows {
+    std::vector<int> ptr, idx;
+    std::vector<double> val;
+};
+
+// rows of a dense column-major n_rows x n_cols matrix (nonzero entries)
+Rows rows_of(const double* A, int n_rows, int n_cols) {
+    Rows r;
+    r.ptr.push_back(0);
+    for (int i = 0; i < n_rows; ++i) {
+        for (int j = 0; j < n_cols; ++j) {
+            double v = A[i + (size_t)n_rows * j];
+            if (v != 0.0) {
+                r.idx.push_back(j);
+                r.val.push_back(v);
+            }
+        }
+        r.ptr.push_back((int)r.idx.size());
+    }
+    return r;
+}
+
+// rows of the transpose
+Rows cols_of(const double* A, int n_rows, int n_cols) {
+    Rows r;
+    r.ptr.push_back(0);
+    for (int j = 0; j < n_cols; ++j) {
+        for (int i = 0; i < n_rows; ++i) {
+            double v = A[i + (size_t)n_rows * j];
+            if (v != 0.0) {
+                r.idx.push_back(i);
+                r.val.push_back(v);
+            }
+        }
+        r.ptr.push_back((int)r.idx.size());
+    }
+    return r;
+}
+
+int validate(const sse_desc* D, std::string& msg) {
+    if (!D) { msg = "descriptor is NULL"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (D->abi_version != SSE_ABI_VERSION) { msg = "ABI version mismatch"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (D->d < 1 || D->d > 3) { msg = "d must be 1, 2 or 3"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (D->N_p <= 0 || D->N_q <= 0 || D->N_f <= 0 || D->N_fac <= 0 || D->N_e <= 0 || D->N_c <= 0) {
+        msg = "sizes must be positive"; return SSE_ERR_INVALID_ARGUMENT;
+    }
+    if (D->N_f % D->N_fac != 0) { msg = "N_f must be a multiple of N_fac"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (D->form != SSE_FLUX_DIFFERENCING) { msg = "only FluxDifferencingForm is implemented"; return SSE_ERR_UNSUPPORTED; }
+    if (D->law == SSE_EULER) {
+        if (D->N_c != D->d + 2) { msg = "Euler needs N_c = d + 2"; return SSE_ERR_INVALID_ARGUMENT; }
+        if (!(D->gamma > 1.0)) { msg = "gamma must exceed 1"; return SSE_ERR_INVALID_ARGUMENT; }
+    } else if (D->law == SSE_LINEAR_ADVECTION) {
+        if (D->N_c != 1) { msg = "linear advection needs N_c = 1"; return SSE_ERR_INVALID_ARGUMENT; }
+    } else {
+        msg = "conservation law not supported by the flux-differencing kernels"; return SSE_ERR_UNSUPPORTED;
+    }
+    if (D->inviscid_flux < 1 || D->inviscid_flux > 3) { msg = "unknown inviscid flux"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (D->two_point_flux < 1 || D->two_point_flux > 2) { msg = "unknown two-point flux"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (!D->V && D->N_p != D->N_q) { msg = "nodal scheme (V = NULL) needs N_p == N_q"; return SSE_ERR_INVALID_ARGUMENT; }
+    if (D->Minv) { msg = "a non-identity weight-adjusted M^{-1} is not supported"; return SSE_ERR_UNSUPPORTED; }
+    if (D->V && D->mass_solver != SSE_MASS_WEIGHT_ADJUSTED) {
+        msg = "modal schemes use the weight-adjusted mass solver"; return SSE_ERR_UNSUPPORTED;
+    }
+    if (!D->V && D->mass_solver != SSE_MASS_DIAGONAL) {
+        msg = "nodal schemes use the diagonal mass solver"; return SSE_ERR_UNSUPPORTED;
+    }
+    if (!D->R || !D->W || !D->B || !D->S || !D->J_q || !D->Lambda_q || !D->nJf || !D->J_f || !D->face_normals ||
+        !D->mapP) {
+        msg = "missing operator or geometry array"; return SSE_ERR_INVALID_ARGUMENT;
+    }
+    if (D->C_kind != SSE_C_NONE && !D->C) { msg = "C is NULL but C_kind says present"; return SSE_ERR_INVALID_ARGUMENT; }
+    const long long nf = (long long)D->N_f * D->N_e;
+    for (long long t = 0; t < nf; ++t) {
+        if (D->mapP[t] < 1 || D->mapP[t] > nf) { msg = "mapP entry out of range"; return SSE_ERR_INVALID_ARGUMENT; }
+    }
+    if (nf > 2147483647LL) { msg = "N_f * N_e exceeds int32 range"; return SSE_ERR_UNSUPPORTED; }
+    if (D->R_selection) {
+        for (int j = 0; j < D->N_f; ++j)
+            if (D->R_selection[j] < 1 || D->R_selection[j] > D->N_q) {
+                msg = "R_selection out of range"; return SSE_ERR_INVALID_ARGUMENT;
+            }
+    }
+    for (long long t = 0; t < (long long)D->N_q * D->N_e; ++t)
+        if (!(D->J_q[t] > 0.0)) { msg = "J_q must be positive"; return SSE_ERR_INVALID_ARGUMENT; }
+    for (long long t = 0; t < nf; ++t)
+        if (!(D->J_f[t] > 0.0)) { msg = "J_f must be positive"; return SSE_ERR_INVALID_ARGUMENT; }
+    return 0;
+}
+
+// The kernels derive 0.5 nJq from Lambda_q and the face normals (the identity
+// of src/SpatialDiscretizations/mesh.jl:262-268); a supplied nJq must agree.
+int check_nJq(const sse_desc* D, std::string& msg) {
+    if (!D->nJq) return 0;
+    const int d = D->d;
+    double scale = 0.0, worst = 0.0;
+    for (long long k = 0; k < D->N_e; ++k)
+        for (int i = 0; i < D->N_q; ++i)
+            for (int f = 0; f < D->N_fac; ++f)
+                for (int n = 0; n < d; ++n) {
+                    double s = 0.0;
+                    for (int m = 0; m < d; ++m)
+                        s += D->Lambda_q[i + (size_t)D->N_q * (m + d * (n + (size_t)d * k))] * D->face_normals[m + d * f];
+                    double g = D->nJq[n + d * (f + (size_t)D->N_fac * (i + (size_t)D->N_q * k))];
+                    scale = fmax(scale, fabs(g));
+                    worst = fmax(worst, fabs(g - s));
+                }
+    if (worst > 1e-12 * fmax(scale, 1.0)) {
+        msg = "nJq is not Lambda_q contracted with the face normals";
+        return SSE_ERR_UNSUPPORTED;
+    }
+    return 0;
+}
+
+int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int which) {
+    hipError_t e = sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int check_status(sse_handle* h) {
+    int st = 0;
+    SSE_HIP_TRY(h, hipMemcpyAsync(&st, h->P.status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (st) {
+        SSE_HIP_TRY(h, hipMemsetAsync(h->P.status, 0, sizeof(int), h->stream));
+        SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return fail(h, SSE_ERR_NONPHYSICAL_STATE, "non-physical state (density or pressure <= 0) in the residual");
+    }
+    return 0;
+}
+
+size_t state_size(const sse_handle* h) { return (size_t)h->P.N_p * h->P.N_c * h->P.N_e; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t sse_abi_version(void) { return SSE_ABI_VERSION; }
+
+const char* sse_last_error(const sse_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int sse_create(const sse_desc* D, sse_handle** out) {
+    if (!out) return fail(nullptr, SSE_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    std::string msg;
+    int code = validate(D, msg);
+    if (!code) code = check_nJq(D, msg);
+    if (code) return fail(nullptr, code, msg);
+
+    sse_handle* h = new sse_handle();
+    DevParams& P = h->P;
+    const int d = D->d;
+    P.d = d; P.N_p = D->N_p; P.N_q = D->N_q; P.N_f = D->N_f; P.N_fac = D->N_fac;
+    P.npf = D->N_f / D->N_fac; P.N_c = D->N_c; P.N_e = D->N_e;
+    P.has_V = D->V != nullptr;
+    P.R_sel = D->R_selection != nullptr;
+    P.project = (D->N_c > 1) && !(!P.has_V && P.R_sel);
+    P.C_kind = D->C_kind;
+    P.mass_diag = D->mass_solver == SSE_MASS_DIAGONAL;
+    P.law.gamma = D->gamma;
+    P.law.gm1 = D->gamma - 1.0;
+    P.law.inv_gm1 = D->law == SSE_EULER ? 1.0 / (D->gamma - 1.0) : 0.0;
+    for (int m = 0; m < 3; ++m) P.law.a[m] = D->a[m];
+    P.law.half_lambda = D->half_lambda;
+    P.law.inviscid_flux = D->inviscid_flux;
+    P.law.two_point_flux = D->two_point_flux;
+    h->element = D->element;
+
+    int rc = 0;
+    auto cleanup_fail = [&](int c) {
+        g_create_error = h->err;
+        sse_destroy(h);
+        return c;
+    };
+    const int N_q = D->N_q, N_f = D->N_f, N_p = D->N_p, N_e = D->N_e;
+    // operators
+    if (P.has_V) {
+        std::vector<double> V(D->V, D->V + (size_t)N_q * N_p);
+        if ((rc = upload(h, V, &P.V))) return cleanup_fail(rc);
+    }
+    Rows R, Rt;
+    if (D->R_selection) {
+        std::vector<double> Rd((size_t)N_f * N_q, 0.0);
+        for (int j = 0; j < N_f; ++j) Rd[j + (size_t)N_f * (D->R_selection[j] - 1)] = 1.0;
+        R = rows_of(Rd.data(), N_f, N_q);
+        Rt = cols_of(Rd.data(), N_f, N_q);
+    } else {
+        R = rows_of(D->R, N_f, N_q);
+        Rt = cols_of(D->R, N_f, N_q);
+    }
+    if ((rc = upload(h, R.ptr, &P.R_ptr)) || (rc = upload(h, R.idx, &P.R_idx)) || (rc = upload(h, R.val, &P.R_val)) ||
+        (rc = upload(h, Rt.ptr, &P.Rt_ptr)) || (rc = upload(h, Rt.idx, &P.Rt_idx)) ||
+        (rc = upload(h, Rt.val, &P.Rt_val)))
+        return cleanup_fail(rc);
+    // union sparsity of the S_m by rows
+    {
+        std::vector<int> ptr(1, 0), idx;
+        std::vector<double> val;
+        for (int i = 0; i < N_q; ++i) {
+            for (int j = 0; j < N_q; ++j) {
+                if (j == i) continue;
+                bool nz = false;
+                for (int m = 0; m < d; ++m) nz |= D->S[i + (size_t)N_q * (j + (size_t)N_q * m)] != 0.0;
+                if (!nz) continue;
+                idx.push_back(j);
+                for (int m = 0; m < d; ++m) val.push_back(D->S[i + (size_t)N_q * (j + (size_t)N_q * m)]);
+            }
+            ptr.push_back((int)idx.size());
+        }
+        if ((rc = upload(h, ptr, &P.S_ptr)) || (rc = upload(h, idx, &P.S_idx)) || (rc = upload(h, val, &P.S_val)))
+            return cleanup_fail(rc);
+    }
+    if (D->C_kind != SSE_C_NONE) {
+        Rows Cr = rows_of(D->C, N_q, N_f), Cc = cols_of(D->C, N_q, N_f);
+        if ((rc = upload(h, Cr.ptr, &P.Cr_ptr)) || (rc = upload(h, Cr.idx, &P.Cr_idx)) ||
+            (rc = upload(h, Cr.val, &P.Cr_val)) || (rc = upload(h, Cc.ptr, &P.Cc_ptr)) ||
+            (rc = upload(h, Cc.idx, &P.Cc_idx)) || (rc = upload(h, Cc.val, &P.Cc_val)))
+            return cleanup_fail(rc);
+    }
+    {
+        std::vector<double> W(D->W, D->W + N_q), B(D->B, D->B + N_f);
+        std::vector<double> fn(D->face_normals, D->face_normals + (size_t)d * D->N_fac);
+        std::vector<double> Jq(D->J_q, D->J_q + (size_t)N_q * N_e);
+        std::vector<double> Lam(D->Lambda_q, D->Lambda_q + (size_t)N_q * d * d * N_e);
+        std::vector<double> nJf(D->nJf, D->nJf + (size_t)d * N_f * N_e);
+        std::vector<double> Jf(D->J_f, D->J_f + (size_t)N_f * N_e);
+        std::vector<int> mapP((size_t)N_f * N_e);
+        for (size_t t = 0; t < mapP.size(); ++t) mapP[t] = (int)(D->mapP[t] - 1);
+        if ((rc = upload(h, W, &P.W)) || (rc = upload(h, B, &P.B)) || (rc = upload(h, fn, &P.fn)) ||
+            (rc = upload(h, Jq, &P.Jq)) || (rc = upload(h, Lam, &P.Lam)) || (rc = upload(h, nJf, &P.nJf)) ||
+            (rc = upload(h, Jf, &P.Jf)) || (rc = upload(h, mapP, &P.mapP)))
+            return cleanup_fail(rc);
+    }
+    if ((rc = alloc(h, (size_t)N_q * D->N_c * N_e, &P.uq)) || (rc = alloc(h, (size_t)N_f * D->N_c * N_e, &P.uf)) ||
+        (rc = alloc(h, 1, &P.status)))
+        return cleanup_fail(rc);
+    if (hipMemset(P.status, 0, sizeof(int)) != hipSuccess) {
+        h->err = "hipMemset failed";
+        return cleanup_fail(SSE_ERR_HIP);
+    }
+    if (sse::lds_time_derivative_generic(P) > 64 * 1024 || sse::lds_nodal_values_generic(P) > 64 * 1024) {
+        h->err = "element too large for the LDS-staged kernels";
+        return cleanup_fail(SSE_ERR_UNSUPPORTED);
+    }
+    if (hipStreamCreate(&h->own_stream) != hipSuccess) {
+        h->err = "hipStreamCreate failed";
+        return cleanup_fail(SSE_ERR_HIP);
+    }
+    h->stream = h->own_stream;
+    *out = h;
+    return 0;
+}
+
+int sse_set_stream(sse_handle* h, void* stream) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+    return 0;
+}
+
+int sse_synchronize(sse_handle* h) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return check_status(h);
+}
+
+int sse_residual(sse_handle* h, const double* u, double* dudt, double t, int32_t where) {
+    (void)t;  // the residual is autonomous (no source terms on this path)
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !dudt) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u or dudt is NULL");
+    RKStage rk{};
+    if (where == SSE_DEVICE) return launch(h, u, dudt, rk, 3);
+    if (where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "where must be SSE_DEVICE or SSE_HOST");
+    const size_t n = state_size(h);
+    int rc;
+    if (!h->stage_u && ((rc = alloc(h, n, &h->stage_u)) || (rc = alloc(h, n, &h->stage_dudt)))) return rc;
+    SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_u, u, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = launch(h, h->stage_u, h->stage_dudt, rk, 3))) return rc;
+    SSE_HIP_TRY(h, hipMemcpyAsync(dudt, h->stage_dudt, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return check_status(h);
+}
+
+int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t where) {
+    (void)t0;
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || nsteps < 0) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u is NULL or nsteps < 0");
+    const size_t n = state_size(h);
+    int rc;
+    if (!h->rk_tmp && (rc = alloc(h, n, &h->rk_tmp))) return rc;
+    double* ud = u;
+    if (where == SSE_HOST) {
+        if (!h->stage_u && ((rc = alloc(h, n, &h->stage_u)) || (rc = alloc(h, n, &h->stage_dudt)))) return rc;
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_u, u, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ud = h->stage_u;
+    } else if (where != SSE_DEVICE) {
+        return fail(h, SSE_ERR_INVALID_ARGUMENT, "where must be SSE_DEVICE or SSE_HOST");
+    }
+    for (int step = 0; step < nsteps; ++step) {
+        for (int s = 0; s < 5; ++s) {
+            RKStage rk{1, CK_A[s], CK_B[s], dt, h->rk_tmp, ud};
+            if ((rc = launch(h, ud, nullptr, rk, 3))) return rc;
+        }
+    }
+    if (where == SSE_HOST) {
+        SSE_HIP_TRY(h, hipMemcpyAsync(u, h->stage_u, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return check_status(h);
+    }
+    return 0;
+}
+
+int sse_profile_residual(sse_handle* h, const double* u, double* dudt, int32_t nrep, double* ms_out) {
+    if (!h || !u || !dudt || !ms_out || nrep <= 0) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad arguments");
+    hipEvent_t ev[3];
+    for (int i = 0; i < 3; ++i) SSE_HIP_TRY(h, hipEventCreate(&ev[i]));
+    double acc[SSE_NUM_KERNELS] = {0.0, 0.0};
+    RKStage rk{};
+    int rc = 0;
+    for (int r = 0; r < nrep && !rc; ++r) {
+        SSE_HIP_TRY(h, hipEventRecord(ev[0], h->stream));
+        if ((rc = launch(h, u, dudt, rk, 1))) break;
+        SSE_HIP_TRY(h, hipEventRecord(ev[1], h->stream));
+        if ((rc = launch(h, u, dudt, rk, 2))) break;
+        SSE_HIP_TRY(h, hipEventRecord(ev[2], h->stream));
+        SSE_HIP_TRY(h, hipEventSynchronize(ev[2]));
+        float a = 0.f, b = 0.f;
+        SSE_HIP_TRY(h, hipEventElapsedTime(&a, ev[0], ev[1]));
+        SSE_HIP_TRY(h, hipEventElapsedTime(&b, ev[1], ev[2]));
+        acc[0] += a;
+        acc[1] += b;
+    }
+    for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
+    if (rc) return rc;
+    for (int i = 0; i < SSE_NUM_KERNELS; ++i) ms_out[i] = acc[i] / nrep;
+    return 0;
+}
+
+void sse_destroy(sse_handle* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocations) (void)hipFree(p);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// sse_kernels.h -- device parameter blocks shared by the host API and kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sse_physics.h"
+
+namespace sse {
+
+// Everything a residual kernel reads, as device pointers. Operator tables are
+// compressed-row lists built once on the host from the dense reference
+// operators (nonzero entries only).
+struct DevParams {
+    int d, N_p, N_q, N_f, N_fac, npf, N_c, N_e;
+    int has_V;       // modal (V != identity)
+    int R_sel;       // diagonal-E selection extrapolation
+    int project;     // entropy projection performed in nodal_values
+    int C_kind;      // 0 none, 1/2 facet correction present
+    int mass_diag;   // DiagonalSolver (W J)^{-1}
+    LawParams law;
+    const double* V;  // N_q x N_p column-major
+    // R rows: facet node j -> (volume node, value)
+    const int* R_ptr; const int* R_idx; const double* R_val;
+    // R^T rows: volume node i -> (facet node, value)
+    const int* Rt_ptr; const int* Rt_idx; const double* Rt_val;
+    // union sparsity of S_1..S_d by rows: volume node i -> (node j != i, S_m[i,j] for m < d)
+    const int* S_ptr; const int* S_idx; const double* S_val;
+    // C rows: volume node i -> (facet node j, C_ij); C columns: facet node j -> (volume node i, C_ij)
+    const int* Cr_ptr; const int* Cr_idx; const double* Cr_val;
+    const int* Cc_ptr; const int* Cc_idx; const double* Cc_val;
+    const double* W;   // N_q
+    const double* B;   // N_f
+    const double* fn;  // d x N_fac reference face normals
+    // geometry (reference layouts)
+    const double* Jq;   // N_q x N_e
+    const double* Lam;  // N_q x d x d x N_e
+    const double* nJf;  // d x N_f x N_e
+    const double* Jf;   // N_f x N_e
+    const int* mapP;    // N_f x N_e, 0-based linear index into (N_f, N_e)
+    // phase-1 outputs: u_q [k][e][i], u_f [k][e][j]
+    double* uq;
+    double* uf;
+    int* status;
+};
+
+// Low-storage RK stage fused into the time-derivative epilogue.
+struct RKStage {
+    int active;     // 0: write dudt
+    double A, B, dt;
+    double* tmp;    // low-storage register, (N_p, N_c, N_e)
+    double* u;      // state updated in place
+};
+
+}  // namespace sse

@@ -1,0 +1,61 @@
+"""Fixed-step time integration of the semi-discrete system on the GPU.
+
+The reference hands its ``ODEProblem`` to OrdinaryDiffEq and integrates with
+``CarpenterKennedy2N54()`` at fixed dt (``test/test_driver.jl:221-227``). Here
+``solve`` advances the state with libsse_hip's ``sse_rk_steps``, which fuses
+each stage's low-storage update into the residual kernel.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .solvers import ODEProblem, Solver
+
+
+class CarpenterKennedy2N54:
+    """Five-stage, fourth-order, 2N-storage RK of Carpenter & Kennedy (1994)."""
+
+
+def to_device(u: np.ndarray):
+    """(N_p, N_c, N_e) Fortran-ordered host array -> contiguous CUDA tensor (N_e, N_c, N_p)."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asfortranarray(u).T)).to("cuda")
+
+
+def to_host(t) -> np.ndarray:
+    """Inverse of ``to_device``."""
+    return np.asfortranarray(t.detach().cpu().numpy().T)
+
+
+def rk_steps(solver: Solver, u_dev, dt: float, nsteps: int, t0: float = 0.0):
+    """Advance the device state ``u_dev`` in place by ``nsteps`` CK2N54 steps."""
+    from .hip import check
+    from .solvers import _check_device_tensor
+    _check_device_tensor(u_dev, solver)
+    lib, h = solver.lib, solver.handle
+    check(lib, h, lib.sse_rk_steps(h, u_dev.data_ptr(), float(t0), float(dt), int(nsteps), abi.SSE_DEVICE))
+    check(lib, h, lib.sse_synchronize(h))
+    return u_dev
+
+
+@dataclass
+class ODESolution:
+    t: float
+    u: np.ndarray
+
+
+def solve(ode: ODEProblem, alg=None, dt: float = None, nsteps: int = None) -> ODESolution:
+    """``solve(ode, CarpenterKennedy2N54(), dt=dt, adaptive=false)`` from tspan[0]
+    to tspan[1] (or for ``nsteps`` steps)."""
+    if dt is None:
+        raise ValueError("fixed-step integration needs dt")
+    t0, t1 = ode.tspan
+    if nsteps is None:
+        nsteps = int(round((t1 - t0) / dt))
+    u = to_device(ode.u0)
+    rk_steps(ode.p, u, dt, nsteps, t0)
+    return ODESolution(t0 + nsteps * dt, to_host(u))

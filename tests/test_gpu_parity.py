@@ -1,0 +1,123 @@
+"""HIP residual (libsse_hip.so, through its C-ABI) against the CPU oracle.
+
+Bar: max-norm error relative to max |dudt| <= 1e-12 for one residual; <= 1e-10
+after 100 CK2N54 steps; the reference's own golden L2 errors reproduced on the
+GPU at the reference's tolerance (atol 1e-10, test/runtests.jl:12).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from cases import (density_wave_3d_config, euler_1d_gauss, euler_vortex_2d_modal, l2_error,
+                   vortex_2d_config)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-12
+RK_TOL = 1e-10
+
+
+def _gpu():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _gpu_residual(ode, u=None):
+    torch = _gpu()
+    from sse import semi_discrete_residual
+    from sse.time_integration import to_device, to_host
+    u = ode.u0 if u is None else u
+    ud = to_device(u)
+    dd = torch.empty_like(ud)
+    semi_discrete_residual(dd, ud, ode.p)
+    torch.cuda.synchronize()
+    return to_host(dd)
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _perturbed(ode, amp=1e-3, seed=20261202):
+    """Initial state plus a small seeded perturbation (exercises every mode)."""
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(ode.u0 + amp * rng.standard_normal(ode.u0.shape) * np.abs(ode.u0).max())
+
+
+CASES = {
+    "euler1d_gauss_p5": lambda: euler_1d_gauss()[4],
+    "vortex2d_modal_p3_warped": lambda: euler_vortex_2d_modal()[4],
+    "vortex2d_tri_p4_8x8": lambda: vortex_2d_config(M=8, p=4)[4],
+    "vortex2d_tri_p2": lambda: vortex_2d_config(M=4, p=2)[4],
+    "vortex2d_tri_p6": lambda: vortex_2d_config(M=3, p=6)[4],
+    "wave3d_tet_p4_curved": lambda: density_wave_3d_config(M=2, p=4)[4],
+    "wave3d_tet_p3_curved": lambda: density_wave_3d_config(M=3, p=3, ic_strength=0.2)[4],
+    "wave3d_tet_p5_curved": lambda: density_wave_3d_config(M=2, p=5, ic_strength=0.2)[4],
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_residual_matches_oracle(name):
+    ode = CASES[name]()
+    u = _perturbed(ode)
+    got = _gpu_residual(ode, u)
+    ref = O.residual(ode.p.arrays, u)
+    assert _rel(got, ref) <= RES_TOL, (name, _rel(got, ref))
+
+
+def test_residual_host_memory_path_equals_device_path():
+    from sse import semi_discrete_residual
+    ode = vortex_2d_config(M=4, p=4)[4]
+    dev = _gpu_residual(ode)
+    host = np.zeros_like(ode.u0, order="F")
+    semi_discrete_residual(host, ode.u0, ode.p)
+    assert np.array_equal(dev, host)
+
+
+def test_rk_100_steps_config1_matches_oracle():
+    from sse.time_integration import rk_steps, to_device, to_host
+    law, sd, ic, form, ode = vortex_2d_config(M=8, p=4)
+    dt = 0.01
+    ud = to_device(ode.u0)
+    rk_steps(ode.p, ud, dt, 100)
+    got = to_host(ud)
+    ref = O.rk_steps(ode.p.arrays, ode.u0, dt, 100)
+    assert _rel(got, ref) <= RK_TOL
+
+
+def test_gpu_reproduces_reference_golden_euler_1d():
+    # test/runtests.jl:89-96, whole trajectory on the GPU
+    from sse.time_integration import CarpenterKennedy2N54, solve
+    law, sd, exact, form, ode = euler_1d_gauss()
+    sol = solve(ode, CarpenterKennedy2N54(), dt=2.0 / 1000)
+    err = l2_error(sd, sol.u, exact, 2.0)
+    gold = [3.5808560177567635e-5, 5.2129828619609155e-5, 0.00012637647535378534]
+    np.testing.assert_allclose(err, gold, rtol=0, atol=1e-10)
+
+
+def test_gpu_reproduces_reference_golden_vortex_modal_tri():
+    # test/runtests.jl:111-121, whole trajectory on the GPU
+    from sse.time_integration import CarpenterKennedy2N54, solve
+    law, sd, exact, form, ode, T = euler_vortex_2d_modal()
+    sol = solve(ode, CarpenterKennedy2N54(), dt=T / 1000)
+    err = l2_error(sd, sol.u, exact, T)
+    gold = [0.015568197027072704, 0.040539693811761104, 0.04060141777050208, 0.043960971468832745]
+    np.testing.assert_allclose(err, gold, rtol=0, atol=1e-10)
+
+
+def test_nonphysical_state_is_reported():
+    from sse import semi_discrete_residual
+    from sse.hip import SseError
+    ode = vortex_2d_config(M=2, p=2)[4]
+    u = np.array(ode.u0, order="F", copy=True)
+    u[:, 3, 0] = -10.0  # negative energy -> negative pressure in element 0
+    out = np.zeros_like(u, order="F")
+    with pytest.raises(SseError) as ei:
+        semi_discrete_residual(out, u, ode.p)
+    assert ei.value.code == 4
+    # the handle stays usable
+    semi_discrete_residual(out, ode.u0, ode.p)
