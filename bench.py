@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: DOF-residuals/s of the FP64 semi-discrete residual on one MI355X.
+
+Workload (one "step" = one residual evaluation over the whole mesh): 3D
+compressible Euler on curved tetrahedra -- periodic cube [0,2]^3, 16^3 hexes x 6
+tets (24,576 elements), UniformWarping factor 0.05, ModalTensor p = 4
+(N_p = 35, N_q = 125, N_f = 100, N_c = 5), isoparametric mapping with
+conservative-curl metrics, density wave rho = 1 + 0.5 sin(pi (x+y+z)),
+u = v = w = 1, p = 1, FluxDifferencingForm with the entropy-conservative
+two-point flux and Lax-Friedrichs interface dissipation. State arrays are
+resident in HBM for the timed region.
+
+Prints one JSON line (rank 0). Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "cloud.jl_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(M: int, p: int):
+    from sse.workloads import density_wave_3d_config
+    return density_wave_3d_config(M=M, p=p, factor=0.05, ic_strength=0.5)
+
+
+def cpu_baseline(arrays, u0, dof, target_s: float):
+    """The CPU oracle (restatement of the reference algorithm, OpenMP over
+    elements like the reference's Threaded()) on the same workload."""
+    from oracle import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    O.residual(arrays, u0)
+    one = time.perf_counter() - t0
+    nrep = max(1, int(target_s / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    O.residual_repeat(arrays, u0, nrep)
+    el = time.perf_counter() - t0
+    return {"value": dof * nrep / el, "unit": "DOF-residuals/s", "cores": threads, "kind": "port",
+            "sample": f"{nrep} full residual evaluations of the same 24,576-element workload "
+                      f"({el:.1f} s, {threads} OpenMP threads, C++ restatement of the reference algorithm)"}
+
+
+def load_traffic(workload: str, kernel: str):
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mesh", type=int, default=16, help="hexes per direction")
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the "
+              "multi-GPU (element-partitioned, halo-exchange) path is not built. Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from sse import abi, semi_discrete_residual
+    from sse.time_integration import to_device
+    from sse.work_model import residual_work
+
+    t0 = time.perf_counter()
+    law, sd, ic, form, ode = build_workload(args.mesh, args.p)
+    solver = ode.p
+    N_p, N_c, N_e = solver.size()
+    dof = N_p * N_c * N_e
+    log(f"prep: {time.perf_counter() - t0:.1f} s, N_e={N_e}, N_p={N_p}, N_c={N_c}, DOF={dof}")
+
+    u = to_device(ode.u0)
+    dudt = torch.empty_like(u)
+    solver.handle
+    for _ in range(args.warmup):
+        semi_discrete_residual(dudt, u, solver)
+    solver.lib.sse_synchronize(solver.handle)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        semi_discrete_residual(dudt, u, solver)
+    solver.lib.sse_synchronize(solver.handle)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = dof * args.steps / elapsed
+
+    # per-kernel durations with HIP events on the library's stream
+    import ctypes as C
+    ms = (C.c_double * abi.SSE_NUM_KERNELS)()
+    code = solver.lib.sse_profile_residual(solver.handle, u.data_ptr(), dudt.data_ptr(),
+                                           max(5, args.steps), ms)
+    if code != 0:
+        raise RuntimeError(solver.lib.sse_last_error(solver.handle))
+    kms = {abi.KERNEL_NAMES[i]: ms[i] for i in range(abi.SSE_NUM_KERNELS)}
+    work = residual_work(sd.reference_approximation, N_c)
+    dom = max(kms, key=kms.get)
+    wk = work[dom]
+    achieved_tf = wk.flops * N_e / (kms[dom] * 1e-3) / 1e12
+    achieved_gbs = wk.bytes * N_e / (kms[dom] * 1e-3) / 1e9
+    log(f"kernels (ms): {kms}; dominant {dom}: {achieved_tf:.2f} TFLOP/s "
+        f"({100 * achieved_tf / FP64_PEAK_TFLOPS:.1f}% FP64), {achieved_gbs:.0f} GB/s algorithmic "
+        f"({100 * achieved_gbs / HBM_PEAK_GBS:.1f}% HBM)")
+    workload = f"euler3d_tet_p{args.p}_curved_{args.mesh}cube"
+    traffic = load_traffic(workload, dom)
+
+    line = {
+        "metric": "DOF-residuals/s (3D Euler p=4 FP64)",
+        "value": value,
+        "unit": "DOF-residuals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D compressible Euler, curved tets, FluxDifferencingForm EC + LF, "
+                               "ModalTensor p=%d, %d^3 hexes x 6 tets, one residual per step" % (args.p, args.mesh),
+                   "N_e": N_e, "p": args.p, "N_p": N_p, "N_q": sd.reference_approximation.N_q,
+                   "N_f": sd.reference_approximation.N_f, "N_c": N_c, "DOF": dof,
+                   "kernel_ms": kms},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": dom, "flops_per_element": wk.flops,
+                     "algorithmic_bytes_per_element": wk.bytes,
+                     "hbm_frac_algorithmic": achieved_gbs / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(solver.arrays, ode.u0, dof, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+"""Discretisations of the reference's test scenarios and of the benchmark
+workloads (host-side set-up only)."""
+
+import math
+
+import numpy as np
+
+from .conservation_laws import (EntropyConservativeNumericalFlux, EulerEquations,
+                                EulerPeriodicTest, IsentropicVortex,
+                                LaxFriedrichsNumericalFlux, ShuVortex)
+from .geometry import ChanWilcoxMetrics, make_spatial_discretization
+from .mesh import ChanWarping, UniformWarping, uniform_periodic_mesh, warp_mesh
+from .reference_approximation import (LGQuadrature, ModalTensor, NodalTensor,
+                                      ReferenceApproximationLine, ReferenceApproximationTet,
+                                      ReferenceApproximationTri)
+from .solvers import FluxDifferencingForm, semidiscretize
+
+
+def euler_1d_gauss():
+    """test/euler_1d_gauss.jl:1-80 (golden L2 at test/runtests.jl:89-96)."""
+    law = EulerEquations(1, 1.4)
+
+    def exact(x, t=0.0):
+        rho = 1.0 + 0.2 * np.sin(np.pi * x[0])
+        return np.stack([rho, rho * 1.0, 1.0 / (1.4 - 1) + 0.5 * rho], -1)
+
+    ref = ReferenceApproximationLine(NodalTensor(5), volume_quadrature_rule=LGQuadrature(5))
+    mesh = uniform_periodic_mesh(ref, (0.0, 2.0), 4)
+    sd = make_spatial_discretization(mesh, ref)
+    form = FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux())
+    return law, sd, exact, form, semidiscretize(law, sd, exact, form, (0.0, 2.0))
+
+
+def euler_vortex_2d_modal(M=4, form=None):
+    """test/euler_vortex_2d_modal.jl:1-73 (golden L2 at test/runtests.jl:111-121)."""
+    g, L, Ma = 1.4, 1.0, 0.4
+    T = L / Ma
+    strength = math.sqrt(2 / (g - 1) * (1 - 0.75 ** (g - 1)))
+    law = EulerEquations(2, g)
+    exact = IsentropicVortex(g, Ma, 0.0, 0.1, strength, 1.0, (L / 2, L / 2))
+    ref = ReferenceApproximationTri(ModalTensor(3), mapping_degree=3)
+    mesh = uniform_periodic_mesh(ref, ((0.0, L), (0.0, L)), (M, M))
+    mesh = warp_mesh(mesh, ref, ChanWarping(1.0 / 16.0, (L, L)))
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    form = form or FluxDifferencingForm(inviscid_numerical_flux=LaxFriedrichsNumericalFlux())
+    return law, sd, exact, form, semidiscretize(law, sd, exact, form, (0.0, T)), T
+
+
+def vortex_2d_config(M=8, p=4, form=None):
+    """2D Euler Shu vortex (strength 5, centre (5,5), free stream rho=u=v=p=1) on an
+    affine M x M periodic grid of [0,10]^2, two triangles per square, ModalTensor(p)."""
+    law = EulerEquations(2, 1.4)
+    ic = ShuVortex(1.4, 5.0, (5.0, 5.0), (1.0, 1.0), (10.0, 10.0))
+    ref = ReferenceApproximationTri(ModalTensor(p), mapping_degree=1)
+    mesh = uniform_periodic_mesh(ref, ((0.0, 10.0), (0.0, 10.0)), (M, M))
+    sd = make_spatial_discretization(mesh, ref)
+    form = form or FluxDifferencingForm(inviscid_numerical_flux=LaxFriedrichsNumericalFlux())
+    return law, sd, ic, form, semidiscretize(law, sd, ic, form, (0.0, 1.0))
+
+
+def density_wave_3d_config(M=2, p=4, factor=0.05, form=None, ic_strength=0.5):
+    """3D Euler on curved tets: periodic [0,2]^3, M^3 hexes x 6 tets, UniformWarping
+    factor 0.05, ModalTensor(p) isoparametric, curl metrics,
+    rho = 1 + 0.5 sin(pi (x+y+z)), u=v=w=1, p=1."""
+    law = EulerEquations(3, 1.4)
+    ic = EulerPeriodicTest(3, 1.4, ic_strength, 2.0)
+    ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=p)
+    mesh = uniform_periodic_mesh(ref, ((0.0, 2.0),) * 3, (M, M, M))
+    if factor:
+        mesh = warp_mesh(mesh, ref, UniformWarping(factor, (2.0, 2.0, 2.0)))
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    form = form or FluxDifferencingForm(inviscid_numerical_flux=LaxFriedrichsNumericalFlux())
+    return law, sd, ic, form, semidiscretize(law, sd, ic, form, (0.0, 1.0))
