@@ -8,6 +8,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -20,6 +22,12 @@ hipError_t launch_generic_dispatch(const DevParams& p, const double* u, double* 
                                    hipStream_t s, int which);
 size_t lds_nodal_values_generic(const DevParams& p);
 size_t lds_time_derivative_generic(const DevParams& p);
+struct TensorConfig {
+    int TS, EPB, R, lds1, lds2, NQ, NF, NP;
+};
+bool tensor_config(int elem, int P, TensorConfig* c);
+hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
+                         int which);
 }  // namespace sse
 
 using sse::DevParams;
@@ -27,6 +35,8 @@ using sse::RKStage;
 
 struct sse_handle {
     DevParams P{};
+    sse::TensorParams TP{};
+    bool use_tensor = false;
     int element = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
@@ -201,8 +211,170 @@ int check_nJq(const sse_desc* D, std::string& msg) {
     return 0;
 }
 
+// ELL table (width = longest row) of a list of rows; -1 pads
+template <typename V>
+void ell(const std::vector<std::vector<std::pair<int, V>>>& rows, int& width, std::vector<int>& idx,
+         std::vector<V>& val) {
+    width = 0;
+    for (auto& r : rows) width = std::max(width, (int)r.size());
+    const int n = (int)rows.size();
+    idx.assign((size_t)std::max(width, 1) * n, -1);
+    val.assign((size_t)std::max(width, 1) * n, V(0));
+    for (int o = 0; o < n; ++o)
+        for (int w = 0; w < (int)rows[o].size(); ++w) {
+            idx[(size_t)w * n + o] = rows[o][w].first;
+            val[(size_t)w * n + o] = rows[o][w].second;
+        }
+}
+
+// Product of the warped-product factors, column-major N_q x N_p, for checking
+// the dense V the caller passed.
+std::vector<double> warped_dense(const sse_desc* D) {
+    const int P = D->V_p, d = D->d;
+    const int M1 = D->V_M[0], M2 = D->V_M[1], M3 = d == 3 ? D->V_M[2] : 1;
+    const int NQ = M1 * M2 * M3;
+    std::vector<double> V;
+    std::vector<std::vector<double>> cols;
+    for (int i = 0; i <= P; ++i)
+        for (int j = 0; j <= P - i; ++j) {
+            if (d == 2) {
+                std::vector<double> c(NQ);
+                for (int a1 = 0; a1 < M1; ++a1)
+                    for (int a2 = 0; a2 < M2; ++a2)
+                        c[a1 * M2 + a2] = D->V_A[a1 + M1 * i] * D->V_B[a2 + M2 * (i + (P + 1) * j)];
+                cols.push_back(c);
+            } else {
+                for (int kk = 0; kk <= P - i - j; ++kk) {
+                    std::vector<double> c(NQ);
+                    for (int a1 = 0; a1 < M1; ++a1)
+                        for (int a2 = 0; a2 < M2; ++a2)
+                            for (int a3 = 0; a3 < M3; ++a3)
+                                c[(a1 * M2 + a2) * M3 + a3] = D->V_A[a1 + M1 * i] *
+                                                              D->V_B[a2 + M2 * (i + (P + 1) * j)] *
+                                                              D->V_C[a3 + M3 * (i + (P + 1) * (j + (P + 1) * kk))];
+                    cols.push_back(c);
+                }
+            }
+        }
+    for (auto& c : cols) V.insert(V.end(), c.begin(), c.end());
+    return V;
+}
+
+// Set up the compile-time-shaped tensor kernels when the discretisation is one
+// of the compiled (element, degree) shapes; returns 0 (and leaves use_tensor
+// false) when it is not.
+int build_tensor(sse_handle* h, const sse_desc* D) {
+    const DevParams& P = h->P;
+    if (D->law != SSE_EULER || D->two_point_flux != SSE_TWO_POINT_EC) return 0;
+    if (!(D->element == SSE_TRI || D->element == SSE_TET) || !D->V || D->V_p <= 0 || !D->V_A || !D->V_B) return 0;
+    if (D->element == SSE_TET && !D->V_C) return 0;
+    if (D->C_kind == SSE_C_NONE || D->mass_solver != SSE_MASS_WEIGHT_ADJUSTED || D->Minv) return 0;
+    sse::TensorConfig cfg;
+    if (!sse::tensor_config(D->element, D->V_p, &cfg)) return 0;
+    for (int m = 0; m < D->d; ++m)
+        if (D->V_M[m] != D->V_p + 1) return 0;
+    if (cfg.NQ != D->N_q || cfg.NF != D->N_f || cfg.NP != D->N_p) return 0;
+    // the dense V must be the product of the factors
+    {
+        std::vector<double> Vw = warped_dense(D);
+        if ((int)Vw.size() != D->N_q * D->N_p) return 0;
+        double scale = 0.0, worst = 0.0;
+        for (size_t t = 0; t < Vw.size(); ++t) {
+            scale = std::max(scale, std::fabs(D->V[t]));
+            worst = std::max(worst, std::fabs(D->V[t] - Vw[t]));
+        }
+        if (worst > 1e-12 * std::max(scale, 1.0)) return 0;
+    }
+    const int d = D->d, N_q = D->N_q, N_f = D->N_f;
+    sse::TensorParams& T = h->TP;
+    T.base = P;
+    T.elem = D->element;
+    T.P = D->V_p;
+    // pair list: volume pairs (union sparsity of S, i < j), then facet pairs (nonzeros of C)
+    std::vector<int> pk;
+    std::vector<std::vector<double>> cfm(d);
+    for (int i = 0; i < N_q; ++i)
+        for (int j = i + 1; j < N_q; ++j) {
+            bool nz = false;
+            for (int m = 0; m < d; ++m) nz |= D->S[i + (size_t)N_q * (j + (size_t)N_q * m)] != 0.0;
+            if (!nz) continue;
+            pk.push_back(i | (j << 12));
+            for (int m = 0; m < d; ++m) cfm[m].push_back(D->S[i + (size_t)N_q * (j + (size_t)N_q * m)]);
+        }
+    T.n_vol_pairs = (int)pk.size();
+    for (int j = 0; j < N_f; ++j)
+        for (int i = 0; i < N_q; ++i) {
+            const double c = D->C[i + (size_t)N_q * j];
+            if (c == 0.0) continue;
+            pk.push_back(i | (j << 12) | (1 << 24));
+            cfm[0].push_back(c);
+            for (int m = 1; m < d; ++m) cfm[m].push_back(0.0);
+        }
+    T.n_pairs = (int)pk.size();
+    T.pass_size = cfg.TS * cfg.R;
+    T.n_pass = (T.n_pairs + T.pass_size - 1) / T.pass_size;
+    std::vector<double> cf;
+    for (int m = 0; m < d; ++m) cf.insert(cf.end(), cfm[m].begin(), cfm[m].end());
+    // owner reduction tables
+    std::vector<std::vector<std::vector<int>>> vin(T.n_pass, std::vector<std::vector<int>>(N_q));
+    std::vector<std::vector<std::vector<int>>> fin(T.n_pass, std::vector<std::vector<int>>(N_f));
+    for (int q = 0; q < T.n_pairs; ++q) {
+        const int pass = q / T.pass_size, slot = q % T.pass_size;
+        const int i = pk[q] & 0xfff, j = (pk[q] >> 12) & 0xfff;
+        if (q < T.n_vol_pairs) {
+            vin[pass][i].push_back(2 * slot + 1);  // r_q[i] -= c
+            vin[pass][j].push_back(2 * slot);      // r_q[j] += c
+        } else {
+            vin[pass][i].push_back(2 * slot + 1);  // r_q[i] -= c
+            fin[pass][j].push_back(2 * slot + 1);  // f_f[j] -= c
+        }
+    }
+    T.wv = T.wf = 0;
+    for (int ps = 0; ps < T.n_pass; ++ps) {
+        for (auto& v : vin[ps]) T.wv = std::max(T.wv, (int)v.size());
+        for (auto& v : fin[ps]) T.wf = std::max(T.wf, (int)v.size());
+    }
+    std::vector<int> vt((size_t)T.n_pass * std::max(T.wv, 1) * N_q, -1), ft((size_t)T.n_pass * std::max(T.wf, 1) * N_f, -1);
+    for (int ps = 0; ps < T.n_pass; ++ps) {
+        for (int o = 0; o < N_q; ++o)
+            for (int w = 0; w < (int)vin[ps][o].size(); ++w) vt[((size_t)ps * T.wv + w) * N_q + o] = vin[ps][o][w];
+        for (int o = 0; o < N_f; ++o)
+            for (int w = 0; w < (int)fin[ps][o].size(); ++w) ft[((size_t)ps * T.wf + w) * N_f + o] = fin[ps][o][w];
+    }
+    // R and R' as ELL tables
+    std::vector<std::vector<std::pair<int, double>>> rrows(N_f), rtrows(N_q);
+    for (int j = 0; j < N_f; ++j)
+        for (int i = 0; i < N_q; ++i) {
+            const double v = D->R[j + (size_t)N_f * i];
+            if (v == 0.0) continue;
+            rrows[j].push_back({i, v});
+            rtrows[i].push_back({j, v});
+        }
+    std::vector<int> r_idx, rt_idx;
+    std::vector<double> r_val, rt_val;
+    ell(rrows, T.wr, r_idx, r_val);
+    ell(rtrows, T.wrt, rt_idx, rt_val);
+    const int P1 = D->V_p + 1;
+    std::vector<double> VA(D->V_A, D->V_A + (size_t)D->V_M[0] * P1);
+    std::vector<double> VB(D->V_B, D->V_B + (size_t)D->V_M[1] * P1 * P1);
+    std::vector<double> VC;
+    if (d == 3) VC.assign(D->V_C, D->V_C + (size_t)D->V_M[2] * P1 * P1 * P1);
+    int rc;
+    if ((rc = upload(h, pk, &T.pk)) || (rc = upload(h, cf, &T.cf)) || (rc = upload(h, vt, &T.vt)) ||
+        (rc = upload(h, ft, &T.ft)) || (rc = upload(h, r_idx, &T.r_idx)) || (rc = upload(h, r_val, &T.r_val)) ||
+        (rc = upload(h, rt_idx, &T.rt_idx)) || (rc = upload(h, rt_val, &T.rt_val)) || (rc = upload(h, VA, &T.VA)) ||
+        (rc = upload(h, VB, &T.VB)))
+        return rc;
+    if (d == 3 && (rc = upload(h, VC, &T.VC))) return rc;
+    T.xcd_remap = 1;
+    if (cfg.lds1 > 64 * 1024 || cfg.lds2 > 64 * 1024) return 0;
+    h->use_tensor = true;
+    return 0;
+}
+
 int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int which) {
-    hipError_t e = sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
+    hipError_t e = h->use_tensor ? sse::launch_tensor(h->TP, u, dudt, rk, h->stream, which)
+                                 : sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -331,6 +503,10 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     if (sse::lds_time_derivative_generic(P) > 64 * 1024 || sse::lds_nodal_values_generic(P) > 64 * 1024) {
         h->err = "element too large for the LDS-staged kernels";
         return cleanup_fail(SSE_ERR_UNSUPPORTED);
+    }
+    if ((rc = build_tensor(h, D))) return cleanup_fail(rc);
+    if (const char* env = getenv("SSE_FORCE_GENERIC")) {
+        if (env[0] == '1') h->use_tensor = false;
     }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) {
         h->err = "hipStreamCreate failed";

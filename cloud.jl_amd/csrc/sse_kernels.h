@@ -43,6 +43,29 @@ struct DevParams {
     int* status;
 };
 
+// Tables of the compile-time-shaped tensor-product kernels (Tri/Tet,
+// ModalTensor, Euler). Built once on the host from the reference operators.
+struct TensorParams {
+    DevParams base;
+    int elem;        // 2 = Tri, 3 = Tet
+    int P;           // polynomial degree
+    int n_pairs;     // volume pairs (union sparsity of S, i < j) followed by facet pairs (nonzeros of C)
+    int n_vol_pairs;
+    int n_pass, pass_size, wv, wf, wrt, wr;
+    const int* pk;       // n_pairs: i | j << 12 | kind << 24 (kind 0 volume pair, 1 facet pair)
+    const double* cf;    // d x n_pairs: S_m[i,j] (volume) or C_ij in row 0 (facet)
+    const int* vt;       // n_pass x wv x N_q: slot * 2 + (1 if subtracted), -1 = none
+    const int* ft;       // n_pass x wf x N_f
+    const int* rt_idx;   // wrt x N_q: R' rows (facet node), -1 = none
+    const double* rt_val;
+    const int* r_idx;    // wr x N_f: R rows (volume node), -1 = none
+    const double* r_val;
+    const double* VA;    // warped-product factors (column-major, as in the reference)
+    const double* VB;
+    const double* VC;
+    int xcd_remap;       // 1: spread contiguous element ranges over the 8 XCDs
+};
+
 // Low-storage RK stage fused into the time-derivative epilogue.
 struct RKStage {
     int active;     // 0: write dudt

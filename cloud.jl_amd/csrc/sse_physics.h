@@ -155,6 +155,98 @@ struct Physics<D, LAW_EULER> {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Hoisted form of the entropy-conservative flux (Euler).
+//
+// Per node, once: h = [rho, v_0..v_{D-1}, p, log rho, beta = rho/p, log beta]
+// (NH = D + 5 values). A pair then needs no logarithm: log(y/x) of the
+// reference's logmean (ConservationLaws.jl:143) becomes log y - log x, which is
+// only evaluated where |(y-x)/(y+x)| >= 1e-2, so its relative error stays
+// below 1e-14. The series/log branch is taken branch-free (one divide each).
+//
+// ec_dot returns c_e = sum_n a_n F(uL,uR)[e,n] for a contraction vector a,
+// without forming the (D+2) x D flux tensor:
+//   c_0     = rho_avg (a . v_avg)
+//   c_{m+1} = v_avg_m c_0 + a_m p_avg
+//   c_{D+1} = C c_0 + (p_L (a . v_R) + p_R (a . v_L)) / 2
+// which is the tensor of euler_navierstokes.jl:377-401 contracted with a.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void hoist(const LawParams& P, const double* u, double* h) {
+    const double rho = u[0];
+    const double irho = 1.0 / rho;
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        const double v = u[m + 1] * irho;
+        h[1 + m] = v;
+        s += v * v;
+    }
+    const double p = P.gm1 * (u[D + 1] - 0.5 * rho * s);
+    const double beta = rho / p;
+    h[0] = rho;
+    h[D + 1] = p;
+    h[D + 2] = log(rho);
+    h[D + 3] = beta;
+    h[D + 4] = log(beta);
+}
+
+// logmean(x, y) from hoisted logs, branch-free selection, one divide after f^2
+__device__ __forceinline__ double logmean_h(double x, double y, double lx, double ly) {
+    const double t = (x - y) / (x + y);
+    const double f2 = t * t;
+    const bool ser = f2 < 1.0e-4;
+    const double num = ser ? (x + y) * 105.0 : (y - x);
+    const double den = ser ? (210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0))) : (ly - lx);
+    return num / den;
+}
+
+__device__ __forceinline__ double inv_logmean_h(double x, double y, double lx, double ly) {
+    const double t = (x - y) / (x + y);
+    const double f2 = t * t;
+    const bool ser = f2 < 1.0e-4;
+    const double num = ser ? (210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0))) : (ly - lx);
+    const double den = ser ? (x + y) * 105.0 : (y - x);
+    return num / den;
+}
+
+template <int D>
+__device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                       double* c) {
+    const double rho_avg = logmean_h(hL[0], hR[0], hL[D + 2], hR[D + 2]);
+    const double ilm = inv_logmean_h(hL[D + 3], hR[D + 3], hL[D + 4], hR[D + 4]);
+    double vv = 0.0, avL = 0.0, avR = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        vv += hL[1 + m] * hR[1 + m];
+        avL += a[m] * hL[1 + m];
+        avR += a[m] * hR[1 + m];
+    }
+    const double pL = hL[D + 1], pR = hR[D + 1];
+    const double p_avg = 0.5 * (pL + pR);
+    const double Cc = 0.5 * vv + P.inv_gm1 * ilm;
+    const double c0 = rho_avg * (0.5 * (avL + avR));
+    c[0] = c0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) c[1 + m] = c0 * (0.5 * (hL[1 + m] + hR[1 + m])) + a[m] * p_avg;
+    c[D + 1] = c0 * Cc + 0.5 * (pL * avR + pR * avL);
+}
+
+// wave_speed from hoisted states (euler_navierstokes.jl:339-356)
+template <int D>
+__device__ __forceinline__ double wave_speed_h(const LawParams& P, const double* hi, const double* ho,
+                                               const double* n) {
+    double vni = 0.0, vno = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        vni += hi[1 + m] * n[m];
+        vno += ho[1 + m] * n[m];
+    }
+    const double c_in = sqrt(P.gamma * hi[D + 1] / hi[0]);
+    const double c_out = sqrt(P.gamma * ho[D + 1] / ho[0]);
+    return fmax(fabs(vni), fabs(vno)) + fmax(c_in, c_out);
+}
+
 template <int D>
 struct Physics<D, LAW_ADVECTION> {
     static constexpr int NC = 1;

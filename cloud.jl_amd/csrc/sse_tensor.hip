@@ -1,0 +1,647 @@
+// sse_tensor.hip -- residual kernels for ModalTensor triangles and tetrahedra
+// with compile-time shapes (element type and degree P), for the Euler equations
+// in FluxDifferencingForm with the entropy-conservative two-point flux.
+//
+// Same algorithm as the reference (src/Solvers/flux_differencing_form.jl), with
+// the work organised for CDNA4:
+//   * V and V' are applied sum-factorised (the WarpedTensorProductMap2D/3D
+//     algorithm of src/MatrixFreeOperators/warped_product_{2d,3d}.jl), one
+//     thread per output point, all variables at once, stages staged in LDS;
+//   * the Hadamard sum and the facet correction are one list of unique pairs:
+//     each volume pair (i < j) of the union sparsity of S_1..S_d and each
+//     nonzero C_ij is evaluated exactly once per element; contributions go to
+//     an LDS staging area and are summed into the owning node in a fixed,
+//     host-built order (deterministic, no atomics);
+//   * per-node quantities (velocity, pressure, log rho, rho/p, log(rho/p)) are
+//     computed once, so a pair costs four divides and no logarithm;
+//   * the flux tensor is never formed: it is contracted with the metric vector
+//     inside the flux (ec_dot).
+// A "team" of TS threads owns one element; a block holds EPB teams.
+#include <hip/hip_runtime.h>
+
+#include "sse_kernels.h"
+
+namespace sse {
+
+enum { ELEM_TRI = 2, ELEM_TET = 3 };
+
+template <int ELEM, int P>
+struct Shape;
+
+template <int P>
+struct Shape<ELEM_TRI, P> {
+    static constexpr int D = 2, N1 = P + 1, NQ = N1 * N1, NFAC = 3, NPF = N1, NF = NFAC * NPF;
+    static constexpr int NP = (P + 1) * (P + 2) / 2, NC = 4, NH = D + 5;
+    static constexpr int NZ = (P + 1) * N1;  // Z[b1][a2]
+    static constexpr int NW = 0;
+};
+
+template <int P>
+struct Shape<ELEM_TET, P> {
+    static constexpr int D = 3, N1 = P + 1, NQ = N1 * N1 * N1, NFAC = 4, NPF = N1 * N1, NF = NFAC * NPF;
+    static constexpr int NP = (P + 1) * (P + 2) * (P + 3) / 6, NC = 5, NH = D + 5;
+    static constexpr int NPAIR12 = (P + 1) * (P + 2) / 2;
+    static constexpr int NZ = NPAIR12 * N1;  // Z[q(b1,b2)][a3]
+    static constexpr int NW = (P + 1) * N1 * N1;  // W[b1][a2][a3]
+};
+
+// team size, elements per block, pairs per thread per pass
+template <int ELEM, int P>
+struct Tune;
+template <> struct Tune<ELEM_TRI, 2> { static constexpr int TS = 64, EPB = 2, R = 1; };
+template <> struct Tune<ELEM_TRI, 3> { static constexpr int TS = 64, EPB = 2, R = 2; };
+template <> struct Tune<ELEM_TRI, 4> { static constexpr int TS = 64, EPB = 2, R = 3; };
+template <> struct Tune<ELEM_TRI, 5> { static constexpr int TS = 64, EPB = 2, R = 5; };
+template <> struct Tune<ELEM_TRI, 6> { static constexpr int TS = 64, EPB = 2, R = 7; };
+template <> struct Tune<ELEM_TET, 2> { static constexpr int TS = 64, EPB = 2, R = 3; };
+template <> struct Tune<ELEM_TET, 3> { static constexpr int TS = 128, EPB = 1, R = 3; };
+template <> struct Tune<ELEM_TET, 4> { static constexpr int TS = 256, EPB = 1, R = 2; };
+template <> struct Tune<ELEM_TET, 5> { static constexpr int TS = 256, EPB = 1, R = 2; };
+
+// q index of (b1, b2), b1 + b2 <= P, enumerated with b1 outer: also the mode
+// number of (b1, b2) in 2D.
+template <int P>
+__device__ __forceinline__ constexpr int q_index(int b1, int b2) {
+    return b1 * (P + 1) - (b1 * (b1 - 1)) / 2 + b2;
+}
+
+// number of 3D modes before (b1, b2, 0)
+template <int P>
+__device__ __forceinline__ constexpr int mode_offset3(int b1, int b2) {
+    int off = 0;
+    for (int i = 0; i < b1; ++i) off += (P - i + 1) * (P - i + 2) / 2;
+    for (int j = 0; j < b2; ++j) off += P - b1 - j + 1;
+    return off;
+}
+
+__device__ __forceinline__ int element_of_block(int b, int nb, int remap) {
+    if (!remap || (nb & 7)) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// ----------------------------------------------------------------------------- sum factorisation
+// All arrays are component-major LDS arrays: x[e * stride + point].
+template <int ELEM, int P, int TS>
+struct SumFact;
+
+template <int P, int TS>
+struct SumFact<ELEM_TRI, P, TS> {
+    using Sh = Shape<ELEM_TRI, P>;
+    static constexpr int N1 = Sh::N1, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ;
+
+    // y[a1,a2] = sum_b1 A[a1,b1] sum_{b2 <= P-b1} B[a2,b1,b2] x[q(b1,b2)]
+    __device__ static void V(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double*) {
+        for (int o = tl; o < NZ; o += TS) {
+            const int b1 = o / N1, a2 = o - b1 * N1;
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+            const int off = q_index<P>(b1, 0);
+            for (int b2 = 0; b2 <= P - b1; ++b2) {
+                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += bv * x[e * NP + off + b2];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
+        }
+        __syncthreads();
+        for (int o = tl; o < NQ; o += TS) {
+            const int a1 = o / N1, a2 = o - a1 * N1;
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+            for (int b1 = 0; b1 <= P; ++b1) {
+                const double av = tp.VA[a1 + N1 * b1];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += av * Z[e * NZ + b1 * N1 + a2];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) y[e * NQ + o] = s[e];
+        }
+        __syncthreads();
+    }
+
+    // y[q(b1,b2)] = sum_a2 B[a2,b1,b2] sum_a1 A[a1,b1] x[a1,a2]
+    __device__ static void Vt(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double*) {
+        for (int o = tl; o < NZ; o += TS) {
+            const int b1 = o / N1, a2 = o - b1 * N1;
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+            for (int a1 = 0; a1 < N1; ++a1) {
+                const double av = tp.VA[a1 + N1 * b1];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += av * x[e * NQ + a1 * N1 + a2];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
+        }
+        __syncthreads();
+        for (int m = tl; m < NP; m += TS) {
+            int b1 = 0;
+            while (q_index<P>(b1 + 1, 0) <= m && b1 < P) ++b1;
+            const int b2 = m - q_index<P>(b1, 0);
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+            for (int a2 = 0; a2 < N1; ++a2) {
+                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += bv * Z[e * NZ + b1 * N1 + a2];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) y[e * NP + m] = s[e];
+        }
+        __syncthreads();
+    }
+};
+
+template <int P, int TS>
+struct SumFact<ELEM_TET, P, TS> {
+    using Sh = Shape<ELEM_TET, P>;
+    static constexpr int N1 = Sh::N1, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ, NW = Sh::NW;
+    static constexpr int NN = N1 * N1;
+
+    __device__ static void decode_q(int q, int& b1, int& b2) {
+        b1 = 0;
+        while (b1 < P && q_index<P>(b1 + 1, 0) <= q) ++b1;
+        b2 = q - q_index<P>(b1, 0);
+    }
+
+    // y[a1,a2,a3] = sum A[a1,b1] B[a2,b1,b2] C[a3,b1,b2,b3] x[sigma(b1,b2,b3)]
+    __device__ static void V(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double* Wt) {
+        for (int o = tl; o < NZ; o += TS) {
+            const int q = o / N1, a3 = o - q * N1;
+            int b1, b2;
+            decode_q(q, b1, b2);
+            const int off = mode_offset3<P>(b1, b2);
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+            for (int b3 = 0; b3 <= P - b1 - b2; ++b3) {
+                const double cv = tp.VC[a3 + N1 * (b1 + (P + 1) * (b2 + (P + 1) * b3))];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += cv * x[e * NP + off + b3];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
+        }
+        __syncthreads();
+        for (int o = tl; o < NW; o += TS) {
+            const int b1 = o / NN, a23 = o - b1 * NN, a2 = a23 / N1, a3 = a23 - a2 * N1;
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+            const int q0 = q_index<P>(b1, 0);
+            for (int b2 = 0; b2 <= P - b1; ++b2) {
+                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += bv * Z[e * NZ + (q0 + b2) * N1 + a3];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) Wt[e * NW + o] = s[e];
+        }
+        __syncthreads();
+        for (int o = tl; o < NQ; o += TS) {
+            const int a1 = o / NN, a23 = o - a1 * NN;
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+            for (int b1 = 0; b1 <= P; ++b1) {
+                const double av = tp.VA[a1 + N1 * b1];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += av * Wt[e * NW + b1 * NN + a23];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) y[e * NQ + o] = s[e];
+        }
+        __syncthreads();
+    }
+
+    __device__ static void Vt(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double* Wt) {
+        for (int o = tl; o < NW; o += TS) {
+            const int b1 = o / NN, a23 = o - b1 * NN;
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+            for (int a1 = 0; a1 < N1; ++a1) {
+                const double av = tp.VA[a1 + N1 * b1];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += av * x[e * NQ + a1 * NN + a23];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) Wt[e * NW + o] = s[e];
+        }
+        __syncthreads();
+        for (int o = tl; o < NZ; o += TS) {
+            const int q = o / N1, a3 = o - q * N1;
+            int b1, b2;
+            decode_q(q, b1, b2);
+            double s[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+            for (int a2 = 0; a2 < N1; ++a2) {
+                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] += bv * Wt[e * NW + b1 * NN + a2 * N1 + a3];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
+        }
+        __syncthreads();
+        for (int q = tl; q < Sh::NPAIR12; q += TS) {
+            int b1, b2;
+            decode_q(q, b1, b2);
+            const int off = mode_offset3<P>(b1, b2);
+            for (int b3 = 0; b3 <= P - b1 - b2; ++b3) {
+                double s[NC];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s[e] = 0.0;
+#pragma unroll
+                for (int a3 = 0; a3 < N1; ++a3) {
+                    const double cv = tp.VC[a3 + N1 * (b1 + (P + 1) * (b2 + (P + 1) * b3))];
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) s[e] += cv * Z[e * NZ + q * N1 + a3];
+                }
+#pragma unroll
+                for (int e = 0; e < NC; ++e) y[e * NP + off + b3] = s[e];
+            }
+        }
+        __syncthreads();
+    }
+};
+
+// ----------------------------------------------------------------------------- nodal_values
+template <int ELEM, int P>
+struct K1Layout {
+    using Sh = Shape<ELEM, P>;
+    static constexpr int NC = Sh::NC;
+    static constexpr int off_u = 0;                         // NC x NP
+    static constexpr int off_a = off_u + NC * Sh::NP;       // NC x NQ
+    static constexpr int off_w = off_a + NC * Sh::NQ;       // NC x NP
+    static constexpr int off_z = off_w + NC * Sh::NP;       // NC x NZ
+    static constexpr int off_W = off_z + NC * Sh::NZ;       // NC x NW
+    static constexpr int size = off_W + NC * Sh::NW;
+};
+
+template <int ELEM, int P>
+__global__ void __launch_bounds__((Tune<ELEM, P>::TS * Tune<ELEM, P>::EPB))
+k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
+    using Sh = Shape<ELEM, P>;
+    using T = Tune<ELEM, P>;
+    using SF = SumFact<ELEM, P, T::TS>;
+    using L = K1Layout<ELEM, P>;
+    constexpr int D = Sh::D, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, TS = T::TS;
+    extern __shared__ double smem[];
+    const DevParams& p = tp.base;
+    const int team = threadIdx.x / TS, tl = threadIdx.x - team * TS;
+    const int nb = gridDim.x;
+    const int k = element_of_block(blockIdx.x, nb, tp.xcd_remap) * T::EPB + team;
+    const bool valid = k < p.N_e;
+    const int kk = valid ? k : 0;
+    double* sm = smem + team * L::size;
+    double* su = sm + L::off_u;
+    double* sa = sm + L::off_a;
+    double* sw = sm + L::off_w;
+    double* sZ = sm + L::off_z;
+    double* sW = sm + L::off_W;
+    using Ph = Physics<D, LAW_EULER>;
+
+    const double* uk = u + (size_t)kk * NP * NC;
+    for (int t = tl; t < NP * NC; t += TS) su[t] = uk[t];
+    __syncthreads();
+    SF::V(tp, tl, su, sa, sZ, sW);  // u_q = V u
+    const double* Jk = p.Jq + (size_t)kk * NQ;
+    bool ok = true;
+    for (int i = tl; i < NQ; i += TS) {  // w_q = WJ c2e(u_q)
+        double ub[NC], wb[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ub[e] = sa[e * NQ + i];
+        ok &= Ph::c2e(p.law, ub, wb);
+        const double wj = p.W[i] * Jk[i];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sa[e * NQ + i] = wb[e] * wj;
+    }
+    __syncthreads();
+    SF::Vt(tp, tl, sa, sw, sZ, sW);  // w = V' (WJ w_q)
+    SF::V(tp, tl, sw, sa, sZ, sW);   // WADG: J^{-1} W (V w)
+    for (int i = tl; i < NQ; i += TS) {
+        const double jw = p.W[i] / Jk[i];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sa[e * NQ + i] *= jw;
+    }
+    __syncthreads();
+    SF::Vt(tp, tl, sa, sw, sZ, sW);  // w = V' (...)
+    SF::V(tp, tl, sw, sa, sZ, sW);   // w_q = V w
+    double* uqk = p.uq + (size_t)kk * NQ * NC;
+    double* ufk = p.uf + (size_t)kk * NF * NC;
+    for (int i = tl; i < NQ; i += TS) {  // u_q = e2c(w_q)
+        double ub[NC], wb[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) wb[e] = sa[e * NQ + i];
+        ok &= Ph::e2c(p.law, wb, ub);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) uqk[e * NQ + i] = ub[e];
+        }
+    }
+    for (int j = tl; j < NF; j += TS) {  // u_f = e2c(R w_q)
+        double ub[NC], wb[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) wb[e] = 0.0;
+        for (int w = 0; w < tp.wr; ++w) {
+            const int i = tp.r_idx[w * NF + j];
+            if (i < 0) break;
+            const double rv = tp.r_val[w * NF + j];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) wb[e] += rv * sa[e * NQ + i];
+        }
+        ok &= Ph::e2c(p.law, wb, ub);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ufk[e * NF + j] = ub[e];
+        }
+    }
+    if (valid && !ok) atomicOr(p.status, 1);
+}
+
+// ----------------------------------------------------------------------------- time_derivative
+template <int ELEM, int P>
+struct K2Layout {
+    using Sh = Shape<ELEM, P>;
+    using T = Tune<ELEM, P>;
+    static constexpr int NC = Sh::NC, D = Sh::D, NH = Sh::NH;
+    static constexpr int PS = T::TS * T::R;
+    static constexpr int off_h = 0;                          // NH x NQ hoisted volume states
+    static constexpr int off_L = off_h + NH * Sh::NQ;        // D*D x NQ metric terms
+    static constexpr int off_hf = off_L + D * D * Sh::NQ;    // NH x NF hoisted facet states
+    static constexpr int off_nJ = off_hf + NH * Sh::NF;      // D x NF scaled normals
+    static constexpr int off_st = off_nJ + D * Sh::NF;       // PS x NC staging
+    static constexpr int size = off_st + PS * NC;
+    // phase-3 aliases
+    static constexpr int off_r = off_h;    // NC x NQ (r_q, then V output)
+    static constexpr int off_y = off_L;    // NC x NP
+    static constexpr int off_z = off_hf;   // NC x NZ
+    static constexpr int off_W = off_z + NC * Sh::NZ;  // NC x NW
+    static_assert(NC * Sh::NQ <= NH * Sh::NQ, "r_q alias");
+    static_assert(NC * Sh::NP <= D * D * Sh::NQ, "y alias");
+    static_assert(off_W + NC * Sh::NW <= size, "sum-factorisation temporaries alias");
+    static_assert(Sh::NF <= PS, "facet flux alias in staging");
+};
+
+template <int ELEM, int P>
+__global__ void __launch_bounds__((Tune<ELEM, P>::TS * Tune<ELEM, P>::EPB))
+k_time_derivative_tensor(TensorParams tp, double* __restrict__ dudt, RKStage rk) {
+    using Sh = Shape<ELEM, P>;
+    using T = Tune<ELEM, P>;
+    using SF = SumFact<ELEM, P, T::TS>;
+    using L = K2Layout<ELEM, P>;
+    constexpr int D = Sh::D, NC = Sh::NC, NH = Sh::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
+    constexpr int TS = T::TS, PS = L::PS;
+    static_assert(NQ <= TS && NF <= TS, "one node per thread");
+    extern __shared__ double smem[];
+    const DevParams& p = tp.base;
+    const int team = threadIdx.x / TS, tl = threadIdx.x - team * TS;
+    const int k = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * T::EPB + team;
+    const bool valid = k < p.N_e;
+    const int kk = valid ? k : 0;
+    double* sm = smem + team * L::size;
+    double* sh = sm + L::off_h;
+    double* sL = sm + L::off_L;
+    double* shf = sm + L::off_hf;
+    double* snJ = sm + L::off_nJ;
+    double* sst = sm + L::off_st;
+    using Ph = Physics<D, LAW_EULER>;
+
+    // ---- phase 0: hoist volume and facet states, interface flux
+    double r[NC], ff[NC];
+#pragma unroll
+    for (int e = 0; e < NC; ++e) r[e] = ff[e] = 0.0;
+    if (tl < NQ) {
+        const double* uqk = p.uq + (size_t)kk * NQ * NC;
+        double ub[NC], h[NH];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ub[e] = uqk[e * NQ + tl];
+        hoist<D>(p.law, ub, h);
+#pragma unroll
+        for (int v = 0; v < NH; ++v) sh[v * NQ + tl] = h[v];
+        const double* Lk = p.Lam + (size_t)kk * NQ * D * D;
+#pragma unroll
+        for (int mn = 0; mn < D * D; ++mn) sL[mn * NQ + tl] = Lk[mn * NQ + tl];
+    }
+    if (tl < NF) {
+        const int j = tl;
+        const double* ufk = p.uf + (size_t)kk * NF * NC;
+        double uin[NC], uout[NC], hin[NH], hout[NH], nrm[D], c[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) uin[e] = ufk[e * NF + j];
+        hoist<D>(p.law, uin, hin);
+#pragma unroll
+        for (int v = 0; v < NH; ++v) shf[v * NF + j] = hin[v];
+        const int g = p.mapP[j + (size_t)NF * kk];
+        const int kn = g / NF, jn = g - kn * NF;
+        const double* ufn = p.uf + (size_t)kn * NF * NC;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) uout[e] = ufn[e * NF + jn];
+        hoist<D>(p.law, uout, hout);
+        const double jf = p.Jf[j + (size_t)NF * kk];
+        const double inv_jf = 1.0 / jf;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            const double nj = p.nJf[m + D * (j + (size_t)NF * kk)];
+            snJ[m * NF + j] = nj;
+            nrm[m] = nj * inv_jf;
+        }
+        ec_dot<D>(p.law, hin, hout, nrm, c);
+        const double bj = p.B[j] * jf;
+        if (p.law.inviscid_flux == 1) {
+            const double a = p.law.half_lambda * wave_speed_h<D>(p.law, hin, hout, nrm);
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ff[e] = (c[e] + a * (uin[e] - uout[e])) * bj;
+        } else {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ff[e] = c[e] * bj;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 1: unique pairs, staged contributions, owner reduction
+    const int npairs = tp.n_pairs;
+    for (int pass = 0; pass < tp.n_pass; ++pass) {
+#pragma unroll
+        for (int rr = 0; rr < T::R; ++rr) {
+            const int slot = rr * TS + tl;
+            const int q = pass * PS + slot;
+            if (q < npairs) {
+                const int code = tp.pk[q];
+                const int i = code & 0xfff, j = (code >> 12) & 0xfff;
+                double hi[NH], hj[NH], a[D], c[NC];
+#pragma unroll
+                for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + i];
+                if (code >> 24) {  // facet pair: C_ij F(u_i, u_f_j) . (nJf_j + nJq_{f(j), i}) / 2
+                    const double cij = tp.cf[q];
+                    const int f = j / NPF;
+#pragma unroll
+                    for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+#pragma unroll
+                    for (int n = 0; n < D; ++n) {
+                        double s = snJ[n * NF + j];
+#pragma unroll
+                        for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * p.fn[l + D * f];
+                        a[n] = 0.5 * cij * s;
+                    }
+                } else {  // volume pair: a_n = sum_m S_m[i,j] (Lambda_i + Lambda_j)[m,n]
+#pragma unroll
+                    for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+                    double sv[D];
+#pragma unroll
+                    for (int m = 0; m < D; ++m) sv[m] = tp.cf[m * npairs + q];
+#pragma unroll
+                    for (int n = 0; n < D; ++n) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int m = 0; m < D; ++m)
+                            s += sv[m] * (sL[(m + D * n) * NQ + i] + sL[(m + D * n) * NQ + j]);
+                        a[n] = s;
+                    }
+                }
+                ec_dot<D>(p.law, hi, hj, a, c);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) sst[slot * NC + e] = c[e];
+            }
+        }
+        __syncthreads();
+        if (tl < NQ) {
+            for (int w = 0; w < tp.wv; ++w) {
+                const int code = tp.vt[(pass * tp.wv + w) * NQ + tl];
+                if (code < 0) break;
+                const double* src = sst + (code >> 1) * NC;
+                if (code & 1) {
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) r[e] -= src[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) r[e] += src[e];
+                }
+            }
+        }
+        if (tl < NF) {
+            for (int w = 0; w < tp.wf; ++w) {
+                const int code = tp.ft[(pass * tp.wf + w) * NF + tl];
+                if (code < 0) break;
+                const double* src = sst + (code >> 1) * NC;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) ff[e] -= src[e];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- phase 2: lift r_q -= R' f_f
+    double* sff = sst;  // NC x NF
+    if (tl < NF) {
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sff[e * NF + tl] = ff[e];
+    }
+    __syncthreads();
+    double* sr = sm + L::off_r;
+    if (tl < NQ) {
+        for (int w = 0; w < tp.wrt; ++w) {
+            const int j = tp.rt_idx[w * NQ + tl];
+            if (j < 0) break;
+            const double rv = tp.rt_val[w * NQ + tl];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) r[e] -= rv * sff[e * NF + j];
+        }
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sr[e * NQ + tl] = r[e];
+    }
+    __syncthreads();
+
+    // ---- phase 3: dudt = V' r_q, then the weight-adjusted mass solve
+    double* sy = sm + L::off_y;
+    double* sZ = sm + L::off_z;
+    double* sW = sm + L::off_W;
+    SF::Vt(tp, tl, sr, sy, sZ, sW);
+    SF::V(tp, tl, sy, sr, sZ, sW);
+    const double* Jk = p.Jq + (size_t)kk * NQ;
+    for (int i = tl; i < NQ; i += TS) {
+        const double jw = p.W[i] / Jk[i];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sr[e * NQ + i] *= jw;
+    }
+    __syncthreads();
+    SF::Vt(tp, tl, sr, sy, sZ, sW);
+    if (!valid) return;
+    const size_t base = (size_t)kk * NP * NC;
+    if (!rk.active) {
+        for (int t = tl; t < NP * NC; t += TS) dudt[base + t] = sy[t];
+    } else {
+        for (int t = tl; t < NP * NC; t += TS) {
+            const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt * sy[t];
+            rk.tmp[base + t] = kt;
+            rk.u[base + t] += rk.B * kt;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- launch
+struct TensorConfig {
+    int TS, EPB, R, lds1, lds2, NQ, NF, NP;
+};
+
+template <int ELEM, int P>
+TensorConfig tensor_config_of() {
+    using Sh = Shape<ELEM, P>;
+    using T = Tune<ELEM, P>;
+    return TensorConfig{T::TS, T::EPB, T::R, (int)(sizeof(double) * K1Layout<ELEM, P>::size * T::EPB),
+                        (int)(sizeof(double) * K2Layout<ELEM, P>::size * T::EPB), Sh::NQ, Sh::NF, Sh::NP};
+}
+
+template <int ELEM, int P>
+hipError_t launch_tensor_t(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
+                           int which) {
+    using T = Tune<ELEM, P>;
+    const TensorConfig c = tensor_config_of<ELEM, P>();
+    const int nb = (tp.base.N_e + T::EPB - 1) / T::EPB;
+    if (which & 1) {
+        hipLaunchKernelGGL((k_nodal_values_tensor<ELEM, P>), dim3(nb), dim3(T::TS * T::EPB), c.lds1, s, tp, u);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (which & 2) {
+        hipLaunchKernelGGL((k_time_derivative_tensor<ELEM, P>), dim3(nb), dim3(T::TS * T::EPB), c.lds2, s, tp, dudt,
+                           rk);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
+// returns false if (elem, P) has no compiled kernels
+bool tensor_config(int elem, int P, TensorConfig* c) {
+#define SSE_CASE(E, PP) \
+    if (elem == E && P == PP) { *c = tensor_config_of<E, PP>(); return true; }
+    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5) SSE_CASE(ELEM_TRI, 6)
+    SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4) SSE_CASE(ELEM_TET, 5)
+#undef SSE_CASE
+    return false;
+}
+
+hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
+                         int which) {
+#define SSE_CASE(E, PP) \
+    if (tp.elem == E && tp.P == PP) return launch_tensor_t<E, PP>(tp, u, dudt, rk, s, which);
+    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5) SSE_CASE(ELEM_TRI, 6)
+    SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4) SSE_CASE(ELEM_TET, 5)
+#undef SSE_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sse

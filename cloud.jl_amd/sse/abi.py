@@ -38,6 +38,7 @@ class SseDesc(C.Structure):
         ("gamma", C.c_double), ("half_lambda", C.c_double), ("a", C.c_double * 3), ("b", C.c_double),
         ("V", _dp), ("R", _dp), ("R_selection", _lp), ("W", _dp), ("B", _dp), ("S", _dp),
         ("C", _dp), ("Minv", _dp),
+        ("V_A", _dp), ("V_B", _dp), ("V_C", _dp), ("V_p", C.c_int32), ("V_M", C.c_int32 * 3),
         ("J_q", _dp), ("Lambda_q", _dp), ("nJf", _dp), ("J_f", _dp), ("nJq", _dp),
         ("face_normals", _dp), ("mapP", _lp),
     ]

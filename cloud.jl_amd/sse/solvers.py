@@ -123,6 +123,10 @@ class DescArrays:
         self.S = np.ascontiguousarray(np.stack([np.asfortranarray(S[m]).T for m in range(d)]))
         self.C = f64(Cmat)
         self.Minv = f64(mass_solver.Minv) if isinstance(mass_solver, WeightAdjustedSolver) else None
+        wp = ref.warped
+        self.V_A = f64(wp.A) if wp is not None else None
+        self.V_B = f64(wp.B) if wp is not None else None
+        self.V_C = f64(wp.C) if (wp is not None and wp.C is not None) else None
         self.J_q = f64(gf.J_q)
         self.Lambda_q = f64(gf.Lambda_q)
         self.nJf = f64(gf.nJf)
@@ -168,6 +172,11 @@ class DescArrays:
         desc.V, desc.R, desc.R_selection = _fptr(self.V), _fptr(self.R), _lptr(self.R_selection)
         desc.W, desc.B, desc.S, desc.C = _fptr(self.W), _fptr(self.B), _fptr(self.S), _fptr(self.C)
         desc.Minv = _fptr(self.Minv)
+        desc.V_A, desc.V_B, desc.V_C = _fptr(self.V_A), _fptr(self.V_B), _fptr(self.V_C)
+        if wp is not None:
+            desc.V_p = wp.p
+            for m, Mm in enumerate(wp.M):
+                desc.V_M[m] = Mm
         desc.J_q, desc.Lambda_q, desc.nJf = _fptr(self.J_q), _fptr(self.Lambda_q), _fptr(self.nJf)
         desc.J_f, desc.nJq = _fptr(self.J_f), _fptr(self.nJq)
         desc.face_normals = _fptr(self.face_normals)

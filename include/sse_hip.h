@@ -104,6 +104,17 @@ typedef struct sse_desc {
     const double* C;            /* N_q x N_f, R' B (NULL when C_kind == SSE_C_NONE) */
     const double* Minv;         /* N_p x N_p weight-adjusted M^{-1}; NULL = identity */
 
+    /* Optional sum factorisation of V: the fields of the reference's
+     * WarpedTensorProductMap2D/3D (src/MatrixFreeOperators/warped_product_2d.jl:2-23,
+     * warped_product_3d.jl:91-124, built by src/SpatialDiscretizations/tensor_simplex.jl:84-140),
+     * modes ordered by nested loops over (i, j[, k]) with i + j + k <= V_p.
+     * V_p = 0 means "not given" (V is then applied densely). */
+    const double* V_A;          /* M1 x (V_p+1) */
+    const double* V_B;          /* M2 x (V_p+1) x (V_p+1) */
+    const double* V_C;          /* M3 x (V_p+1)^3, 3D only */
+    int32_t V_p;
+    int32_t V_M[3];             /* M1, M2, M3: 1D volume quadrature sizes */
+
     /* geometric factors (GeometricFactors, SpatialDiscretizations.jl:283-289) */
     const double* J_q;       /* N_q x N_e */
     const double* Lambda_q;  /* N_q x d x d x N_e, J dxi_l/dx_m */

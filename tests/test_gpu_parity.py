@@ -121,3 +121,19 @@ def test_nonphysical_state_is_reported():
     assert ei.value.code == 4
     # the handle stays usable
     semi_discrete_residual(out, ode.u0, ode.p)
+
+
+@pytest.mark.parametrize("name", ["vortex2d_tri_p4_8x8", "wave3d_tet_p4_curved"])
+def test_tensor_kernels_match_generic_kernels(name, monkeypatch):
+    """The compile-time-shaped tensor kernels and the runtime-shaped generic
+    kernels (selected with SSE_FORCE_GENERIC=1 at handle creation) agree."""
+    ode = CASES[name]()
+    u = _perturbed(ode)
+    fast = _gpu_residual(ode, u)
+    monkeypatch.setenv("SSE_FORCE_GENERIC", "1")
+    ode2 = CASES[name]()
+    generic = _gpu_residual(ode2, u)
+    ref = O.residual(ode.p.arrays, u)
+    assert _rel(fast, ref) <= RES_TOL
+    assert _rel(generic, ref) <= RES_TOL
+    assert _rel(fast, generic) <= RES_TOL
