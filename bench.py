@@ -97,7 +97,7 @@ def main():
 
     u = to_device(ode.u0)
     dudt = torch.empty_like(u)
-    solver.handle
+    log("kernels: " + solver.describe())
     for _ in range(args.warmup):
         semi_discrete_residual(dudt, u, solver)
     solver.lib.sse_synchronize(solver.handle)

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -25,7 +26,7 @@ size_t lds_time_derivative_generic(const DevParams& p);
 struct TensorConfig {
     int TS, EPB, R, lds1, lds2, NQ, NF, NP;
 };
-bool tensor_config(int elem, int P, TensorConfig* c);
+bool tensor_config(int elem, int P, int var, TensorConfig* c);
 hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                          int which);
 }  // namespace sse
@@ -270,7 +271,12 @@ int build_tensor(sse_handle* h, const sse_desc* D) {
     if (D->element == SSE_TET && !D->V_C) return 0;
     if (D->C_kind == SSE_C_NONE || D->mass_solver != SSE_MASS_WEIGHT_ADJUSTED || D->Minv) return 0;
     sse::TensorConfig cfg;
-    if (!sse::tensor_config(D->element, D->V_p, &cfg)) return 0;
+    int variant = 0;
+    if (const char* env = getenv("SSE_TENSOR_VARIANT")) variant = atoi(env);
+    if (!sse::tensor_config(D->element, D->V_p, variant, &cfg)) {
+        variant = 0;
+        if (!sse::tensor_config(D->element, D->V_p, variant, &cfg)) return 0;
+    }
     for (int m = 0; m < D->d; ++m)
         if (D->V_M[m] != D->V_p + 1) return 0;
     if (cfg.NQ != D->N_q || cfg.NF != D->N_f || cfg.NP != D->N_p) return 0;
@@ -290,6 +296,7 @@ int build_tensor(sse_handle* h, const sse_desc* D) {
     T.base = P;
     T.elem = D->element;
     T.P = D->V_p;
+    T.variant = variant;
     // pair list: volume pairs (union sparsity of S, i < j), then facet pairs (nonzeros of C)
     std::vector<int> pk;
     std::vector<std::vector<double>> cfm(d);
@@ -302,14 +309,44 @@ int build_tensor(sse_handle* h, const sse_desc* D) {
             for (int m = 0; m < d; ++m) cfm[m].push_back(D->S[i + (size_t)N_q * (j + (size_t)N_q * m)]);
         }
     T.n_vol_pairs = (int)pk.size();
-    for (int j = 0; j < N_f; ++j)
-        for (int i = 0; i < N_q; ++i) {
-            const double c = D->C[i + (size_t)N_q * j];
-            if (c == 0.0) continue;
+    // order volume pairs by (j - i, i): lanes of a wave then walk neighbouring
+    // nodes of one line family at one distance (few LDS bank conflicts)
+    {
+        std::vector<int> ord(pk.size());
+        for (size_t t = 0; t < ord.size(); ++t) ord[t] = (int)t;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+            const int ia = pk[a] & 0xfff, ja = (pk[a] >> 12) & 0xfff, ib = pk[b] & 0xfff, jb = (pk[b] >> 12) & 0xfff;
+            if (ja - ia != jb - ib) return ja - ia < jb - ib;
+            return ia < ib;
+        });
+        std::vector<int> pk2;
+        std::vector<std::vector<double>> cf2(d);
+        for (int t : ord) {
+            pk2.push_back(pk[t]);
+            for (int m = 0; m < d; ++m) cf2[m].push_back(cfm[m][t]);
+        }
+        pk.swap(pk2);
+        cfm.swap(cf2);
+    }
+    // facet pairs ordered by (face, rank of i within its column, facet node j)
+    {
+        const int npf = N_f / D->N_fac;
+        std::vector<std::array<int, 4>> fp;  // face, rank, j, i
+        for (int j = 0; j < N_f; ++j) {
+            int rank = 0;
+            for (int i = 0; i < N_q; ++i) {
+                if (D->C[i + (size_t)N_q * j] == 0.0) continue;
+                fp.push_back({j / npf, rank++, j, i});
+            }
+        }
+        std::stable_sort(fp.begin(), fp.end());
+        for (auto& e : fp) {
+            const int j = e[2], i = e[3];
             pk.push_back(i | (j << 12) | (1 << 24));
-            cfm[0].push_back(c);
+            cfm[0].push_back(D->C[i + (size_t)N_q * j]);
             for (int m = 1; m < d; ++m) cfm[m].push_back(0.0);
         }
+    }
     T.n_pairs = (int)pk.size();
     T.pass_size = cfg.TS * cfg.R;
     T.n_pass = (T.n_pairs + T.pass_size - 1) / T.pass_size;
@@ -367,7 +404,8 @@ int build_tensor(sse_handle* h, const sse_desc* D) {
         return rc;
     if (d == 3 && (rc = upload(h, VC, &T.VC))) return rc;
     T.xcd_remap = 1;
-    if (cfg.lds1 > 64 * 1024 || cfg.lds2 > 64 * 1024) return 0;
+    if (cfg.lds1 > 160 * 1024 || cfg.lds2 > 160 * 1024) return 0;
+    if (T.wv > 16 || T.wf > 16 || T.wrt > 10) return 0;  // TENSOR_MAXW, TENSOR_MAXR
     h->use_tensor = true;
     return 0;
 }
@@ -514,6 +552,19 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     }
     h->stream = h->own_stream;
     *out = h;
+    return 0;
+}
+
+int sse_describe(const sse_handle* h, char* buf, int32_t len) {
+    if (!h || !buf || len <= 0) return SSE_ERR_INVALID_ARGUMENT;
+    const DevParams& P = h->P;
+    if (h->use_tensor) {
+        const sse::TensorParams& T = h->TP;
+        snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",
+                 T.elem, T.P, T.variant, T.n_pairs, T.n_vol_pairs, T.n_pass, T.pass_size, T.wv, T.wf, T.wrt, T.wr);
+    } else {
+        snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d", P.d, P.N_p, P.N_q, P.N_f, P.N_c, P.N_e);
+    }
     return 0;
 }
 

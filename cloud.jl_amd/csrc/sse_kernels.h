@@ -49,6 +49,7 @@ struct TensorParams {
     DevParams base;
     int elem;        // 2 = Tri, 3 = Tet
     int P;           // polynomial degree
+    int variant;     // compiled launch-shape variant
     int n_pairs;     // volume pairs (union sparsity of S, i < j) followed by facet pairs (nonzeros of C)
     int n_vol_pairs;
     int n_pass, pass_size, wv, wf, wrt, wr;

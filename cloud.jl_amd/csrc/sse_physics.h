@@ -210,11 +210,32 @@ __device__ __forceinline__ double inv_logmean_h(double x, double y, double lx, d
     return num / den;
 }
 
+// The logarithmic mean as a fraction num/den, branch-free:
+//   |y - x| < 1e-2 (x + y): the reference's series 105 s / (210 + 70 f2 + 42 f2^2 + 30 f2^3),
+//                           f2 = d^2 / s^2, with numerator and denominator scaled by s^6
+//                           (s = x + y, d = y - x), so no divide is needed for f2;
+//   otherwise:              (y - x) / (log y - log x).
+// The inverse logarithmic mean is den/num of the same fraction.
+__device__ __forceinline__ void logmean_frac(double x, double y, double lx, double ly, double& num, double& den) {
+    const double s = x + y, d = y - x;
+    const double s2 = s * s, d2 = d * d;
+    const bool ser = d2 < 1.0e-4 * s2;
+    const double s6 = s2 * s2 * s2;
+    const double poly = ((210.0 * s2 + 70.0 * d2) * s2 + 42.0 * d2 * d2) * s2 + 30.0 * d2 * d2 * d2;
+    num = ser ? 105.0 * s * s6 : d;
+    den = ser ? poly : (ly - lx);
+}
+
 template <int D>
 __device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, const double* hR, const double* a,
                                        double* c) {
-    const double rho_avg = logmean_h(hL[0], hR[0], hL[D + 2], hR[D + 2]);
-    const double ilm = inv_logmean_h(hL[D + 3], hR[D + 3], hL[D + 4], hR[D + 4]);
+    // rho_avg = n1/d1 (logmean of rho), ilm = d2/n2 (inverse logmean of rho/p): one divide
+    double n1, d1, n2, d2;
+    logmean_frac(hL[0], hR[0], hL[D + 2], hR[D + 2], n1, d1);
+    logmean_frac(hL[D + 3], hR[D + 3], hL[D + 4], hR[D + 4], n2, d2);
+    const double rcp = 1.0 / (d1 * n2);
+    const double rho_avg = n1 * n2 * rcp;
+    const double ilm = d2 * d1 * rcp;
     double vv = 0.0, avL = 0.0, avR = 0.0;
 #pragma unroll
     for (int m = 0; m < D; ++m) {

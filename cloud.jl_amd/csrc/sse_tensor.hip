@@ -25,6 +25,10 @@ namespace sse {
 
 enum { ELEM_TRI = 2, ELEM_TET = 3 };
 
+// widest owner-reduction table and R' table the tensor kernels accept (checked on the host)
+constexpr int TENSOR_MAXW = 16;
+constexpr int TENSOR_MAXR = 10;
+
 template <int ELEM, int P>
 struct Shape;
 
@@ -45,18 +49,30 @@ struct Shape<ELEM_TET, P> {
     static constexpr int NW = (P + 1) * N1 * N1;  // W[b1][a2][a3]
 };
 
-// team size, elements per block, pairs per thread per pass
-template <int ELEM, int P>
+// Launch shape of the two kernels. K2 (time_derivative): team size TS, EPB
+// elements per block, R pairs per thread per pass. K1 (nodal_values): TS1, EPB1.
+// V selects among variants of the same (element, degree) compiled for tuning.
+template <int ELEM, int P, int V>
 struct Tune;
-template <> struct Tune<ELEM_TRI, 2> { static constexpr int TS = 64, EPB = 2, R = 1; };
-template <> struct Tune<ELEM_TRI, 3> { static constexpr int TS = 64, EPB = 2, R = 2; };
-template <> struct Tune<ELEM_TRI, 4> { static constexpr int TS = 64, EPB = 2, R = 3; };
-template <> struct Tune<ELEM_TRI, 5> { static constexpr int TS = 64, EPB = 2, R = 5; };
-template <> struct Tune<ELEM_TRI, 6> { static constexpr int TS = 64, EPB = 2, R = 7; };
-template <> struct Tune<ELEM_TET, 2> { static constexpr int TS = 64, EPB = 2, R = 3; };
-template <> struct Tune<ELEM_TET, 3> { static constexpr int TS = 128, EPB = 1, R = 3; };
-template <> struct Tune<ELEM_TET, 4> { static constexpr int TS = 256, EPB = 1, R = 2; };
-template <> struct Tune<ELEM_TET, 5> { static constexpr int TS = 256, EPB = 1, R = 2; };
+#define SSE_TUNE(E, PP, VV, ts, epb, r, ts1, epb1)                                       \
+    template <>                                                                        \
+    struct Tune<E, PP, VV> {                                                           \
+        static constexpr int TS = ts, EPB = epb, R = r, TS1 = ts1, EPB1 = epb1;         \
+    };
+SSE_TUNE(ELEM_TRI, 2, 0, 64, 2, 1, 64, 2)
+SSE_TUNE(ELEM_TRI, 3, 0, 64, 2, 2, 64, 2)
+SSE_TUNE(ELEM_TRI, 4, 0, 64, 2, 3, 64, 2)
+SSE_TUNE(ELEM_TRI, 5, 0, 64, 2, 5, 64, 2)
+SSE_TUNE(ELEM_TRI, 6, 0, 64, 2, 7, 64, 2)
+SSE_TUNE(ELEM_TET, 2, 0, 64, 2, 3, 64, 2)
+SSE_TUNE(ELEM_TET, 3, 0, 128, 1, 3, 64, 2)
+SSE_TUNE(ELEM_TET, 4, 0, 128, 1, 2, 128, 1)
+SSE_TUNE(ELEM_TET, 4, 1, 128, 1, 2, 64, 1)
+SSE_TUNE(ELEM_TET, 4, 2, 128, 1, 2, 128, 2)
+SSE_TUNE(ELEM_TET, 4, 3, 128, 2, 2, 128, 4)
+SSE_TUNE(ELEM_TET, 4, 4, 128, 1, 1, 64, 2)
+SSE_TUNE(ELEM_TET, 5, 0, 256, 1, 2, 256, 1)
+#undef SSE_TUNE
 
 // q index of (b1, b2), b1 + b2 <= P, enumerated with b1 outer: also the mode
 // number of (b1, b2) in 2D.
@@ -291,19 +307,19 @@ struct K1Layout {
     static constexpr int size = off_W + NC * Sh::NW;
 };
 
-template <int ELEM, int P>
-__global__ void __launch_bounds__((Tune<ELEM, P>::TS * Tune<ELEM, P>::EPB))
+template <int ELEM, int P, int VAR>
+__global__ void __launch_bounds__((Tune<ELEM, P, VAR>::TS1 * Tune<ELEM, P, VAR>::EPB1))
 k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
     using Sh = Shape<ELEM, P>;
-    using T = Tune<ELEM, P>;
-    using SF = SumFact<ELEM, P, T::TS>;
+    using T = Tune<ELEM, P, VAR>;
+    using SF = SumFact<ELEM, P, T::TS1>;
     using L = K1Layout<ELEM, P>;
-    constexpr int D = Sh::D, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, TS = T::TS;
+    constexpr int D = Sh::D, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, TS = T::TS1;
     extern __shared__ double smem[];
     const DevParams& p = tp.base;
     const int team = threadIdx.x / TS, tl = threadIdx.x - team * TS;
     const int nb = gridDim.x;
-    const int k = element_of_block(blockIdx.x, nb, tp.xcd_remap) * T::EPB + team;
+    const int k = element_of_block(blockIdx.x, nb, tp.xcd_remap) * T::EPB1 + team;
     const bool valid = k < p.N_e;
     const int kk = valid ? k : 0;
     double* sm = smem + team * L::size;
@@ -373,10 +389,10 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
 }
 
 // ----------------------------------------------------------------------------- time_derivative
-template <int ELEM, int P>
+template <int ELEM, int P, int VAR>
 struct K2Layout {
     using Sh = Shape<ELEM, P>;
-    using T = Tune<ELEM, P>;
+    using T = Tune<ELEM, P, VAR>;
     static constexpr int NC = Sh::NC, D = Sh::D, NH = Sh::NH;
     static constexpr int PS = T::TS * T::R;
     static constexpr int off_h = 0;                          // NH x NQ hoisted volume states
@@ -396,15 +412,15 @@ struct K2Layout {
     static_assert(Sh::NF <= PS, "facet flux alias in staging");
 };
 
-template <int ELEM, int P>
-__global__ void __launch_bounds__((Tune<ELEM, P>::TS * Tune<ELEM, P>::EPB))
+template <int ELEM, int P, int VAR>
+__global__ void __launch_bounds__((Tune<ELEM, P, VAR>::TS * Tune<ELEM, P, VAR>::EPB))
 k_time_derivative_tensor(TensorParams tp, double* __restrict__ dudt, RKStage rk) {
     using Sh = Shape<ELEM, P>;
-    using T = Tune<ELEM, P>;
+    using T = Tune<ELEM, P, VAR>;
     using SF = SumFact<ELEM, P, T::TS>;
-    using L = K2Layout<ELEM, P>;
+    using L = K2Layout<ELEM, P, VAR>;
     constexpr int D = Sh::D, NC = Sh::NC, NH = Sh::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
-    constexpr int TS = T::TS, PS = L::PS;
+    constexpr int TS = T::TS, PS = L::PS, MAXW = TENSOR_MAXW;
     static_assert(NQ <= TS && NF <= TS, "one node per thread");
     extern __shared__ double smem[];
     const DevParams& p = tp.base;
@@ -514,31 +530,37 @@ k_time_derivative_tensor(TensorParams tp, double* __restrict__ dudt, RKStage rk)
                 }
                 ec_dot<D>(p.law, hi, hj, a, c);
 #pragma unroll
-                for (int e = 0; e < NC; ++e) sst[slot * NC + e] = c[e];
+                for (int e = 0; e < NC; ++e) sst[e * PS + slot] = c[e];
             }
         }
         __syncthreads();
         if (tl < NQ) {
-            for (int w = 0; w < tp.wv; ++w) {
-                const int code = tp.vt[(pass * tp.wv + w) * NQ + tl];
-                if (code < 0) break;
-                const double* src = sst + (code >> 1) * NC;
-                if (code & 1) {
+            int codes[MAXW];
 #pragma unroll
-                    for (int e = 0; e < NC; ++e) r[e] -= src[e];
-                } else {
+            for (int w = 0; w < MAXW; ++w) codes[w] = w < tp.wv ? tp.vt[(pass * tp.wv + w) * NQ + tl] : -1;
 #pragma unroll
-                    for (int e = 0; e < NC; ++e) r[e] += src[e];
+            for (int w = 0; w < MAXW; ++w) {
+                const int code = codes[w];
+                if (code >= 0) {
+                    const double sg = (code & 1) ? -1.0 : 1.0;
+                    const double* src = sst + (code >> 1);
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) r[e] += sg * src[e * PS];
                 }
             }
         }
         if (tl < NF) {
-            for (int w = 0; w < tp.wf; ++w) {
-                const int code = tp.ft[(pass * tp.wf + w) * NF + tl];
-                if (code < 0) break;
-                const double* src = sst + (code >> 1) * NC;
+            int codes[MAXW];
 #pragma unroll
-                for (int e = 0; e < NC; ++e) ff[e] -= src[e];
+            for (int w = 0; w < MAXW; ++w) codes[w] = w < tp.wf ? tp.ft[(pass * tp.wf + w) * NF + tl] : -1;
+#pragma unroll
+            for (int w = 0; w < MAXW; ++w) {
+                const int code = codes[w];
+                if (code >= 0) {
+                    const double* src = sst + (code >> 1);
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) ff[e] -= src[e * PS];
+                }
             }
         }
         __syncthreads();
@@ -553,12 +575,20 @@ k_time_derivative_tensor(TensorParams tp, double* __restrict__ dudt, RKStage rk)
     __syncthreads();
     double* sr = sm + L::off_r;
     if (tl < NQ) {
-        for (int w = 0; w < tp.wrt; ++w) {
-            const int j = tp.rt_idx[w * NQ + tl];
-            if (j < 0) break;
-            const double rv = tp.rt_val[w * NQ + tl];
+        constexpr int MAXR = TENSOR_MAXR;
+        int js[MAXR];
+        double rvs[MAXR];
 #pragma unroll
-            for (int e = 0; e < NC; ++e) r[e] -= rv * sff[e * NF + j];
+        for (int w = 0; w < MAXR; ++w) {
+            js[w] = w < tp.wrt ? tp.rt_idx[w * NQ + tl] : -1;
+            rvs[w] = w < tp.wrt ? tp.rt_val[w * NQ + tl] : 0.0;
+        }
+#pragma unroll
+        for (int w = 0; w < MAXR; ++w) {
+            if (js[w] >= 0) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) r[e] -= rvs[w] * sff[e * NF + js[w]];
+            }
         }
 #pragma unroll
         for (int e = 0; e < NC; ++e) sr[e * NQ + tl] = r[e];
@@ -597,49 +627,64 @@ struct TensorConfig {
     int TS, EPB, R, lds1, lds2, NQ, NF, NP;
 };
 
-template <int ELEM, int P>
+template <int ELEM, int P, int VAR>
 TensorConfig tensor_config_of() {
     using Sh = Shape<ELEM, P>;
-    using T = Tune<ELEM, P>;
-    return TensorConfig{T::TS, T::EPB, T::R, (int)(sizeof(double) * K1Layout<ELEM, P>::size * T::EPB),
-                        (int)(sizeof(double) * K2Layout<ELEM, P>::size * T::EPB), Sh::NQ, Sh::NF, Sh::NP};
+    using T = Tune<ELEM, P, VAR>;
+    return TensorConfig{T::TS, T::EPB, T::R, (int)(sizeof(double) * K1Layout<ELEM, P>::size * T::EPB1),
+                        (int)(sizeof(double) * K2Layout<ELEM, P, VAR>::size * T::EPB), Sh::NQ, Sh::NF, Sh::NP};
 }
 
-template <int ELEM, int P>
+template <int ELEM, int P, int VAR>
 hipError_t launch_tensor_t(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                            int which) {
-    using T = Tune<ELEM, P>;
-    const TensorConfig c = tensor_config_of<ELEM, P>();
-    const int nb = (tp.base.N_e + T::EPB - 1) / T::EPB;
+    using T = Tune<ELEM, P, VAR>;
+    const TensorConfig c = tensor_config_of<ELEM, P, VAR>();
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e1 = hipFuncSetAttribute((const void*)k_nodal_values_tensor<ELEM, P, VAR>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds1);
+        hipError_t e2 = hipFuncSetAttribute((const void*)k_time_derivative_tensor<ELEM, P, VAR>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds2);
+        if (e1 != hipSuccess) return e1;
+        if (e2 != hipSuccess) return e2;
+        attr_set = true;
+    }
     if (which & 1) {
-        hipLaunchKernelGGL((k_nodal_values_tensor<ELEM, P>), dim3(nb), dim3(T::TS * T::EPB), c.lds1, s, tp, u);
+        const int nb = (tp.base.N_e + T::EPB1 - 1) / T::EPB1;
+        hipLaunchKernelGGL((k_nodal_values_tensor<ELEM, P, VAR>), dim3(nb), dim3(T::TS1 * T::EPB1), c.lds1, s, tp, u);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (which & 2) {
-        hipLaunchKernelGGL((k_time_derivative_tensor<ELEM, P>), dim3(nb), dim3(T::TS * T::EPB), c.lds2, s, tp, dudt,
-                           rk);
+        const int nb = (tp.base.N_e + T::EPB - 1) / T::EPB;
+        hipLaunchKernelGGL((k_time_derivative_tensor<ELEM, P, VAR>), dim3(nb), dim3(T::TS * T::EPB), c.lds2, s, tp,
+                           dudt, rk);
         return hipGetLastError();
     }
     return hipSuccess;
 }
 
-// returns false if (elem, P) has no compiled kernels
-bool tensor_config(int elem, int P, TensorConfig* c) {
-#define SSE_CASE(E, PP) \
-    if (elem == E && P == PP) { *c = tensor_config_of<E, PP>(); return true; }
-    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5) SSE_CASE(ELEM_TRI, 6)
-    SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4) SSE_CASE(ELEM_TET, 5)
+#define SSE_ALL_CASES                                                                               \
+    SSE_CASE(ELEM_TRI, 2, 0) SSE_CASE(ELEM_TRI, 3, 0) SSE_CASE(ELEM_TRI, 4, 0) SSE_CASE(ELEM_TRI, 5, 0) \
+    SSE_CASE(ELEM_TRI, 6, 0) SSE_CASE(ELEM_TET, 2, 0) SSE_CASE(ELEM_TET, 3, 0) SSE_CASE(ELEM_TET, 4, 0) \
+    SSE_CASE(ELEM_TET, 4, 1) SSE_CASE(ELEM_TET, 4, 2) SSE_CASE(ELEM_TET, 4, 3) SSE_CASE(ELEM_TET, 4, 4) \
+    SSE_CASE(ELEM_TET, 5, 0)
+
+// returns false if (elem, P, variant) has no compiled kernels
+bool tensor_config(int elem, int P, int var, TensorConfig* c) {
+#define SSE_CASE(E, PP, VV) \
+    if (elem == E && P == PP && var == VV) { *c = tensor_config_of<E, PP, VV>(); return true; }
+    SSE_ALL_CASES
 #undef SSE_CASE
     return false;
 }
 
 hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                          int which) {
-#define SSE_CASE(E, PP) \
-    if (tp.elem == E && tp.P == PP) return launch_tensor_t<E, PP>(tp, u, dudt, rk, s, which);
-    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5) SSE_CASE(ELEM_TRI, 6)
-    SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4) SSE_CASE(ELEM_TET, 5)
+#define SSE_CASE(E, PP, VV) \
+    if (tp.elem == E && tp.P == PP && tp.variant == VV) return launch_tensor_t<E, PP, VV>(tp, u, dudt, rk, s, which);
+    SSE_ALL_CASES
 #undef SSE_CASE
     return hipErrorInvalidValue;
 }

@@ -47,5 +47,5 @@ class SseDesc(C.Structure):
 # Names of every entry point declared in include/sse_hip.h.
 EXPORTED_SYMBOLS = (
     "sse_create", "sse_residual", "sse_rk_steps", "sse_profile_residual", "sse_set_stream",
-    "sse_synchronize", "sse_last_error", "sse_destroy", "sse_abi_version",
+    "sse_synchronize", "sse_last_error", "sse_destroy", "sse_abi_version", "sse_describe",
 )

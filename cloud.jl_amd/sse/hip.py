@@ -30,6 +30,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     if not os.path.exists(path):
         raise HipLibraryMissing(
             f"{path} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7);
+    # loading it first makes the library and torch share one runtime instance.
+    import torch  # noqa: F401
     lib = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     vp = C.c_void_p
@@ -49,6 +52,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_last_error.restype = C.c_char_p
     lib.sse_destroy.argtypes = [vp]
     lib.sse_destroy.restype = None
+    lib.sse_describe.argtypes = [vp, C.c_char_p, C.c_int32]
+    lib.sse_describe.restype = C.c_int
     lib.sse_abi_version.argtypes = []
     lib.sse_abi_version.restype = C.c_int32
     for name in EXPORTED_SYMBOLS:

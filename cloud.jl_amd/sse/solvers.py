@@ -232,6 +232,12 @@ class Solver:
         self.handle
         return self._lib
 
+    def describe(self) -> str:
+        """Which kernels the handle runs (tensor/generic, launch variant, table sizes)."""
+        buf = C.create_string_buffer(256)
+        self.lib.sse_describe(self.handle, buf, 256)
+        return buf.value.decode()
+
     def close(self):
         if self._handle is not None:
             self._lib.sse_destroy(self._handle)

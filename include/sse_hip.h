@@ -158,6 +158,11 @@ int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps,
 int sse_profile_residual(sse_handle* h, const double* u_dev, double* dudt_dev, int32_t nrep,
                          double* ms_out);
 
+/* Writes a one-line description of the kernels a handle runs (kernel family,
+ * launch variant, pair and table sizes) into buf (NUL-terminated, at most len
+ * bytes). No counterpart in the reference; for tests and benchmarks. */
+int sse_describe(const sse_handle* h, char* buf, int32_t len);
+
 /* Uses the given hipStream_t (0 = the handle's own stream) for later calls. */
 int sse_set_stream(sse_handle* h, void* hip_stream);
 

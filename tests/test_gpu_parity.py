@@ -67,6 +67,9 @@ def test_residual_matches_oracle(name):
     got = _gpu_residual(ode, u)
     ref = O.residual(ode.p.arrays, u)
     assert _rel(got, ref) <= RES_TOL, (name, _rel(got, ref))
+    # Tri/Tet ModalTensor Euler runs the compile-time-shaped kernels
+    expected = "generic" if name.startswith("euler1d") else "tensor"
+    assert ode.p.describe().startswith(expected), ode.p.describe()
 
 
 def test_residual_host_memory_path_equals_device_path():
