@@ -29,6 +29,24 @@ struct TensorConfig {
 bool tensor_config(int elem, int P, int var, TensorConfig* c);
 hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                          int which);
+struct StructTables {
+    const double* SG;
+    const int* cr_j;
+    const double* cr_v;
+    const int* ci_i;
+    const double* ci_v;
+    const int* ci_j;
+    const int* fi;
+    int wcr, ch, n_items, wfi;
+    const double* uf_ext;
+};
+struct StructConfig {
+    int TS, EPB, lds, max_item;
+};
+bool struct_config(int elem, int P, StructConfig* c);
+int struct_occupancy(int elem, int P);
+hipError_t launch_struct(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
+                         hipStream_t s, int n_blocks);
 }  // namespace sse
 
 using sse::DevParams;
@@ -38,6 +56,9 @@ struct sse_handle {
     DevParams P{};
     sse::TensorParams TP{};
     bool use_tensor = false;
+    sse::StructTables ST{};
+    bool use_struct = false;
+    int struct_blocks = 0;
     int element = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
@@ -410,7 +431,117 @@ int build_tensor(sse_handle* h, const sse_desc* D) {
     return 0;
 }
 
+// Tables of the line-structured time_derivative kernel (requires the tensor
+// kernels to be set up). Returns 0 and leaves use_struct false when the
+// operators do not have the tensor-line structure.
+int build_struct(sse_handle* h, const sse_desc* D) {
+    if (!h->use_tensor) return 0;
+    sse::StructConfig cfg;
+    if (!sse::struct_config(D->element, D->V_p, &cfg)) return 0;
+    const int d = D->d, N_q = D->N_q, N_f = D->N_f, N1 = D->V_p + 1, N_e = D->N_e;
+    // mapP must pair facet nodes (an involution) for the exterior-state scatter
+    for (long long t = 0; t < (long long)N_f * N_e; ++t) {
+        const long long g = D->mapP[t] - 1;
+        if (D->mapP[g] - 1 != t) return 0;
+    }
+    int stride[3] = {1, 1, 1};
+    for (int f = d - 2; f >= 0; --f) stride[f] = stride[f + 1] * N1;
+    // S on tensor lines, S_m[i, j] = 0 for m < f (family f); checked against every nonzero
+    std::vector<double> SG((size_t)d * N1 * d * N_q, 0.0);
+    std::vector<char> covered((size_t)N_q * N_q, 0);
+    for (int i = 0; i < N_q; ++i) {
+        int a[3], t = i;
+        for (int f = d - 1; f >= 0; --f) {
+            a[f] = t % N1;
+            t /= N1;
+        }
+        for (int f = 0; f < d; ++f)
+            for (int b = 0; b < N1; ++b) {
+                if (b == a[f]) continue;
+                const int j = i + (b - a[f]) * stride[f];
+                covered[(size_t)i * N_q + j] = 1;
+                for (int m = 0; m < d; ++m) {
+                    const double v = D->S[i + (size_t)N_q * (j + (size_t)N_q * m)];
+                    if (m < f && v != 0.0) return 0;
+                    SG[(((size_t)f * N1 + b) * d + m) * N_q + i] = v;
+                }
+            }
+    }
+    for (int i = 0; i < N_q; ++i)
+        for (int j = 0; j < N_q; ++j) {
+            if (i == j || covered[(size_t)i * N_q + j]) continue;
+            for (int m = 0; m < d; ++m)
+                if (D->S[i + (size_t)N_q * (j + (size_t)N_q * m)] != 0.0) return 0;
+        }
+    // C rows (volume side) and column chunks (facet side)
+    std::vector<std::vector<std::pair<int, double>>> crows(N_q);
+    std::vector<std::vector<std::pair<int, double>>> ccols(N_f);
+    for (int j = 0; j < N_f; ++j)
+        for (int i = 0; i < N_q; ++i) {
+            const double c = D->C[i + (size_t)N_q * j];
+            if (c == 0.0) continue;
+            crows[i].push_back({j, c});
+            ccols[j].push_back({i, c});
+        }
+    sse::StructTables& S = h->ST;
+    std::vector<int> cr_j;
+    std::vector<double> cr_v;
+    ell(crows, S.wcr, cr_j, cr_v);
+    const int ch = N1;
+    const int max_item = cfg.max_item;
+    std::vector<int> ci_i((size_t)ch * max_item, -1), ci_j(max_item, -1);
+    std::vector<double> ci_v((size_t)ch * max_item, 0.0);
+    std::vector<std::vector<std::pair<int, int>>> fitems(N_f);
+    int n_items = 0;
+    for (int j = 0; j < N_f; ++j) {
+        for (size_t s0 = 0; s0 < ccols[j].size(); s0 += ch) {
+            if (n_items >= max_item) return 0;
+            for (int w = 0; w < ch && s0 + w < ccols[j].size(); ++w) {
+                ci_i[(size_t)w * max_item + n_items] = ccols[j][s0 + w].first;
+                ci_v[(size_t)w * max_item + n_items] = ccols[j][s0 + w].second;
+            }
+            ci_j[n_items] = j;
+            fitems[j].push_back({n_items, 0});
+            ++n_items;
+        }
+    }
+    std::vector<int> fi;
+    std::vector<int> fi_dummy;
+    ell(fitems, S.wfi, fi, fi_dummy);
+    if (S.wcr > 12 || S.wfi > 8 || ch > 8) return 0;  // kernel register tables
+    S.ch = ch;
+    S.n_items = n_items;
+    int rc;
+    if ((rc = upload(h, SG, &S.SG)) || (rc = upload(h, cr_j, &S.cr_j)) || (rc = upload(h, cr_v, &S.cr_v)) ||
+        (rc = upload(h, ci_i, &S.ci_i)) || (rc = upload(h, ci_v, &S.ci_v)) || (rc = upload(h, ci_j, &S.ci_j)) ||
+        (rc = upload(h, fi, &S.fi)))
+        return rc;
+    double* ext = nullptr;
+    if ((rc = alloc(h, (size_t)N_f * D->N_c * N_e, &ext))) return rc;
+    S.uf_ext = ext;
+    h->TP.uf_ext = ext;
+    int dev = 0, n_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(h, SSE_ERR_HIP, "device query failed");
+    const int per_cu = sse::struct_occupancy(D->element, D->V_p);
+    if (per_cu <= 0) {
+        h->TP.uf_ext = nullptr;
+        return 0;
+    }
+    h->struct_blocks = (N_e + cfg.EPB - 1) / cfg.EPB;
+    h->use_struct = true;
+    return 0;
+}
+
 int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int which) {
+    if (h->use_struct) {
+        hipError_t e = hipSuccess;
+        if (which & 1) e = sse::launch_tensor(h->TP, u, dudt, rk, h->stream, 1);
+        if (e == hipSuccess && (which & 2)) e = sse::launch_struct(h->TP, h->ST, dudt, rk, h->stream, h->struct_blocks);
+        if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return 0;
+    }
     hipError_t e = h->use_tensor ? sse::launch_tensor(h->TP, u, dudt, rk, h->stream, which)
                                  : sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -460,6 +591,7 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     P.law.gamma = D->gamma;
     P.law.gm1 = D->gamma - 1.0;
     P.law.inv_gm1 = D->law == SSE_EULER ? 1.0 / (D->gamma - 1.0) : 0.0;
+    P.law.log_gm1 = D->law == SSE_EULER ? std::log(D->gamma - 1.0) : 0.0;
     for (int m = 0; m < 3; ++m) P.law.a[m] = D->a[m];
     P.law.half_lambda = D->half_lambda;
     P.law.inviscid_flux = D->inviscid_flux;
@@ -543,9 +675,14 @@ int sse_create(const sse_desc* D, sse_handle** out) {
         return cleanup_fail(SSE_ERR_UNSUPPORTED);
     }
     if ((rc = build_tensor(h, D))) return cleanup_fail(rc);
+    const char* kenv = getenv("SSE_KERNELS");  // "generic", "tensor" (table-driven pairs) or unset
+    if (!(kenv && (!strcmp(kenv, "generic") || !strcmp(kenv, "tensor"))) && (rc = build_struct(h, D)))
+        return cleanup_fail(rc);
     if (const char* env = getenv("SSE_FORCE_GENERIC")) {
         if (env[0] == '1') h->use_tensor = false;
     }
+    if (kenv && !strcmp(kenv, "generic")) h->use_tensor = false;
+    if (!h->use_tensor) h->use_struct = false;
     if (hipStreamCreate(&h->own_stream) != hipSuccess) {
         h->err = "hipStreamCreate failed";
         return cleanup_fail(SSE_ERR_HIP);
@@ -558,7 +695,12 @@ int sse_create(const sse_desc* D, sse_handle** out) {
 int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     if (!h || !buf || len <= 0) return SSE_ERR_INVALID_ARGUMENT;
     const DevParams& P = h->P;
-    if (h->use_tensor) {
+    if (h->use_struct) {
+        const sse::TensorParams& T = h->TP;
+        const sse::StructTables& S = h->ST;
+        snprintf(buf, len, "struct elem=%d P=%d blocks=%d wcr=%d items=%d ch=%d wfi=%d wrt=%d",
+                 T.elem, T.P, h->struct_blocks, S.wcr, S.n_items, S.ch, S.wfi, T.wrt);
+    } else if (h->use_tensor) {
         const sse::TensorParams& T = h->TP;
         snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",
                  T.elem, T.P, T.variant, T.n_pairs, T.n_vol_pairs, T.n_pass, T.pass_size, T.wv, T.wf, T.wrt, T.wr);

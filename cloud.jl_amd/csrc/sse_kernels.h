@@ -65,6 +65,8 @@ struct TensorParams {
     const double* VB;
     const double* VC;
     int xcd_remap;       // 1: spread contiguous element ranges over the 8 XCDs
+    double* uf_ext;      // when set, nodal_values also writes each facet state into the
+                         // neighbour's exterior slot [k][e][j] (mapP is an involution)
 };
 
 // Low-storage RK stage fused into the time-derivative epilogue.

@@ -19,7 +19,7 @@ namespace sse {
 enum { LAW_EULER = 1, LAW_ADVECTION = 2 };
 
 struct LawParams {
-    double gamma, gm1, inv_gm1;
+    double gamma, gm1, inv_gm1, log_gm1;
     double a[3];
     double half_lambda;
     int inviscid_flux;   // sse_inviscid_flux
@@ -251,6 +251,45 @@ __device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, con
 #pragma unroll
     for (int m = 0; m < D; ++m) c[1 + m] = c0 * (0.5 * (hL[1 + m] + hR[1 + m])) + a[m] * p_avg;
     c[D + 1] = c0 * Cc + 0.5 * (pL * avR + pR * avL);
+}
+
+// conservative_to_entropy! / entropy_to_conservative! (euler_navierstokes.jl:306-337)
+// with the powers written through logarithms:
+//   log(p / rho^gamma)                 = log p - gamma log rho
+//   ((gamma-1)/(-w_E)^gamma)^(1/(gamma-1)) exp(-s/(gamma-1))
+//                                      = exp((log(gamma-1) - gamma log(-w_E) - s) / (gamma-1))
+// (two transcendental calls per node instead of five; same values up to rounding).
+template <int D>
+__device__ __forceinline__ bool c2e_fast(const LawParams& P, const double* u, double* w) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) s += u[m + 1] * u[m + 1];
+    const double k = (0.5 / u[0]) * s;
+    const double p = P.gm1 * (u[D + 1] - k);
+    const double inv_p = 1.0 / p;
+    w[0] = P.inv_gm1 * (P.gamma - (log(p) - P.gamma * log(u[0]))) - k * inv_p;
+#pragma unroll
+    for (int m = 0; m < D; ++m) w[m + 1] = u[m + 1] * inv_p;
+    w[D + 1] = -u[0] * inv_p;
+    return (u[0] > 0.0) && (p > 0.0);
+}
+
+template <int D>
+__device__ __forceinline__ bool e2c_fast(const LawParams& P, const double* win, double* u) {
+    double w[D + 2];
+#pragma unroll
+    for (int e = 0; e < D + 2; ++e) w[e] = win[e] * P.gm1;
+    double s2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) s2 += w[m + 1] * w[m + 1];
+    const double k = s2 / (2.0 * w[D + 1]);
+    const double s = P.gamma - w[0] + k;
+    const double rho_e = exp(P.inv_gm1 * (P.log_gm1 - P.gamma * log(-w[D + 1]) - s));
+    u[0] = -w[D + 1] * rho_e;
+#pragma unroll
+    for (int m = 0; m < D; ++m) u[m + 1] = w[m + 1] * rho_e;
+    u[D + 1] = rho_e * (1.0 - k);
+    return win[D + 1] < 0.0;
 }
 
 // wave_speed from hoisted states (euler_navierstokes.jl:339-356)

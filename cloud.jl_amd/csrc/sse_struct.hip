@@ -1,0 +1,402 @@
+// sse_struct.hip -- time_derivative for ModalTensor Tri/Tet (Euler, FluxDifferencingForm,
+// entropy-conservative two-point flux) driven by the tensor-line structure of
+// the operators instead of pair tables.
+//
+// The union sparsity of S_1..S_d on collapsed-coordinate tensor elements is a
+// set of 1D lines (src/Solvers/operators.jl:182-200 with D_xi of
+// src/SpatialDiscretizations/SpatialDiscretizations.jl:414-418): node i pairs
+// with every node j that differs from i in exactly one tensor index (family f).
+// Each volume node's thread gathers its row of the Hadamard sum
+//   r_q[i,e] -= sum_{f, b != a_f(i)} sum_n a_n F(u_i, u_j)[e,n],
+//   a_n = sum_m S_m[i,j] (Lambda_i + Lambda_j)[m,n]
+// (the reference's flux_difference!, src/Solvers/flux_differencing_form.jl:37-75,
+// summed by rows; S antisymmetric and F symmetric make the two equal), and its
+// row of the facet correction (C nonzeros). The facet side of the facet
+// correction is gathered by column chunks (balanced across the team) and summed
+// through a small LDS stage. No atomics: every sum has a fixed order.
+//
+// A team of TS threads owns one element. The exterior facet state is read
+// contiguously: phase 1 writes each facet state also into the neighbour's
+// exterior slot (the reference's u_f[CI[mapP[:,k]], :] gather).
+#include <hip/hip_runtime.h>
+
+#include "sse_tensor.h"
+
+namespace sse {
+
+// team size, elements per block, threads per node (SPLIT threads share a
+// node's partner list; their partial sums meet in LDS)
+template <int ELEM, int P>
+struct StructTune;
+template <int P> struct StructTune<ELEM_TRI, P> { static constexpr int TS = 64, EPB = 2, SPLIT = 1; };
+template <> struct StructTune<ELEM_TET, 2> { static constexpr int TS = 64, EPB = 1, SPLIT = 1; };
+template <> struct StructTune<ELEM_TET, 3> { static constexpr int TS = 128, EPB = 1, SPLIT = 2; };
+template <> struct StructTune<ELEM_TET, 4> { static constexpr int TS = 256, EPB = 1, SPLIT = 2; };
+template <> struct StructTune<ELEM_TET, 5> { static constexpr int TS = 256, EPB = 1, SPLIT = 1; };
+
+template <int ELEM, int P>
+struct StructLayout {
+    using Sh = Shape<ELEM, P>;
+    using T = StructTune<ELEM, P>;
+    static constexpr int NC = Sh::NC, D = Sh::D, NH = Sh::NH;
+    // facet column chunks per element (N1 entries each): faces with one volume
+    // line per facet node, plus N1 chunks per node of a face coupled to N1^2 nodes
+    static constexpr int ITEMS = ELEM == ELEM_TET ? 3 * Sh::N1 * Sh::N1 + Sh::N1 * Sh::N1 * Sh::N1 : 3 * Sh::N1;
+    static constexpr int MAXITEM = (ITEMS + T::TS - 1) / T::TS * T::TS;
+    static constexpr int off_h = 0;                            // NH x NQ
+    static constexpr int off_L = off_h + NH * Sh::NQ;          // D*D x NQ
+    static constexpr int off_hf = off_L + D * D * Sh::NQ;      // NH x NF
+    static constexpr int off_nJ = off_hf + NH * Sh::NF;        // D x NF
+    static constexpr int off_st = off_nJ + D * Sh::NF;         // NC x MAXITEM partials
+    static constexpr int size = off_st + NC * MAXITEM;
+    static constexpr int off_ff = off_h;                       // NC x NF (after the pair phase)
+    static constexpr int off_r = off_hf;                       // NC x NQ
+    static constexpr int off_rb = off_L;                       // NC x NQ partial rows of split threads
+    static constexpr int off_y = off_L;                        // NC x NP
+    static constexpr int off_z = off_st;                       // NC x NZ
+    static constexpr int off_W = off_z + NC * Sh::NZ;          // NC x NW
+    static_assert(NC * Sh::NF <= NH * Sh::NQ, "f_f alias");
+    static_assert(NC * Sh::NQ <= NH * Sh::NF + D * Sh::NF + NC * MAXITEM, "r_q alias");
+    static_assert(NC * Sh::NP <= D * D * Sh::NQ, "y alias");
+    static_assert(NC * Sh::NQ <= D * D * Sh::NQ, "split partial-row alias");
+    static_assert(off_W + NC * Sh::NW <= size, "sum-factorisation temporaries alias");
+};
+
+// Structured operator tables (host-built from the descriptor's S, C and R).
+struct StructTables {
+    const double* SG;    // D x N1 x D x NQ: S_m[i, j(i, f, b)], index ((f * N1 + b) * D + m) * NQ + i
+    const int* cr_j;     // WCR x NQ: facet node of each C row entry (-1 pad)
+    const double* cr_v;  // WCR x NQ
+    const int* ci_i;     // CH x MAXITEM: volume nodes of each column chunk (-1 pad)
+    const double* ci_v;  // CH x MAXITEM
+    const int* ci_j;     // MAXITEM: facet node of each chunk (-1 = no chunk)
+    const int* fi;       // WFI x NF: chunks summed into each facet node (-1 pad)
+    int wcr, ch, n_items, wfi;
+    const double* uf_ext;  // exterior facet states [k][e][j]
+};
+
+template <int ELEM, int P>
+__global__ void __launch_bounds__((StructTune<ELEM, P>::TS * StructTune<ELEM, P>::EPB))
+k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ dudt, RKStage rk, int n_teams) {
+    using Sh = Shape<ELEM, P>;
+    using T = StructTune<ELEM, P>;
+    using SF = SumFact<ELEM, P, T::TS>;
+    using L = StructLayout<ELEM, P>;
+    constexpr int D = Sh::D, NC = Sh::NC, NH = Sh::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
+    constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
+    constexpr int MAXCR = 12, MAXCH = 8, MAXFI = 8, MAXR = TENSOR_MAXR;
+    constexpr int NPART = D * (N1 - 1), QH = (NPART + SPLIT - 1) / SPLIT;  // volume partners per thread
+    static_assert(NQ <= NT && NF <= NT, "one node per thread");
+    extern __shared__ double smem[];
+    const DevParams& p = tp.base;
+    const int team = threadIdx.x / TS, tt = threadIdx.x - team * TS;
+    const int half = tt / NT, tl = tt - half * NT;  // node index tl, partner share `half`
+    double* sm = smem + team * L::size;
+    double* sh = sm + L::off_h;
+    double* sL = sm + L::off_L;
+    double* shf = sm + L::off_hf;
+    double* snJ = sm + L::off_nJ;
+    double* sst = sm + L::off_st;
+    using Ph = Physics<D, LAW_EULER>;
+
+    const double Wi = tl < NQ ? p.W[tl] : 0.0;
+    const double Bj = tl < NF ? p.B[tl] : 0.0;
+
+    // ---- prefetch registers
+    double pu[NC], pL[D * D], pJ, pf[NC], px[NC], pn[D], pJf;
+    auto prefetch = [&](int k) {
+        if (tl < NQ) {
+            const double* uqk = p.uq + (size_t)k * NQ * NC;
+            const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) pu[e] = uqk[e * NQ + tl];
+#pragma unroll
+            for (int mn = 0; mn < D * D; ++mn) pL[mn] = Lk[mn * NQ + tl];
+            pJ = p.Jq[(size_t)k * NQ + tl];
+        }
+        if (tl < NF) {
+            const double* ufk = p.uf + (size_t)k * NF * NC;
+            const double* uxk = st.uf_ext + (size_t)k * NF * NC;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+                pf[e] = ufk[e * NF + tl];
+                px[e] = uxk[e * NF + tl];
+            }
+#pragma unroll
+            for (int m = 0; m < D; ++m) pn[m] = p.nJf[m + D * (tl + (size_t)NF * k)];
+            pJf = p.Jf[tl + (size_t)NF * k];
+        }
+    };
+
+    // one element per team; a team past the last element computes on a valid
+    // element (block-wide barriers) and stores nothing
+    const int last = p.N_e - 1;
+    const int kb = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * T::EPB;
+    {
+        const int k = min(kb + team, last);
+        const bool valid = kb + team < p.N_e;
+        if (half == 0) prefetch(k);
+        // ---- phase A: hoist, interface flux
+        double hi[NH], Li[D * D], r[NC], ff[NC], Ji = 1.0;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) r[e] = ff[e] = 0.0;
+        if (half == 0 && tl < NQ) {
+            hoist<D>(p.law, pu, hi);
+#pragma unroll
+            for (int v = 0; v < NH; ++v) sh[v * NQ + tl] = hi[v];
+#pragma unroll
+            for (int mn = 0; mn < D * D; ++mn) {
+                Li[mn] = pL[mn];
+                sL[mn * NQ + tl] = pL[mn];
+            }
+            Ji = pJ;
+        }
+        if (half == 0 && tl < NF) {
+            double hin[NH], hout[NH], nrm[D], c[NC];
+            hoist<D>(p.law, pf, hin);
+            hoist<D>(p.law, px, hout);
+#pragma unroll
+            for (int v = 0; v < NH; ++v) shf[v * NF + tl] = hin[v];
+            const double inv_jf = 1.0 / pJf;
+#pragma unroll
+            for (int m = 0; m < D; ++m) {
+                snJ[m * NF + tl] = pn[m];
+                nrm[m] = pn[m] * inv_jf;
+            }
+            ec_dot<D>(p.law, hin, hout, nrm, c);
+            const double bj = Bj * pJf;
+            if (p.law.inviscid_flux == 1) {
+                const double al = p.law.half_lambda * wave_speed_h<D>(p.law, hin, hout, nrm);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) ff[e] = (c[e] + al * (pf[e] - px[e])) * bj;
+            } else {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) ff[e] = c[e] * bj;
+            }
+        }
+        __syncthreads();
+
+        // ---- phase B: volume Hadamard row and facet-correction row of node tl
+        if (SPLIT > 1 && half > 0 && tl < NQ) {
+#pragma unroll
+            for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
+#pragma unroll
+            for (int mn = 0; mn < D * D; ++mn) Li[mn] = sL[mn * NQ + tl];
+        }
+        if (tl < NQ) {
+            double hj[NH], av[D], c[NC];
+#pragma unroll 1
+            for (int q = half * QH; q < (half + 1) * QH && q < NPART; ++q) {
+                const int f = q / (N1 - 1), dlt = q - f * (N1 - 1) + 1;
+                {
+                    int stride = 1;
+                    for (int g = f + 1; g < D; ++g) stride *= N1;
+                    const int af = (tl / stride) % N1;  // tensor index of node tl in family f
+                    const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+                    const int j = tl + (b - af) * stride;
+#pragma unroll
+                    for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+                    // a_n = sum_{m >= f} S_m[i,j] (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
+#pragma unroll
+                    for (int n = 0; n < D; ++n) av[n] = 0.0;
+                    for (int m = f; m < D; ++m) {
+                        const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+#pragma unroll
+                        for (int n = 0; n < D; ++n) av[n] += s * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
+                    }
+                    ec_dot<D>(p.law, hi, hj, av, c);
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) r[e] -= c[e];
+                }
+            }
+            const int wh = (st.wcr + SPLIT - 1) / SPLIT;
+#pragma unroll 1
+            for (int w = half * wh; w < (half + 1) * wh && w < st.wcr; ++w) {
+                const int j = st.cr_j[w * NQ + tl];
+                if (j < 0) continue;
+                const double crv = st.cr_v[w * NQ + tl];
+                const int fc = j / NPF;
+#pragma unroll
+                for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+#pragma unroll
+                for (int n = 0; n < D; ++n) {
+                    double s = snJ[n * NF + j];
+#pragma unroll
+                    for (int l = 0; l < D; ++l) s += Li[l + D * n] * p.fn[l + D * fc];
+                    av[n] = 0.5 * crv * s;
+                }
+                ec_dot<D>(p.law, hi, hj, av, c);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) r[e] -= c[e];
+            }
+        }
+        // ---- phase C: facet-correction column chunks (facet side)
+#pragma unroll 1
+        for (int it = 0; it < MAXITEM / TS; ++it) {
+            const int item = it * TS + tt;
+            const int j = item < st.n_items ? st.ci_j[item] : -1;
+            if (j < 0) continue;
+            double hj[NH], hv[NH], av[D], c[NC], part[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) part[e] = 0.0;
+#pragma unroll
+            for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+            const int fc = j / NPF;
+#pragma unroll 1
+            for (int w = 0; w < st.ch; ++w) {
+                const int i = st.ci_i[w * MAXITEM + item];
+                if (i < 0) continue;
+                const double cv = st.ci_v[w * MAXITEM + item];
+#pragma unroll
+                for (int v = 0; v < NH; ++v) hv[v] = sh[v * NQ + i];
+#pragma unroll
+                for (int n = 0; n < D; ++n) {
+                    double s = snJ[n * NF + j];
+#pragma unroll
+                    for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * p.fn[l + D * fc];
+                    av[n] = 0.5 * cv * s;
+                }
+                ec_dot<D>(p.law, hv, hj, av, c);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) part[e] += c[e];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sst[e * MAXITEM + item] = part[e];
+        }
+        __syncthreads();
+        // ---- phase D: facet node sums its chunks; f_f to LDS
+        double* sff = sm + L::off_ff;
+        double* srb = sm + L::off_rb;
+        if (SPLIT > 1 && half == 1 && tl < NQ) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) srb[e * NQ + tl] = r[e];
+        }
+        if (half == 0 && tl < NF) {
+#pragma unroll
+            for (int w = 0; w < MAXFI; ++w) {
+                const int item = w < st.wfi ? st.fi[w * NF + tl] : -1;
+                if (item < 0) continue;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) ff[e] -= sst[e * MAXITEM + item];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sff[e * NF + tl] = ff[e];
+        }
+        __syncthreads();
+        // ---- phase E: lift r_q -= R' f_f
+        double* sr = sm + L::off_r;
+        if (half == 0 && tl < NQ) {
+            if (SPLIT > 1) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) r[e] += srb[e * NQ + tl];
+            }
+#pragma unroll
+            for (int w = 0; w < MAXR; ++w) {
+                const int jj = w < tp.wrt ? tp.rt_idx[w * NQ + tl] : -1;
+                if (jj < 0) continue;
+                const double rv = tp.rt_val[w * NQ + tl];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) r[e] -= rv * sff[e * NF + jj];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sr[e * NQ + tl] = r[e];
+        }
+        __syncthreads();
+        // ---- phase F: dudt = V' r_q, weight-adjusted mass solve
+        double* sy = sm + L::off_y;
+        double* sZ = sm + L::off_z;
+        double* sW = sm + L::off_W;
+        SF::Vt(tp, tt, sr, sy, sZ, sW);
+        SF::V(tp, tt, sy, sr, sZ, sW);
+        if (half == 0 && tl < NQ) {
+            const double jw = Wi / Ji;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sr[e * NQ + tl] *= jw;
+        }
+        __syncthreads();
+        SF::Vt(tp, tt, sr, sy, sZ, sW);
+        const size_t base = (size_t)k * NP * NC;
+        if (!valid) {
+        } else if (!rk.active) {
+            for (int t = tt; t < NP * NC; t += TS) dudt[base + t] = sy[t];
+        } else {
+            for (int t = tt; t < NP * NC; t += TS) {
+                const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt * sy[t];
+                rk.tmp[base + t] = kt;
+                rk.u[base + t] += rk.B * kt;
+            }
+        }
+    }
+}
+
+struct StructConfig {
+    int TS, EPB, lds, max_item;
+};
+
+template <int ELEM, int P>
+StructConfig struct_config_of() {
+    using T = StructTune<ELEM, P>;
+    return StructConfig{T::TS, T::EPB, (int)(sizeof(double) * StructLayout<ELEM, P>::size * T::EPB),
+                        StructLayout<ELEM, P>::MAXITEM};
+}
+
+template <int ELEM, int P>
+hipError_t launch_struct_t(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
+                           hipStream_t s, int n_blocks) {
+    using T = StructTune<ELEM, P>;
+    const StructConfig c = struct_config_of<ELEM, P>();
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_time_derivative_struct<ELEM, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_time_derivative_struct<ELEM, P>), dim3(n_blocks), dim3(T::TS * T::EPB), c.lds, s, tp, st,
+                       dudt, rk, n_blocks * T::EPB);
+    return hipGetLastError();
+}
+
+#define SSE_STRUCT_CASES                                                                       \
+    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5) \
+    SSE_CASE(ELEM_TRI, 6) SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4) SSE_CASE(ELEM_TET, 5)
+
+template <int ELEM, int P>
+int struct_blocks_per_cu() {
+    using T = StructTune<ELEM, P>;
+    const StructConfig c = struct_config_of<ELEM, P>();
+    (void)hipFuncSetAttribute((const void*)k_time_derivative_struct<ELEM, P>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_time_derivative_struct<ELEM, P>, T::TS * T::EPB, c.lds) !=
+        hipSuccess)
+        return 0;
+    return nb;
+}
+
+int struct_occupancy(int elem, int P) {
+#define SSE_CASE(E, PP) \
+    if (elem == E && P == PP) return struct_blocks_per_cu<E, PP>();
+    SSE_STRUCT_CASES
+#undef SSE_CASE
+    return 0;
+}
+
+bool struct_config(int elem, int P, StructConfig* c) {
+#define SSE_CASE(E, PP) \
+    if (elem == E && P == PP) { *c = struct_config_of<E, PP>(); return true; }
+    SSE_STRUCT_CASES
+#undef SSE_CASE
+    return false;
+}
+
+hipError_t launch_struct(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
+                         hipStream_t s, int n_blocks) {
+#define SSE_CASE(E, PP) \
+    if (tp.elem == E && tp.P == PP) return launch_struct_t<E, PP>(tp, st, dudt, rk, s, n_blocks);
+    SSE_STRUCT_CASES
+#undef SSE_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sse

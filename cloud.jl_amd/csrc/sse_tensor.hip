@@ -19,280 +19,9 @@
 // A "team" of TS threads owns one element; a block holds EPB teams.
 #include <hip/hip_runtime.h>
 
-#include "sse_kernels.h"
+#include "sse_tensor.h"
 
 namespace sse {
-
-enum { ELEM_TRI = 2, ELEM_TET = 3 };
-
-// widest owner-reduction table and R' table the tensor kernels accept (checked on the host)
-constexpr int TENSOR_MAXW = 16;
-constexpr int TENSOR_MAXR = 10;
-
-template <int ELEM, int P>
-struct Shape;
-
-template <int P>
-struct Shape<ELEM_TRI, P> {
-    static constexpr int D = 2, N1 = P + 1, NQ = N1 * N1, NFAC = 3, NPF = N1, NF = NFAC * NPF;
-    static constexpr int NP = (P + 1) * (P + 2) / 2, NC = 4, NH = D + 5;
-    static constexpr int NZ = (P + 1) * N1;  // Z[b1][a2]
-    static constexpr int NW = 0;
-};
-
-template <int P>
-struct Shape<ELEM_TET, P> {
-    static constexpr int D = 3, N1 = P + 1, NQ = N1 * N1 * N1, NFAC = 4, NPF = N1 * N1, NF = NFAC * NPF;
-    static constexpr int NP = (P + 1) * (P + 2) * (P + 3) / 6, NC = 5, NH = D + 5;
-    static constexpr int NPAIR12 = (P + 1) * (P + 2) / 2;
-    static constexpr int NZ = NPAIR12 * N1;  // Z[q(b1,b2)][a3]
-    static constexpr int NW = (P + 1) * N1 * N1;  // W[b1][a2][a3]
-};
-
-// Launch shape of the two kernels. K2 (time_derivative): team size TS, EPB
-// elements per block, R pairs per thread per pass. K1 (nodal_values): TS1, EPB1.
-// V selects among variants of the same (element, degree) compiled for tuning.
-template <int ELEM, int P, int V>
-struct Tune;
-#define SSE_TUNE(E, PP, VV, ts, epb, r, ts1, epb1)                                       \
-    template <>                                                                        \
-    struct Tune<E, PP, VV> {                                                           \
-        static constexpr int TS = ts, EPB = epb, R = r, TS1 = ts1, EPB1 = epb1;         \
-    };
-SSE_TUNE(ELEM_TRI, 2, 0, 64, 2, 1, 64, 2)
-SSE_TUNE(ELEM_TRI, 3, 0, 64, 2, 2, 64, 2)
-SSE_TUNE(ELEM_TRI, 4, 0, 64, 2, 3, 64, 2)
-SSE_TUNE(ELEM_TRI, 5, 0, 64, 2, 5, 64, 2)
-SSE_TUNE(ELEM_TRI, 6, 0, 64, 2, 7, 64, 2)
-SSE_TUNE(ELEM_TET, 2, 0, 64, 2, 3, 64, 2)
-SSE_TUNE(ELEM_TET, 3, 0, 128, 1, 3, 64, 2)
-SSE_TUNE(ELEM_TET, 4, 0, 128, 1, 2, 128, 1)
-SSE_TUNE(ELEM_TET, 4, 1, 128, 1, 2, 64, 1)
-SSE_TUNE(ELEM_TET, 4, 2, 128, 1, 2, 128, 2)
-SSE_TUNE(ELEM_TET, 4, 3, 128, 2, 2, 128, 4)
-SSE_TUNE(ELEM_TET, 4, 4, 128, 1, 1, 64, 2)
-SSE_TUNE(ELEM_TET, 5, 0, 256, 1, 2, 256, 1)
-#undef SSE_TUNE
-
-// q index of (b1, b2), b1 + b2 <= P, enumerated with b1 outer: also the mode
-// number of (b1, b2) in 2D.
-template <int P>
-__device__ __forceinline__ constexpr int q_index(int b1, int b2) {
-    return b1 * (P + 1) - (b1 * (b1 - 1)) / 2 + b2;
-}
-
-// number of 3D modes before (b1, b2, 0)
-template <int P>
-__device__ __forceinline__ constexpr int mode_offset3(int b1, int b2) {
-    int off = 0;
-    for (int i = 0; i < b1; ++i) off += (P - i + 1) * (P - i + 2) / 2;
-    for (int j = 0; j < b2; ++j) off += P - b1 - j + 1;
-    return off;
-}
-
-__device__ __forceinline__ int element_of_block(int b, int nb, int remap) {
-    if (!remap || (nb & 7)) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
-// ----------------------------------------------------------------------------- sum factorisation
-// All arrays are component-major LDS arrays: x[e * stride + point].
-template <int ELEM, int P, int TS>
-struct SumFact;
-
-template <int P, int TS>
-struct SumFact<ELEM_TRI, P, TS> {
-    using Sh = Shape<ELEM_TRI, P>;
-    static constexpr int N1 = Sh::N1, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ;
-
-    // y[a1,a2] = sum_b1 A[a1,b1] sum_{b2 <= P-b1} B[a2,b1,b2] x[q(b1,b2)]
-    __device__ static void V(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double*) {
-        for (int o = tl; o < NZ; o += TS) {
-            const int b1 = o / N1, a2 = o - b1 * N1;
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-            const int off = q_index<P>(b1, 0);
-            for (int b2 = 0; b2 <= P - b1; ++b2) {
-                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += bv * x[e * NP + off + b2];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
-        }
-        __syncthreads();
-        for (int o = tl; o < NQ; o += TS) {
-            const int a1 = o / N1, a2 = o - a1 * N1;
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-            for (int b1 = 0; b1 <= P; ++b1) {
-                const double av = tp.VA[a1 + N1 * b1];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += av * Z[e * NZ + b1 * N1 + a2];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) y[e * NQ + o] = s[e];
-        }
-        __syncthreads();
-    }
-
-    // y[q(b1,b2)] = sum_a2 B[a2,b1,b2] sum_a1 A[a1,b1] x[a1,a2]
-    __device__ static void Vt(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double*) {
-        for (int o = tl; o < NZ; o += TS) {
-            const int b1 = o / N1, a2 = o - b1 * N1;
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-            for (int a1 = 0; a1 < N1; ++a1) {
-                const double av = tp.VA[a1 + N1 * b1];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += av * x[e * NQ + a1 * N1 + a2];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
-        }
-        __syncthreads();
-        for (int m = tl; m < NP; m += TS) {
-            int b1 = 0;
-            while (q_index<P>(b1 + 1, 0) <= m && b1 < P) ++b1;
-            const int b2 = m - q_index<P>(b1, 0);
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-            for (int a2 = 0; a2 < N1; ++a2) {
-                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += bv * Z[e * NZ + b1 * N1 + a2];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) y[e * NP + m] = s[e];
-        }
-        __syncthreads();
-    }
-};
-
-template <int P, int TS>
-struct SumFact<ELEM_TET, P, TS> {
-    using Sh = Shape<ELEM_TET, P>;
-    static constexpr int N1 = Sh::N1, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ, NW = Sh::NW;
-    static constexpr int NN = N1 * N1;
-
-    __device__ static void decode_q(int q, int& b1, int& b2) {
-        b1 = 0;
-        while (b1 < P && q_index<P>(b1 + 1, 0) <= q) ++b1;
-        b2 = q - q_index<P>(b1, 0);
-    }
-
-    // y[a1,a2,a3] = sum A[a1,b1] B[a2,b1,b2] C[a3,b1,b2,b3] x[sigma(b1,b2,b3)]
-    __device__ static void V(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double* Wt) {
-        for (int o = tl; o < NZ; o += TS) {
-            const int q = o / N1, a3 = o - q * N1;
-            int b1, b2;
-            decode_q(q, b1, b2);
-            const int off = mode_offset3<P>(b1, b2);
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-            for (int b3 = 0; b3 <= P - b1 - b2; ++b3) {
-                const double cv = tp.VC[a3 + N1 * (b1 + (P + 1) * (b2 + (P + 1) * b3))];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += cv * x[e * NP + off + b3];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
-        }
-        __syncthreads();
-        for (int o = tl; o < NW; o += TS) {
-            const int b1 = o / NN, a23 = o - b1 * NN, a2 = a23 / N1, a3 = a23 - a2 * N1;
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-            const int q0 = q_index<P>(b1, 0);
-            for (int b2 = 0; b2 <= P - b1; ++b2) {
-                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += bv * Z[e * NZ + (q0 + b2) * N1 + a3];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) Wt[e * NW + o] = s[e];
-        }
-        __syncthreads();
-        for (int o = tl; o < NQ; o += TS) {
-            const int a1 = o / NN, a23 = o - a1 * NN;
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-            for (int b1 = 0; b1 <= P; ++b1) {
-                const double av = tp.VA[a1 + N1 * b1];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += av * Wt[e * NW + b1 * NN + a23];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) y[e * NQ + o] = s[e];
-        }
-        __syncthreads();
-    }
-
-    __device__ static void Vt(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double* Wt) {
-        for (int o = tl; o < NW; o += TS) {
-            const int b1 = o / NN, a23 = o - b1 * NN;
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-            for (int a1 = 0; a1 < N1; ++a1) {
-                const double av = tp.VA[a1 + N1 * b1];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += av * x[e * NQ + a1 * NN + a23];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) Wt[e * NW + o] = s[e];
-        }
-        __syncthreads();
-        for (int o = tl; o < NZ; o += TS) {
-            const int q = o / N1, a3 = o - q * N1;
-            int b1, b2;
-            decode_q(q, b1, b2);
-            double s[NC];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-            for (int a2 = 0; a2 < N1; ++a2) {
-                const double bv = tp.VB[a2 + N1 * (b1 + (P + 1) * b2)];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] += bv * Wt[e * NW + b1 * NN + a2 * N1 + a3];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
-        }
-        __syncthreads();
-        for (int q = tl; q < Sh::NPAIR12; q += TS) {
-            int b1, b2;
-            decode_q(q, b1, b2);
-            const int off = mode_offset3<P>(b1, b2);
-            for (int b3 = 0; b3 <= P - b1 - b2; ++b3) {
-                double s[NC];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] = 0.0;
-#pragma unroll
-                for (int a3 = 0; a3 < N1; ++a3) {
-                    const double cv = tp.VC[a3 + N1 * (b1 + (P + 1) * (b2 + (P + 1) * b3))];
-#pragma unroll
-                    for (int e = 0; e < NC; ++e) s[e] += cv * Z[e * NZ + q * N1 + a3];
-                }
-#pragma unroll
-                for (int e = 0; e < NC; ++e) y[e * NP + off + b3] = s[e];
-            }
-        }
-        __syncthreads();
-    }
-};
 
 // ----------------------------------------------------------------------------- nodal_values
 template <int ELEM, int P>
@@ -340,7 +69,7 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
         double ub[NC], wb[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) ub[e] = sa[e * NQ + i];
-        ok &= Ph::c2e(p.law, ub, wb);
+        ok &= c2e_fast<D>(p.law, ub, wb);
         const double wj = p.W[i] * Jk[i];
 #pragma unroll
         for (int e = 0; e < NC; ++e) sa[e * NQ + i] = wb[e] * wj;
@@ -362,7 +91,7 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
         double ub[NC], wb[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) wb[e] = sa[e * NQ + i];
-        ok &= Ph::e2c(p.law, wb, ub);
+        ok &= e2c_fast<D>(p.law, wb, ub);
         if (valid) {
 #pragma unroll
             for (int e = 0; e < NC; ++e) uqk[e * NQ + i] = ub[e];
@@ -379,10 +108,17 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
 #pragma unroll
             for (int e = 0; e < NC; ++e) wb[e] += rv * sa[e * NQ + i];
         }
-        ok &= Ph::e2c(p.law, wb, ub);
+        ok &= e2c_fast<D>(p.law, wb, ub);
         if (valid) {
 #pragma unroll
             for (int e = 0; e < NC; ++e) ufk[e * NF + j] = ub[e];
+            if (tp.uf_ext) {  // also the neighbour's exterior state: uf_ext at mapP(j, k) = u_f(j, k)
+                const int g = p.mapP[j + (size_t)NF * k];
+                const int kn = g / NF, jn = g - kn * NF;
+                double* uxn = tp.uf_ext + (size_t)kn * NF * NC;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) uxn[e * NF + jn] = ub[e];
+            }
         }
     }
     if (valid && !ok) atomicOr(p.status, 1);

@@ -67,8 +67,8 @@ def test_residual_matches_oracle(name):
     got = _gpu_residual(ode, u)
     ref = O.residual(ode.p.arrays, u)
     assert _rel(got, ref) <= RES_TOL, (name, _rel(got, ref))
-    # Tri/Tet ModalTensor Euler runs the compile-time-shaped kernels
-    expected = "generic" if name.startswith("euler1d") else "tensor"
+    # Tri/Tet ModalTensor Euler runs the line-structured compile-time-shaped kernels
+    expected = "generic" if name.startswith("euler1d") else "struct"
     assert ode.p.describe().startswith(expected), ode.p.describe()
 
 
@@ -126,17 +126,24 @@ def test_nonphysical_state_is_reported():
     semi_discrete_residual(out, ode.u0, ode.p)
 
 
-@pytest.mark.parametrize("name", ["vortex2d_tri_p4_8x8", "wave3d_tet_p4_curved"])
-def test_tensor_kernels_match_generic_kernels(name, monkeypatch):
-    """The compile-time-shaped tensor kernels and the runtime-shaped generic
-    kernels (selected with SSE_FORCE_GENERIC=1 at handle creation) agree."""
-    ode = CASES[name]()
-    u = _perturbed(ode)
-    fast = _gpu_residual(ode, u)
-    monkeypatch.setenv("SSE_FORCE_GENERIC", "1")
-    ode2 = CASES[name]()
-    generic = _gpu_residual(ode2, u)
-    ref = O.residual(ode.p.arrays, u)
-    assert _rel(fast, ref) <= RES_TOL
-    assert _rel(generic, ref) <= RES_TOL
-    assert _rel(fast, generic) <= RES_TOL
+@pytest.mark.parametrize("name", ["vortex2d_tri_p4_8x8", "wave3d_tet_p4_curved", "wave3d_tet_p3_curved"])
+def test_kernel_families_agree(name, monkeypatch):
+    """The three kernel families (line-structured, table-driven tensor, and the
+    runtime-shaped generic kernels; chosen with SSE_KERNELS at handle creation)
+    agree with each other and with the oracle."""
+    u = _perturbed(CASES[name]())
+    ref = None
+    got = {}
+    for fam in ("struct", "tensor", "generic"):
+        if fam == "struct":
+            monkeypatch.delenv("SSE_KERNELS", raising=False)
+        else:
+            monkeypatch.setenv("SSE_KERNELS", fam)
+        ode = CASES[name]()
+        got[fam] = _gpu_residual(ode, u)
+        assert ode.p.describe().startswith(fam), ode.p.describe()
+        if ref is None:
+            ref = O.residual(ode.p.arrays, u)
+        assert _rel(got[fam], ref) <= RES_TOL, (fam, _rel(got[fam], ref))
+    assert _rel(got["struct"], got["tensor"]) <= RES_TOL
+    assert _rel(got["struct"], got["generic"]) <= RES_TOL
