@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, run 5 copy kernels (dst = 1.0 * src) over this many MiB "
+                         "(a kernel of known traffic for calibrating HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the "
@@ -154,6 +157,14 @@ def main():
                      "algorithmic_bytes_per_element": wk.bytes,
                      "hbm_frac_algorithmic": achieved_gbs / HBM_PEAK_GBS},
     }
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device=u.device)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads src once, writes dst once
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 x {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(solver.arrays, ode.u0, dof, args.cpu_seconds)
     print(json.dumps(line), flush=True)
