@@ -13,7 +13,7 @@
 //     an LDS staging area and are summed into the owning node in a fixed,
 //     host-built order (deterministic, no atomics);
 //   * per-node quantities (velocity, pressure, log rho, rho/p, log(rho/p)) are
-//     computed once, so a pair costs four divides and no logarithm;
+//     computed once, so a pair costs one divide and no logarithm;
 //   * the flux tensor is never formed: it is contracted with the metric vector
 //     inside the flux (ec_dot).
 // A "team" of TS threads owns one element; a block holds EPB teams.

@@ -147,3 +147,66 @@ def test_kernel_families_agree(name, monkeypatch):
         assert _rel(got[fam], ref) <= RES_TOL, (fam, _rel(got[fam], ref))
     assert _rel(got["struct"], got["tensor"]) <= RES_TOL
     assert _rel(got["struct"], got["generic"]) <= RES_TOL
+
+
+# ----------------------------------------------------------------------------- full benchmark size
+# 3D Euler, curved tets, ModalTensor p=4, 16^3 hexes x 6 tets (24,576 elements):
+# one residual and 20 CK2N54 steps against the oracle, and the size-independent
+# invariants (conservation, entropy conservation with the EC interface flux,
+# free-stream preservation) on the GPU residual itself.
+@pytest.fixture(scope="module")
+def full_lf():
+    return density_wave_3d_config(M=16, p=4, factor=0.05, ic_strength=0.5)
+
+
+def test_full_size_residual_and_20_rk_steps_match_oracle(full_lf):
+    from sse.time_integration import rk_steps, to_device, to_host
+    law, sd, ic, form, ode = full_lf
+    O.set_threads(min(16, O.lib().sse_oracle_max_threads()))
+    got = _gpu_residual(ode)
+    ref = O.residual(ode.p.arrays, ode.u0)
+    assert _rel(got, ref) <= RES_TOL, _rel(got, ref)
+    dt = 0.1 * (2.0 / 16) / (4 + 1) ** 2
+    ud = to_device(ode.u0)
+    rk_steps(ode.p, ud, dt, 20)
+    got = to_host(ud)
+    ref = O.rk_steps(ode.p.arrays, ode.u0, dt, 20)
+    assert _rel(got, ref) <= RK_TOL, _rel(got, ref)
+
+
+def _entropy_rate(sd, law, u, dudt):
+    # src/Analysis/conservation.jl:169-189: with P = M_k^{-1} V' WJ_k the rate
+    # (P w_q)' M_k dudt reduces to w_q' WJ_k V dudt
+    from sse.conservation_laws import conservative_to_entropy
+    ref = sd.reference_approximation
+    V = ref.Vdense()
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q
+    wq = conservative_to_entropy(law, np.einsum("qp,pck->qkc", V, u))
+    terms = WJ[:, :, None] * wq * np.einsum("qp,pck->qkc", V, dudt)
+    return terms.sum(), np.abs(terms).sum()
+
+
+def test_full_size_conservation_and_free_stream(full_lf):
+    law, sd, ic, form, ode = full_lf
+    ref = sd.reference_approximation
+    V = ref.Vdense()
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q
+    dudt = _gpu_residual(ode)
+    rate = np.einsum("qk,qp,pck->c", WJ, V, dudt)
+    scale = np.einsum("qk,qp,pck->c", WJ, np.abs(V), np.abs(dudt))
+    assert np.all(np.abs(rate) <= 1e-13 * scale), rate / scale
+    rho, v, p = 1.2, np.array([0.3, -0.2, 0.5]), 0.9
+    ucons = np.array([rho, *(rho * v), p / 0.4 + 0.5 * rho * v @ v])
+    coef0 = np.linalg.lstsq(V, np.ones(ref.N_q), rcond=None)[0]
+    u = np.asfortranarray(coef0[:, None, None] * ucons[None, :, None] * np.ones((1, 1, sd.N_e)))
+    assert np.abs(_gpu_residual(ode, u)).max() < 1e-11
+
+
+def test_full_size_entropy_conservation_with_ec_interface_flux():
+    from sse.conservation_laws import EntropyConservativeNumericalFlux
+    from sse.solvers import FluxDifferencingForm
+    form = FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux())
+    law, sd, ic, form, ode = density_wave_3d_config(M=16, p=4, factor=0.05, form=form, ic_strength=0.5)
+    u = _perturbed(ode)
+    rate, scale = _entropy_rate(sd, law, u, _gpu_residual(ode, u))
+    assert abs(rate) <= 1e-12 * scale, (rate, scale)
