@@ -23,6 +23,8 @@ hipError_t launch_generic_dispatch(const DevParams& p, const double* u, double* 
                                    hipStream_t s, int which);
 size_t lds_nodal_values_generic(const DevParams& p);
 size_t lds_time_derivative_generic(const DevParams& p);
+hipError_t launch_standard(const DevParams& p, double* dudt, const RKStage& rk, hipStream_t s, int which);
+size_t lds_standard(const DevParams& p);
 struct TensorConfig {
     int TS, EPB, R, lds1, lds2, NQ, NF, NP;
 };
@@ -167,14 +169,32 @@ int validate(const sse_desc* D, std::string& msg) {
         msg = "sizes must be positive"; return SSE_ERR_INVALID_ARGUMENT;
     }
     if (D->N_f % D->N_fac != 0) { msg = "N_f must be a multiple of N_fac"; return SSE_ERR_INVALID_ARGUMENT; }
-    if (D->form != SSE_FLUX_DIFFERENCING) { msg = "only FluxDifferencingForm is implemented"; return SSE_ERR_UNSUPPORTED; }
+    const bool standard = (D->form == SSE_STANDARD);
+    if (!standard && D->form != SSE_FLUX_DIFFERENCING) { msg = "unknown residual form"; return SSE_ERR_INVALID_ARGUMENT; }
     if (D->law == SSE_EULER) {
         if (D->N_c != D->d + 2) { msg = "Euler needs N_c = d + 2"; return SSE_ERR_INVALID_ARGUMENT; }
         if (!(D->gamma > 1.0)) { msg = "gamma must exceed 1"; return SSE_ERR_INVALID_ARGUMENT; }
-    } else if (D->law == SSE_LINEAR_ADVECTION) {
-        if (D->N_c != 1) { msg = "linear advection needs N_c = 1"; return SSE_ERR_INVALID_ARGUMENT; }
+    } else if (D->law == SSE_LINEAR_ADVECTION || D->law == SSE_LINEAR_ADVECTION_DIFFUSION) {
+        if (D->N_c != 1) { msg = "linear advection(-diffusion) needs N_c = 1"; return SSE_ERR_INVALID_ARGUMENT; }
+        if (D->law == SSE_LINEAR_ADVECTION_DIFFUSION) {
+            if (!standard) {
+                msg = "second-order laws are implemented for StandardForm (as in the reference)";
+                return SSE_ERR_UNSUPPORTED;
+            }
+            if (D->viscous_flux != SSE_BR1) { msg = "second-order laws need the BR1 viscous flux"; return SSE_ERR_UNSUPPORTED; }
+            if (!(D->b >= 0.0)) { msg = "diffusion coefficient must be non-negative"; return SSE_ERR_INVALID_ARGUMENT; }
+        }
     } else {
-        msg = "conservation law not supported by the flux-differencing kernels"; return SSE_ERR_UNSUPPORTED;
+        msg = "unknown conservation law"; return SSE_ERR_INVALID_ARGUMENT;
+    }
+    if (standard) {
+        if (!D->D) { msg = "StandardForm needs the derivative operators D"; return SSE_ERR_INVALID_ARGUMENT; }
+        if (D->mapping_form != SSE_SKEW_SYMMETRIC_MAPPING && D->mapping_form != SSE_STANDARD_MAPPING) {
+            msg = "unknown mapping form"; return SSE_ERR_INVALID_ARGUMENT;
+        }
+        if (D->two_point_flux != SSE_TWO_POINT_CONSERVATIVE) {
+            msg = "StandardForm's interface flux uses the conservative two-point flux"; return SSE_ERR_INVALID_ARGUMENT;
+        }
     }
     if (D->inviscid_flux < 1 || D->inviscid_flux > 3) { msg = "unknown inviscid flux"; return SSE_ERR_INVALID_ARGUMENT; }
     if (D->two_point_flux < 1 || D->two_point_flux > 2) { msg = "unknown two-point flux"; return SSE_ERR_INVALID_ARGUMENT; }
@@ -186,7 +206,7 @@ int validate(const sse_desc* D, std::string& msg) {
     if (!D->V && D->mass_solver != SSE_MASS_DIAGONAL) {
         msg = "nodal schemes use the diagonal mass solver"; return SSE_ERR_UNSUPPORTED;
     }
-    if (!D->R || !D->W || !D->B || !D->S || !D->J_q || !D->Lambda_q || !D->nJf || !D->J_f || !D->face_normals ||
+    if (!D->R || !D->W || !D->B || (!standard && !D->S) || !D->J_q || !D->Lambda_q || !D->nJf || !D->J_f || !D->face_normals ||
         !D->mapP) {
         msg = "missing operator or geometry array"; return SSE_ERR_INVALID_ARGUMENT;
     }
@@ -542,8 +562,14 @@ int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int 
         if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
         return 0;
     }
-    hipError_t e = h->use_tensor ? sse::launch_tensor(h->TP, u, dudt, rk, h->stream, which)
-                                 : sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
+    hipError_t e;
+    if (h->P.standard) {  // phase 1 without entropy projection, then the StandardForm kernels
+        e = (which & 1) ? sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, 1) : hipSuccess;
+        if (e == hipSuccess) e = sse::launch_standard(h->P, dudt, rk, h->stream, which);
+    } else {
+        e = h->use_tensor ? sse::launch_tensor(h->TP, u, dudt, rk, h->stream, which)
+                          : sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
+    }
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -575,7 +601,7 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     *out = nullptr;
     std::string msg;
     int code = validate(D, msg);
-    if (!code) code = check_nJq(D, msg);
+    if (!code && D->form == SSE_FLUX_DIFFERENCING) code = check_nJq(D, msg);  // StandardForm does not use nJq
     if (code) return fail(nullptr, code, msg);
 
     sse_handle* h = new sse_handle();
@@ -585,7 +611,12 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     P.npf = D->N_f / D->N_fac; P.N_c = D->N_c; P.N_e = D->N_e;
     P.has_V = D->V != nullptr;
     P.R_sel = D->R_selection != nullptr;
-    P.project = (D->N_c > 1) && !(!P.has_V && P.R_sel);
+    P.standard = D->form == SSE_STANDARD;
+    P.skew = D->mapping_form != SSE_STANDARD_MAPPING;
+    P.second_order = D->law == SSE_LINEAR_ADVECTION_DIFFUSION;
+    P.b = D->b;
+    // entropy projection only in the flux-differencing form (flux_differencing_form.jl:253-292)
+    P.project = !P.standard && (D->N_c > 1) && !(!P.has_V && P.R_sel);
     P.C_kind = D->C_kind;
     P.mass_diag = D->mass_solver == SSE_MASS_DIAGONAL;
     P.law.gamma = D->gamma;
@@ -624,23 +655,36 @@ int sse_create(const sse_desc* D, sse_handle** out) {
         (rc = upload(h, Rt.ptr, &P.Rt_ptr)) || (rc = upload(h, Rt.idx, &P.Rt_idx)) ||
         (rc = upload(h, Rt.val, &P.Rt_val)))
         return cleanup_fail(rc);
-    // union sparsity of the S_m by rows
-    {
+    // union sparsity by rows of d stacked N_q x N_q operators A_m (column-major,
+    // A[i + N_q (j + N_q m)]), optionally transposed, with or without the diagonal
+    auto union_rows = [&](const double* A, bool transpose, bool skip_diag, const int** p_ptr, const int** p_idx,
+                          const double** p_val) {
         std::vector<int> ptr(1, 0), idx;
         std::vector<double> val;
+        auto at = [&](int i, int j, int m) {
+            return transpose ? A[j + (size_t)N_q * (i + (size_t)N_q * m)] : A[i + (size_t)N_q * (j + (size_t)N_q * m)];
+        };
         for (int i = 0; i < N_q; ++i) {
             for (int j = 0; j < N_q; ++j) {
-                if (j == i) continue;
+                if (skip_diag && j == i) continue;
                 bool nz = false;
-                for (int m = 0; m < d; ++m) nz |= D->S[i + (size_t)N_q * (j + (size_t)N_q * m)] != 0.0;
+                for (int m = 0; m < d; ++m) nz |= at(i, j, m) != 0.0;
                 if (!nz) continue;
                 idx.push_back(j);
-                for (int m = 0; m < d; ++m) val.push_back(D->S[i + (size_t)N_q * (j + (size_t)N_q * m)]);
+                for (int m = 0; m < d; ++m) val.push_back(at(i, j, m));
             }
             ptr.push_back((int)idx.size());
         }
-        if ((rc = upload(h, ptr, &P.S_ptr)) || (rc = upload(h, idx, &P.S_idx)) || (rc = upload(h, val, &P.S_val)))
+        int r;
+        if ((r = upload(h, ptr, p_ptr)) || (r = upload(h, idx, p_idx)) || (r = upload(h, val, p_val))) return r;
+        return 0;
+    };
+    if (P.standard) {  // rows of D_m and of D_m' (StandardForm volume operators)
+        if ((rc = union_rows(D->D, false, false, &P.D_ptr, &P.D_idx, &P.D_val)) ||
+            (rc = union_rows(D->D, true, false, &P.Dt_ptr, &P.Dt_idx, &P.Dt_val)))
             return cleanup_fail(rc);
+    } else if ((rc = union_rows(D->S, false, true, &P.S_ptr, &P.S_idx, &P.S_val))) {
+        return cleanup_fail(rc);
     }
     if (D->C_kind != SSE_C_NONE) {
         Rows Cr = rows_of(D->C, N_q, N_f), Cc = cols_of(D->C, N_q, N_f);
@@ -670,11 +714,15 @@ int sse_create(const sse_desc* D, sse_handle** out) {
         h->err = "hipMemset failed";
         return cleanup_fail(SSE_ERR_HIP);
     }
-    if (sse::lds_time_derivative_generic(P) > 64 * 1024 || sse::lds_nodal_values_generic(P) > 64 * 1024) {
+    if (P.second_order &&
+        ((rc = alloc(h, (size_t)N_q * d * N_e, &P.qq)) || (rc = alloc(h, (size_t)N_f * d * N_e, &P.qf))))
+        return cleanup_fail(rc);
+    if (sse::lds_time_derivative_generic(P) > 64 * 1024 || sse::lds_nodal_values_generic(P) > 64 * 1024 ||
+        (P.standard && sse::lds_standard(P) > 64 * 1024)) {
         h->err = "element too large for the LDS-staged kernels";
         return cleanup_fail(SSE_ERR_UNSUPPORTED);
     }
-    if ((rc = build_tensor(h, D))) return cleanup_fail(rc);
+    if (!P.standard && (rc = build_tensor(h, D))) return cleanup_fail(rc);
     const char* kenv = getenv("SSE_KERNELS");  // "generic", "tensor" (table-driven pairs) or unset
     if (!(kenv && (!strcmp(kenv, "generic") || !strcmp(kenv, "tensor"))) && (rc = build_struct(h, D)))
         return cleanup_fail(rc);
@@ -704,6 +752,9 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
         const sse::TensorParams& T = h->TP;
         snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",
                  T.elem, T.P, T.variant, T.n_pairs, T.n_vol_pairs, T.n_pass, T.pass_size, T.wv, T.wf, T.wrt, T.wr);
+    } else if (P.standard) {
+        snprintf(buf, len, "standard d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d mapping=%s%s", P.d, P.N_p, P.N_q, P.N_f,
+                 P.N_c, P.N_e, P.skew ? "skew" : "standard", P.second_order ? " br1" : "");
     } else {
         snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d", P.d, P.N_p, P.N_q, P.N_f, P.N_c, P.N_e);
     }

@@ -17,6 +17,10 @@ struct DevParams {
     int project;     // entropy projection performed in nodal_values
     int C_kind;      // 0 none, 1/2 facet correction present
     int mass_diag;   // DiagonalSolver (W J)^{-1}
+    int standard;    // StandardForm (1) or FluxDifferencingForm (0)
+    int skew;        // StandardForm: skew-symmetric (1) or standard (0) mapping form
+    int second_order;  // BR1 auxiliary-variable path (linear advection-diffusion)
+    double b;          // diffusion coefficient
     LawParams law;
     const double* V;  // N_q x N_p column-major
     // R rows: facet node j -> (volume node, value)
@@ -28,6 +32,10 @@ struct DevParams {
     // C rows: volume node i -> (facet node j, C_ij); C columns: facet node j -> (volume node i, C_ij)
     const int* Cr_ptr; const int* Cr_idx; const double* Cr_val;
     const int* Cc_ptr; const int* Cc_idx; const double* Cc_val;
+    // StandardForm: union sparsity of D_1..D_d by rows (node i -> (node j, D_m[i,j] for m < d))
+    // and of D_1^T..D_d^T (node i -> (node j, D_m[j,i] for m < d))
+    const int* D_ptr; const int* D_idx; const double* D_val;
+    const int* Dt_ptr; const int* Dt_idx; const double* Dt_val;
     const double* W;   // N_q
     const double* B;   // N_f
     const double* fn;  // d x N_fac reference face normals
@@ -40,6 +48,9 @@ struct DevParams {
     // phase-1 outputs: u_q [k][e][i], u_f [k][e][j]
     double* uq;
     double* uf;
+    // auxiliary variable of second-order laws: q_q [k][m][e][i], q_f [k][m][e][j]
+    double* qq;
+    double* qf;
     int* status;
 };
 

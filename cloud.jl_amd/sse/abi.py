@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-SSE_ABI_VERSION = 1
+SSE_ABI_VERSION = 2
 
 SSE_OK = 0
 SSE_ERR_INVALID_ARGUMENT = 1
@@ -17,6 +17,8 @@ SSE_LINE, SSE_TRI, SSE_TET = 1, 2, 3
 SSE_EULER, SSE_LINEAR_ADVECTION, SSE_LINEAR_ADVECTION_DIFFUSION = 1, 2, 3
 SSE_FLUX_DIFFERENCING, SSE_STANDARD = 1, 2
 SSE_LAX_FRIEDRICHS, SSE_CENTRAL, SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX = 1, 2, 3
+SSE_SKEW_SYMMETRIC_MAPPING, SSE_STANDARD_MAPPING = 1, 2
+SSE_NO_VISCOUS_FLUX, SSE_BR1 = 0, 1
 SSE_TWO_POINT_EC, SSE_TWO_POINT_CONSERVATIVE = 1, 2
 SSE_MASS_WEIGHT_ADJUSTED, SSE_MASS_DIAGONAL = 1, 2
 SSE_C_NONE, SSE_C_DENSE, SSE_C_SPARSE = 0, 1, 2
@@ -34,10 +36,11 @@ class SseDesc(C.Structure):
         ("N_p", C.c_int32), ("N_q", C.c_int32), ("N_f", C.c_int32), ("N_fac", C.c_int32),
         ("N_c", C.c_int32), ("N_e", C.c_int32), ("law", C.c_int32), ("form", C.c_int32),
         ("inviscid_flux", C.c_int32), ("two_point_flux", C.c_int32), ("mass_solver", C.c_int32),
-        ("S_sparse", C.c_int32), ("C_kind", C.c_int32),
+        ("S_sparse", C.c_int32), ("C_kind", C.c_int32), ("mapping_form", C.c_int32),
+        ("viscous_flux", C.c_int32),
         ("gamma", C.c_double), ("half_lambda", C.c_double), ("a", C.c_double * 3), ("b", C.c_double),
         ("V", _dp), ("R", _dp), ("R_selection", _lp), ("W", _dp), ("B", _dp), ("S", _dp),
-        ("C", _dp), ("Minv", _dp),
+        ("C", _dp), ("Minv", _dp), ("D", _dp),
         ("V_A", _dp), ("V_B", _dp), ("V_C", _dp), ("V_p", C.c_int32), ("V_M", C.c_int32 * 3),
         ("J_q", _dp), ("Lambda_q", _dp), ("nJf", _dp), ("J_f", _dp), ("nJq", _dp),
         ("face_normals", _dp), ("mapP", _lp),

@@ -217,8 +217,44 @@ class InitialDataGaussian:
 
 @dataclass
 class InitialDataSine:
+    """``InitialDataSine(A, k)`` (``src/GridFunctions/GridFunctions.jl:30-38, 124-128``)."""
     A: float
     k: Tuple[float, ...]
 
     def __call__(self, x, t=0.0):
         return (self.A * np.prod([np.sin(self.k[m] * x[m]) for m in range(len(self.k))], axis=0))[..., None]
+
+
+@dataclass
+class InitialDataCosine:
+    """``InitialDataCosine(A, k)`` (``src/GridFunctions/GridFunctions.jl:40-48, 130-134``)."""
+    A: float
+    k: Tuple[float, ...]
+
+    def __call__(self, x, t=0.0):
+        return (self.A * np.prod([np.cos(self.k[m] * x[m]) for m in range(len(self.k))], axis=0))[..., None]
+
+
+@dataclass
+class ExactSolution:
+    """``ExactSolution(conservation_law, initial_data)`` for the linear laws
+    (``src/ConservationLaws/linear_advection_diffusion.jl:130-182``): the initial
+    data translated by a t, times exp(-b |k|^2 t) for a sine under diffusion, or
+    the spreading Gaussian. Not periodic-wrapped, like the reference's default."""
+    conservation_law: object
+    initial_data: object
+
+    def __call__(self, x, t=0.0):
+        law, f = self.conservation_law, self.initial_data
+        a = law.a
+        z = tuple(x[m] - a[m] * t for m in range(len(a)))
+        b = getattr(law, "b", 0.0)
+        if isinstance(f, InitialDataGaussian) and b > 0.0:
+            d = len(a)
+            r2 = sum((z[m] - f.x_0[m]) ** 2 for m in range(d))
+            t0 = f.sigma ** 2 / (2.0 * b)
+            return (f.A * (t0 / (t + t0)) ** (0.5 * d) * np.exp(-r2 / (4.0 * b * (t0 + t))))[..., None]
+        out = f(z)
+        if isinstance(f, InitialDataSine) and b > 0.0:
+            out = out * np.exp(-b * sum(km ** 2 for km in f.k) * t)
+        return out

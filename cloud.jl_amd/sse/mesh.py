@@ -256,8 +256,11 @@ def _warp_coords(xyz, w):
     raise TypeError(w)
 
 
-def warp_mesh(mesh: MeshData, ref, warping) -> MeshData:
-    """Displace the mapping nodes; connectivity is kept (``mesh.jl:23-120``)."""
+def warp_mesh(mesh: MeshData, ref, warping, L: float = 1.0) -> MeshData:
+    """Displace the mapping nodes; connectivity is kept (``mesh.jl:23-120``). A
+    number as ``warping`` is ``DelReyWarping(factor, (L,) * d)`` (``mesh.jl:23-28``)."""
+    if isinstance(warping, (int, float)):
+        warping = DelReyWarping(float(warping), (float(L),) * len(mesh.xyz))
     rd = ref.reference_element
     xyz = _warp_coords(mesh.xyz, warping)
     return replace(mesh, xyz=xyz, xyzq=tuple(rd.Vq @ x for x in xyz),

@@ -392,6 +392,13 @@ class ModalTensor:
 
 
 @dataclass
+class ModalMulti:
+    """Modal basis on a multidimensional (here: Line) quadrature
+    (``src/SpatialDiscretizations/multidimensional.jl:1-41``)."""
+    p: int
+
+
+@dataclass
 class ReferenceMapping:
     J_ref: np.ndarray
     Lambda_ref: np.ndarray  # N_q x d x d
@@ -453,9 +460,23 @@ def _kron(*mats):
     return out
 
 
-def ReferenceApproximationLine(approx_type: NodalTensor, mapping_degree: int = 1,
+def ReferenceApproximationLine(approx_type, mapping_degree: int = 1,
                                volume_quadrature_rule: Optional[QuadratureRule] = None):
-    """NodalTensor on the Line (``tensor_cartesian.jl:1-32``)."""
+    """NodalTensor (``tensor_cartesian.jl:1-32``) or ModalMulti
+    (``multidimensional.jl:1-41``) on the Line."""
+    if isinstance(approx_type, ModalMulti):
+        p = approx_type.p
+        # DefaultQuadrature(2p) on the Line is LG of degree ceil((2p - 1) / 2)
+        # (``quadrature_rules.jl:48-50``)
+        rule = volume_quadrature_rule or LGQuadrature(-(-(2 * p - 1) // 2))
+        rq, wq = quadrature_line(rule)
+        rd = ref_elem_data_line(mapping_degree, rq, wq)
+        VDM = P.vandermonde_1d(p, rq)
+        dVDM = P.grad_vandermonde_1d(p, rq)
+        Vf = P.vandermonde_1d(p, rd.rstf[0])
+        Pr = np.linalg.inv(VDM.T @ (wq[:, None] * VDM)) @ (VDM.T * wq[None, :])
+        return ReferenceApproximation(approx_type, rd, (dVDM @ Pr,), VDM, Vf @ Pr, wq.copy(),
+                                      rd.wf.copy(), None, None, None, (rule,), ())
     rule = volume_quadrature_rule or LGLQuadrature(approx_type.p)
     rq, wq = quadrature_line(rule)
     rd = ref_elem_data_line(mapping_degree, rq, wq)

@@ -49,11 +49,22 @@ def _xy_to_rs(x, y):
     return -l2 + l3 - l1, -l2 - l3 + l1
 
 
-def nodes_tri(n: int):
-    """Warp & blend interpolation nodes of degree n on the bi-unit triangle."""
+def nodes_tri(n: int, alpha=None):
+    """Warp & blend interpolation nodes of degree n on the bi-unit triangle.
+
+    The reference's mapping nodes (StartUpDG's ``RefElemData(Tri(), N)``) are
+    reproduced with blending exponent alpha = 0: at N = 4 this matches its
+    "Advection 2D Energy-Conservative ModalTensor Tri" golden L2 error
+    (test/runtests.jl:38-61) to 2e-16, where the optimised alpha = 0.1001 misses
+    it by 3e-8; for N <= 3 the nodes do not depend on alpha. The optimised
+    exponents (Hesthaven & Warburton, Table 6.1) remain available by argument.
+    """
     if n == 0:
         return np.array([-1.0 / 3.0]), np.array([-1.0 / 3.0])
-    alpha = _ALPHA_TRI[n - 1] if n < 16 else 5.0 / 3.0
+    if alpha is None:
+        alpha = 0.0
+    elif alpha == "optimised":
+        alpha = _ALPHA_TRI[n - 1] if n < 16 else 5.0 / 3.0
     L1, L3 = [], []
     for i in range(1, n + 2):
         for m in range(1, n + 3 - i):

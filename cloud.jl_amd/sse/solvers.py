@@ -25,7 +25,7 @@ from .conservation_laws import (BR1, CentralNumericalFlux, ConservativeFlux,
                                 EntropyConservativeFlux, EntropyConservativeNumericalFlux,
                                 EulerEquations, LaxFriedrichsNumericalFlux,
                                 LinearAdvectionDiffusionEquation, LinearAdvectionEquation)
-from .geometry import SpatialDiscretization
+from .geometry import SpatialDiscretization, apply_reference_mapping
 from .reference_approximation import (LINE, TET, TRI, face_reference_normals,
                                       flux_differencing_operators, num_faces)
 
@@ -102,17 +102,27 @@ def _lptr(a: Optional[np.ndarray]):
 class DescArrays:
     """Host arrays behind one ``sse_desc`` (kept alive while the descriptor is used)."""
 
-    def __init__(self, law, sd: SpatialDiscretization, form, mass_solver):
+    def __init__(self, law, sd: SpatialDiscretization, form, mass_solver, strategy=None):
         ref = sd.reference_approximation
         gf = sd.geometric_factors
         d = ref.d
         self.law = law
         self.sd = sd
         self.form = form
-        if not isinstance(form, FluxDifferencingForm):
-            raise NotImplementedError("the HIP path implements FluxDifferencingForm")
-        S, S_sparse, Cmat, C_kind = flux_differencing_operators(ref)
+        standard = isinstance(form, StandardForm)
+        if not (standard or isinstance(form, FluxDifferencingForm)):
+            raise NotImplementedError(f"unknown residual form {form!r}")
         f64 = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64)
+        if standard:
+            # ReferenceOperators / PhysicalOperators hold the geometric factors after
+            # the reference mapping and the reference derivative operators D
+            # (operators.jl:1-37, 83-160)
+            S, S_sparse, Cmat, C_kind = None, False, None, abi.SSE_C_NONE
+            gf = apply_reference_mapping(gf, ref)
+            self.D = np.asfortranarray(np.stack([np.asarray(Dm, dtype=np.float64) for Dm in ref.D], axis=2))
+        else:
+            S, S_sparse, Cmat, C_kind = flux_differencing_operators(ref)
+            self.D = None
         self.V = f64(ref.V)
         self.R = f64(ref.R)
         self.R_selection = (None if ref.R_selection is None
@@ -120,7 +130,8 @@ class DescArrays:
         self.W = f64(ref.W)
         self.B = f64(ref.B)
         # S[:, :, m] column-major  <->  C-order array (d, N_q, N_q) holding S_m transposed
-        self.S = np.ascontiguousarray(np.stack([np.asfortranarray(S[m]).T for m in range(d)]))
+        self.S = (None if S is None else
+                  np.ascontiguousarray(np.stack([np.asfortranarray(S[m]).T for m in range(d)])))
         self.C = f64(Cmat)
         self.Minv = f64(mass_solver.Minv) if isinstance(mass_solver, WeightAdjustedSolver) else None
         wp = ref.warped
@@ -131,7 +142,7 @@ class DescArrays:
         self.Lambda_q = f64(gf.Lambda_q)
         self.nJf = f64(gf.nJf)
         self.J_f = f64(gf.J_f)
-        self.nJq = f64(gf.nJq)
+        self.nJq = None if standard else f64(gf.nJq)  # only the facet correction uses nJq
         self.face_normals = f64(face_reference_normals(ref))
         self.mapP = np.asfortranarray(sd.mesh.mapP, dtype=np.int64)
         desc = abi.SseDesc()
@@ -153,7 +164,18 @@ class DescArrays:
         if hasattr(law, "a"):
             for m, am in enumerate(law.a):
                 desc.a[m] = am
-        desc.form = abi.SSE_FLUX_DIFFERENCING
+        desc.form = abi.SSE_STANDARD if standard else abi.SSE_FLUX_DIFFERENCING
+        if standard:
+            # the ReferenceOperator algorithm is the skew-symmetric one whatever the
+            # mapping form (standard_form_first_order.jl:16-63); PhysicalOperators
+            # follow the form's mapping (operators.jl:83-160)
+            skew = (isinstance(form.mapping_form, SkewSymmetricMapping)
+                    or (isinstance(strategy, ReferenceOperator) and not law.second_order))
+            desc.mapping_form = abi.SSE_SKEW_SYMMETRIC_MAPPING if skew else abi.SSE_STANDARD_MAPPING
+            if law.second_order:
+                if not isinstance(form.viscous_numerical_flux, BR1):
+                    raise NotImplementedError("second-order laws use the BR1 viscous flux")
+                desc.viscous_flux = abi.SSE_BR1
         flux = form.inviscid_numerical_flux
         if isinstance(flux, LaxFriedrichsNumericalFlux):
             desc.inviscid_flux = abi.SSE_LAX_FRIEDRICHS
@@ -162,8 +184,10 @@ class DescArrays:
             desc.inviscid_flux = abi.SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX
         else:
             desc.inviscid_flux = abi.SSE_CENTRAL
+        # StandardForm's interface flux uses the conservative two-point flux
+        # (numerical_flux!'s default, ConservationLaws.jl:75-86)
         desc.two_point_flux = (abi.SSE_TWO_POINT_CONSERVATIVE
-                               if isinstance(form.two_point_flux, ConservativeFlux)
+                               if standard or isinstance(form.two_point_flux, ConservativeFlux)
                                else abi.SSE_TWO_POINT_EC)
         desc.mass_solver = (abi.SSE_MASS_DIAGONAL if isinstance(mass_solver, DiagonalSolver)
                             else abi.SSE_MASS_WEIGHT_ADJUSTED)
@@ -172,6 +196,7 @@ class DescArrays:
         desc.V, desc.R, desc.R_selection = _fptr(self.V), _fptr(self.R), _lptr(self.R_selection)
         desc.W, desc.B, desc.S, desc.C = _fptr(self.W), _fptr(self.B), _fptr(self.S), _fptr(self.C)
         desc.Minv = _fptr(self.Minv)
+        desc.D = _fptr(self.D)
         desc.V_A, desc.V_B, desc.V_C = _fptr(self.V_A), _fptr(self.V_B), _fptr(self.V_C)
         if wp is not None:
             desc.V_p = wp.p
@@ -203,7 +228,8 @@ class Solver:
         self.strategy = strategy or ReferenceOperator()
         self.mass_solver = mass_solver or default_mass_matrix_solver(spatial_discretization)
         self.parallelism = parallelism or MI355X()
-        self.arrays = DescArrays(conservation_law, spatial_discretization, form, self.mass_solver)
+        self.arrays = DescArrays(conservation_law, spatial_discretization, form, self.mass_solver,
+                                 self.strategy)
         self._handle = None
         self._lib = None
 

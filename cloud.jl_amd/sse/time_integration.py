@@ -50,12 +50,19 @@ class ODESolution:
 
 def solve(ode: ODEProblem, alg=None, dt: float = None, nsteps: int = None) -> ODESolution:
     """``solve(ode, CarpenterKennedy2N54(), dt=dt, adaptive=false)`` from tspan[0]
-    to tspan[1] (or for ``nsteps`` steps)."""
+    to tspan[1] (or for ``nsteps`` steps). As OrdinaryDiffEq does with the end of
+    tspan as a stop time, a last shorter step lands exactly on tspan[1]."""
     if dt is None:
         raise ValueError("fixed-step integration needs dt")
     t0, t1 = ode.tspan
-    if nsteps is None:
-        nsteps = int(round((t1 - t0) / dt))
     u = to_device(ode.u0)
-    rk_steps(ode.p, u, dt, nsteps, t0)
-    return ODESolution(t0 + nsteps * dt, to_host(u))
+    if nsteps is not None:
+        rk_steps(ode.p, u, dt, nsteps, t0)
+        return ODESolution(t0 + nsteps * dt, to_host(u))
+    n = int(np.floor((t1 - t0) / dt * (1.0 + 1e-12)))
+    rk_steps(ode.p, u, dt, n, t0)
+    t = t0 + n * dt
+    rest = t1 - t
+    if rest > 1e-12 * max(abs(t1), dt):
+        rk_steps(ode.p, u, rest, 1, t)
+    return ODESolution(t1, to_host(u))

@@ -5,15 +5,19 @@ import math
 
 import numpy as np
 
-from .conservation_laws import (EntropyConservativeNumericalFlux, EulerEquations,
-                                EulerPeriodicTest, IsentropicVortex,
-                                LaxFriedrichsNumericalFlux, ShuVortex)
+from .conservation_laws import (BR1, CentralNumericalFlux, EntropyConservativeNumericalFlux,
+                                EulerEquations, EulerPeriodicTest, ExactSolution,
+                                InitialDataCosine, InitialDataGaussian, InitialDataSine,
+                                IsentropicVortex, LaxFriedrichsNumericalFlux,
+                                LinearAdvectionDiffusionEquation, LinearAdvectionEquation,
+                                ShuVortex)
 from .geometry import ChanWilcoxMetrics, make_spatial_discretization
 from .mesh import ChanWarping, UniformWarping, uniform_periodic_mesh, warp_mesh
-from .reference_approximation import (LGQuadrature, ModalTensor, NodalTensor,
+from .reference_approximation import (LGQuadrature, ModalMulti, ModalTensor, NodalTensor,
                                       ReferenceApproximationLine, ReferenceApproximationTet,
                                       ReferenceApproximationTri)
-from .solvers import FluxDifferencingForm, semidiscretize
+from .solvers import (FluxDifferencingForm, PhysicalOperator, ReferenceOperator,
+                      SkewSymmetricMapping, StandardForm, StandardMapping, semidiscretize)
 
 
 def euler_1d_gauss():
@@ -71,3 +75,70 @@ def density_wave_3d_config(M=2, p=4, factor=0.05, form=None, ic_strength=0.5):
     sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
     form = form or FluxDifferencingForm(inviscid_numerical_flux=LaxFriedrichsNumericalFlux())
     return law, sd, ic, form, semidiscretize(law, sd, ic, form, (0.0, 1.0))
+
+
+def advection_diffusion_1d_modal_multi():
+    """test/runtests.jl:13-36 (golden L2 6.988216111882884e-6): ModalMulti(4) on 4
+    periodic elements of [0,1], a = 1, b = 5e-2, sine data, StandardForm with
+    StandardMapping, LF + BR1, PhysicalOperator; dt = 1/100 to T = 1."""
+    law = LinearAdvectionDiffusionEquation((1.0,), 5.0e-2)
+    ic = InitialDataSine(1.0, (2 * math.pi,))
+    ref = ReferenceApproximationLine(ModalMulti(4))
+    mesh = uniform_periodic_mesh(ref, (0.0, 1.0), 4)
+    sd = make_spatial_discretization(mesh, ref)
+    form = StandardForm(mapping_form=StandardMapping(),
+                        inviscid_numerical_flux=LaxFriedrichsNumericalFlux(),
+                        viscous_numerical_flux=BR1())
+    ode = semidiscretize(law, sd, ic, form, (0.0, 1.0), PhysicalOperator())
+    return law, sd, ExactSolution(law, ic), form, ode
+
+
+def advection_2d_modal_tensor_tri(M=2, p=4):
+    """test/runtests.jl:38-61 (golden L2 0.2660013939427627): ModalTensor(4) Tri,
+    mapping degree 4, a = (1,1), sine data, StandardForm skew-symmetric with
+    LF(0) (central), ReferenceOperator; DelRey-warped (0.1) 2x2 periodic mesh of
+    [0,1]^2 with exact metrics and projected Jacobian; dt = 1/100 to T = 1."""
+    law = LinearAdvectionEquation((1.0, 1.0))
+    ic = InitialDataSine(1.0, (2 * math.pi, 2 * math.pi))
+    ref = ReferenceApproximationTri(ModalTensor(p), mapping_degree=p)
+    mesh = warp_mesh(uniform_periodic_mesh(ref, ((0.0, 1.0), (0.0, 1.0)), (M, M)), ref, 0.1)
+    sd = make_spatial_discretization(mesh, ref)
+    form = StandardForm(mapping_form=SkewSymmetricMapping(),
+                        inviscid_numerical_flux=LaxFriedrichsNumericalFlux(0.0))
+    ode = semidiscretize(law, sd, ic, form, (0.0, 1.0), ReferenceOperator())
+    return law, sd, ExactSolution(law, ic), form, ode
+
+
+def advection_3d_modal_tensor_tet(M=2, p=4):
+    """test/advection_3d.jl (golden L2 0.1876141674772107 under a Jaskowiec-Sukumar
+    degree-11 error quadrature): ModalTensor(4) Tet, a = (1,1,1), cosine data,
+    StandardForm skew-symmetric with the central flux, ReferenceOperator,
+    DelRey-warped (0.1) M^3 x 6 periodic mesh of [0,1]^3, curl metrics."""
+    law = LinearAdvectionEquation((1.0, 1.0, 1.0))
+    ic = InitialDataCosine(1.0, (2 * math.pi,) * 3)
+    ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=4)
+    mesh = warp_mesh(uniform_periodic_mesh(ref, ((0.0, 1.0),) * 3, (M, M, M)), ref, 0.1, 1.0)
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    form = StandardForm(mapping_form=SkewSymmetricMapping(),
+                        inviscid_numerical_flux=CentralNumericalFlux())
+    ode = semidiscretize(law, sd, ic, form, (0.0, 1.0), ReferenceOperator())
+    return law, sd, ExactSolution(law, ic), form, ode
+
+
+def advection_diffusion_3d_config(M=4, p=5, b=5.0e-3, sigma=0.1, factor=0.05):
+    """3D linear advection-diffusion on curved tets: a = (1, 0.5, 0.25), diffusivity
+    b, BR1, periodic [0,1]^3 with M^3 hexes x 6 tets, UniformWarping, ModalTensor(p)
+    isoparametric with curl metrics, Gaussian pulse of width sigma at the centre;
+    StandardForm (skew-symmetric, LF) with PhysicalOperator."""
+    law = LinearAdvectionDiffusionEquation((1.0, 0.5, 0.25), b)
+    ic = InitialDataGaussian(1.0, sigma, (0.5, 0.5, 0.5))
+    ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=p)
+    mesh = uniform_periodic_mesh(ref, ((0.0, 1.0),) * 3, (M, M, M))
+    if factor:
+        mesh = warp_mesh(mesh, ref, UniformWarping(factor, (1.0, 1.0, 1.0)))
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    form = StandardForm(mapping_form=SkewSymmetricMapping(),
+                        inviscid_numerical_flux=LaxFriedrichsNumericalFlux(),
+                        viscous_numerical_flux=BR1())
+    ode = semidiscretize(law, sd, ic, form, (0.0, 1.0), PhysicalOperator())
+    return law, sd, ExactSolution(law, ic), form, ode

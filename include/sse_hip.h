@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define SSE_ABI_VERSION 1
+#define SSE_ABI_VERSION 2
 
 typedef enum {
     SSE_OK = 0,
@@ -52,6 +52,12 @@ typedef enum {
     SSE_CENTRAL = 2,
     SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX = 3
 } sse_inviscid_flux;
+
+/* mapping forms of StandardForm (src/Solvers/Solvers.jl:74-75; operators.jl:1-37, 83-160) */
+typedef enum { SSE_SKEW_SYMMETRIC_MAPPING = 1, SSE_STANDARD_MAPPING = 2 } sse_mapping_form;
+
+/* viscous numerical fluxes (src/ConservationLaws/ConservationLaws.jl:64-67) */
+typedef enum { SSE_NO_VISCOUS_FLUX = 0, SSE_BR1 = 1 } sse_viscous_flux;
 
 /* two-point fluxes (src/ConservationLaws/ConservationLaws.jl:69-73) */
 typedef enum { SSE_TWO_POINT_EC = 1, SSE_TWO_POINT_CONSERVATIVE = 2 } sse_two_point_flux;
@@ -88,6 +94,8 @@ typedef struct sse_desc {
     int32_t mass_solver;     /* sse_mass_solver */
     int32_t S_sparse;        /* 1: S_m stored as SparseMatrixCSC in the reference */
     int32_t C_kind;          /* sse_c_kind */
+    int32_t mapping_form;    /* sse_mapping_form (StandardForm only) */
+    int32_t viscous_flux;    /* sse_viscous_flux (second-order laws only) */
     double gamma;            /* Euler: ratio of specific heats */
     double half_lambda;      /* LaxFriedrichsNumericalFlux.halfλ */
     double a[3];             /* advection velocity */
@@ -103,6 +111,8 @@ typedef struct sse_desc {
     const double* S;            /* N_q x N_q x d, S[:, :, m] = 1/2 (W D_m - D_m' W) */
     const double* C;            /* N_q x N_f, R' B (NULL when C_kind == SSE_C_NONE) */
     const double* Minv;         /* N_p x N_p weight-adjusted M^{-1}; NULL = identity */
+    const double* D;            /* N_q x N_q x d reference derivative operators D[:, :, m] = D_m
+                                   (ReferenceApproximation.D; StandardForm only, else NULL) */
 
     /* Optional sum factorisation of V: the fields of the reference's
      * WarpedTensorProductMap2D/3D (src/MatrixFreeOperators/warped_product_2d.jl:2-23,
@@ -115,7 +125,9 @@ typedef struct sse_desc {
     int32_t V_p;
     int32_t V_M[3];             /* M1, M2, M3: 1D volume quadrature sizes */
 
-    /* geometric factors (GeometricFactors, SpatialDiscretizations.jl:283-289) */
+    /* geometric factors (GeometricFactors, SpatialDiscretizations.jl:283-289). For
+     * StandardForm, Lambda_q, nJf and J_f are those after apply_reference_mapping
+     * (SpatialDiscretizations.jl:398-411), as ReferenceOperators/PhysicalOperators hold them. */
     const double* J_q;       /* N_q x N_e */
     const double* Lambda_q;  /* N_q x d x d x N_e, J dxi_l/dx_m */
     const double* nJf;       /* d x N_f x N_e */

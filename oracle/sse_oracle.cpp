@@ -17,6 +17,10 @@
 //   logmean, inv_logmean                       src/ConservationLaws/ConservationLaws.jl:132-156
 //   Euler entropy variables, wave speed, EC flux src/ConservationLaws/euler_navierstokes.jl:93-225
 //   linear advection two-point flux            src/ConservationLaws/linear_advection_diffusion.jl:106-128
+//   StandardForm nodal values / time deriv.   src/Solvers/standard_form_first_order.jl:1-94
+//   BR1 auxiliary variable / time deriv.      src/Solvers/standard_form_second_order.jl:1-75
+//   volume operators (skew / standard mapping) src/Solvers/operators.jl:1-37, 83-160
+//   BR1 numerical fluxes                       src/ConservationLaws/linear_advection_diffusion.jl:54-102
 //   CarpenterKennedy2N54 (OrdinaryDiffEq)      used at test/test_driver.jl:221-227
 //
 // The restatement is templated on the scalar type so that a dual-number
@@ -208,6 +212,9 @@ struct Oracle {
     vector<T> u_q;            // N_q x N_c x N_e
     vector<T> u_f;            // N_f x N_e x N_c ("switched order")
     vector<int64_t> mapP0;    // 0-based
+    bool second_order = false;
+    vector<T> q_q;            // N_q x N_c x d x N_e (auxiliary variable, SecondOrder)
+    vector<T> q_f;            // N_f x N_e x N_c x d
 
     std::string init(const sse_desc* desc) {
         D = *desc;
@@ -217,7 +224,13 @@ struct Oracle {
         law.kind = D.law; law.d = d; law.N_c = N_c;
         law.gamma = D.gamma; law.gm1 = D.gamma - 1.0; law.inv_gm1 = 1.0 / (D.gamma - 1.0);
         for (int m = 0; m < 3; ++m) law.a[m] = D.a[m];
-        if (D.form != SSE_FLUX_DIFFERENCING) return "oracle: only FluxDifferencingForm";
+        if (D.form == SSE_STANDARD) {
+            if (!D.D) return "StandardForm needs the derivative operators D";
+            second_order = (D.law == SSE_LINEAR_ADVECTION_DIFFUSION);
+            if (second_order && D.viscous_flux != SSE_BR1) return "second-order laws need BR1";
+        } else if (D.form != SSE_FLUX_DIFFERENCING) {
+            return "unknown residual form";
+        }
         if (D.S_sparse)
             for (int m = 0; m < d; ++m) S_csc.push_back(to_csc(D.S + (size_t)N_q * N_q * m, N_q, N_q));
         if (D.C_kind == SSE_C_SPARSE) C_csc = to_csc(D.C, N_q, N_f);
@@ -254,6 +267,10 @@ struct Oracle {
             }
         u_q.assign((size_t)N_q * N_c * N_e, T(0.0));
         u_f.assign((size_t)N_f * N_e * N_c, T(0.0));
+        if (second_order) {
+            q_q.assign((size_t)N_q * N_c * d * N_e, T(0.0));
+            q_f.assign((size_t)N_f * N_e * N_c * d, T(0.0));
+        }
         mapP0.resize((size_t)N_f * N_e);
         for (size_t i = 0; i < mapP0.size(); ++i) mapP0[i] = D.mapP[i] - 1;
         return "";
@@ -487,10 +504,189 @@ struct Oracle {
         mass_solve(k, dk, temp);
     }
 
+    // ------------------------------------------------------------------ StandardForm
+    // y = D_m x, y = D_m' x (N_q x N_c)
+    void applyD(int m, const T* x, T* y) const { matmul(D.D + (size_t)N_q * N_q * m, N_q, N_q, x, N_q, y, N_q); }
+    void applyDt(int m, const T* x, T* y) const { matmul_t(D.D + (size_t)N_q * N_q * m, N_q, N_q, x, N_q, y, N_q); }
+
+    // r += (volume operator of direction n) x, before V' and the mass solve:
+    //   SkewSymmetricMapping  sum_m D_m' (1/2 W Lambda_mn x) - 1/2 W Lambda_mn (D_m x)
+    //                         + R' (1/2 B nJf_n (R x))        (operators.jl:132-160)
+    //   StandardMapping       sum_m D_m' (W Lambda_mn x)        (operators.jl:83-130)
+    void volume_term(int k, int n, const T* x, T* r, T* t1, T* t2, T* tf) const {
+        const double* Lk = D.Lambda_q + (size_t)N_q * d * d * k;
+        const bool skew = (D.mapping_form != SSE_STANDARD_MAPPING);
+        for (int m = 0; m < d; ++m) {
+            const double* Lmn = Lk + (size_t)N_q * (m + d * n);
+            const double h = skew ? 0.5 : 1.0;
+            for (int e = 0; e < N_c; ++e)
+                for (int i = 0; i < N_q; ++i) t1[i + (size_t)N_q * e] = h * D.W[i] * Lmn[i] * x[i + (size_t)N_q * e];
+            applyDt(m, t1, t2);
+            for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] += t2[i];
+            if (skew) {
+                applyD(m, x, t2);
+                for (int e = 0; e < N_c; ++e)
+                    for (int i = 0; i < N_q; ++i)
+                        r[i + (size_t)N_q * e] -= 0.5 * D.W[i] * Lmn[i] * t2[i + (size_t)N_q * e];
+            }
+        }
+        if (skew) {
+            applyR(x, tf, N_f);
+            for (int e = 0; e < N_c; ++e)
+                for (int j = 0; j < N_f; ++j)
+                    tf[j + (size_t)N_f * e] *= 0.5 * D.B[j] * D.nJf[n + d * (j + (size_t)N_f * k)];
+            applyRt(tf, t2);
+            for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] += t2[i];
+        }
+    }
+
+    // nodal_values! for StandardForm (standard_form_first_order.jl:1-14)
+    void nodal_values_standard(const T* u, int k, T* w_f) {
+        const T* uk = u + (size_t)N_p * N_c * k;
+        T* uqk = u_q.data() + (size_t)N_q * N_c * k;
+        applyV(uk, uqk);
+        applyR(uqk, w_f, N_f);
+        for (int e = 0; e < N_c; ++e)
+            for (int j = 0; j < N_f; ++j) u_f[j + (size_t)N_f * (k + (size_t)N_e * e)] = w_f[j + (size_t)N_f * e];
+    }
+
+    // interface flux f* (N_f x N_c) at element k: numerical_flux! with the
+    // inviscid flux and the conservative two-point flux (ConservationLaws.jl:75-128)
+    void inviscid_interface(int k, T* f_f) const {
+        T uin[5], uout[5], F[15];
+        for (int j = 0; j < N_f; ++j) {
+            int64_t g = mapP0[j + (size_t)N_f * k];
+            int jn = (int)(g % N_f), kn = (int)(g / N_f);
+            for (int e = 0; e < N_c; ++e) {
+                uin[e] = u_f[j + (size_t)N_f * (k + (size_t)N_e * e)];
+                uout[e] = u_f[jn + (size_t)N_f * (kn + (size_t)N_e * e)];
+            }
+            law.two_point(D.two_point_flux, uin, uout, F);
+            const double* nf = n_f.data() + d * (j + (size_t)N_f * k);
+            T a = 0.0;
+            if (D.inviscid_flux == SSE_LAX_FRIEDRICHS) a = D.half_lambda * law.wave_speed(uin, uout, nf);
+            for (int e = 0; e < N_c; ++e) {
+                T fn = 0.0;
+                for (int m = 0; m < d; ++m) fn += F[e + N_c * m] * nf[m];
+                if (D.inviscid_flux == SSE_LAX_FRIEDRICHS) fn += a * (uin[e] - uout[e]);
+                f_f[j + (size_t)N_f * e] = fn;
+            }
+        }
+    }
+
+    // auxiliary_variable! (standard_form_second_order.jl:3-32): for each m,
+    // q_m = M^{-1} V' [-(volume term m)(u_q) + R' (B J_f u*_m)], u*_m = {u} n_m (BR1,
+    // linear_advection_diffusion.jl:75-86); q_q = V q_m, q_f = R q_q
+    void auxiliary_variable(int k, T* r, T* t1, T* t2, T* tf, T* x) {
+        const T* uqk = u_q.data() + (size_t)N_q * N_c * k;
+        for (int m = 0; m < d; ++m) {
+            for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] = 0.0;
+            volume_term(k, m, uqk, r, t1, t2, tf);
+            for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] = -r[i];
+            for (int j = 0; j < N_f; ++j) {
+                int64_t g = mapP0[j + (size_t)N_f * k];
+                int jn = (int)(g % N_f), kn = (int)(g / N_f);
+                const double nm = n_f[m + d * (j + (size_t)N_f * k)];
+                for (int e = 0; e < N_c; ++e) {
+                    T avg = 0.5 * (u_f[j + (size_t)N_f * (k + (size_t)N_e * e)] +
+                                   u_f[jn + (size_t)N_f * (kn + (size_t)N_e * e)]);
+                    tf[j + (size_t)N_f * e] = avg * nm * BJf[j + (size_t)N_f * k];
+                }
+            }
+            applyRt(tf, t2);
+            for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] += t2[i];
+            applyVt(r, x);
+            mass_solve(k, x, t1);
+            T* qq = q_q.data() + (size_t)N_q * N_c * (m + (size_t)d * k);
+            applyV(x, qq);
+            applyR(qq, tf, N_f);
+            for (int e = 0; e < N_c; ++e)
+                for (int j = 0; j < N_f; ++j)
+                    q_f[j + (size_t)N_f * (k + (size_t)N_e * (e + (size_t)N_c * m))] = tf[j + (size_t)N_f * e];
+        }
+    }
+
+    // time_derivative! for StandardForm: first order (standard_form_first_order.jl:16-94)
+    // and second order (standard_form_second_order.jl:36-75), as
+    //   dudt = M^{-1} V' [ sum_n (volume term n)(f_n) - R' (B J_f f*) ]
+    void time_derivative_standard(T* dudt, int k, T* f_f, T* r, T* t1, T* t2, T* tf, T* fq) {
+        const T* uqk = u_q.data() + (size_t)N_q * N_c * k;
+        inviscid_interface(k, f_f);
+        if (second_order) {  // BR1 viscous flux (linear_advection_diffusion.jl:88-102)
+            for (int j = 0; j < N_f; ++j) {
+                int64_t g = mapP0[j + (size_t)N_f * k];
+                int jn = (int)(g % N_f), kn = (int)(g / N_f);
+                for (int e = 0; e < N_c; ++e) {
+                    T s = 0.0;
+                    for (int m = 0; m < d; ++m) {
+                        T qin = q_f[j + (size_t)N_f * (k + (size_t)N_e * (e + (size_t)N_c * m))];
+                        T qout = q_f[jn + (size_t)N_f * (kn + (size_t)N_e * (e + (size_t)N_c * m))];
+                        s += D.b * (-0.5 * (qin + qout)) * n_f[m + d * (j + (size_t)N_f * k)];
+                    }
+                    f_f[j + (size_t)N_f * e] += s;
+                }
+            }
+        }
+        for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] = 0.0;
+        T ub[5], F[15];
+        for (int n = 0; n < d; ++n) {
+            // f_n at the volume nodes (physical_flux!)
+            for (int i = 0; i < N_q; ++i) {
+                if (D.law == SSE_EULER) {
+                    for (int e = 0; e < N_c; ++e) ub[e] = uqk[i + (size_t)N_q * e];
+                    law.physical_flux(ub, F);
+                    for (int e = 0; e < N_c; ++e) fq[i + (size_t)N_q * e] = F[e + N_c * n];
+                } else {
+                    T f = D.a[n] * uqk[i];
+                    if (second_order) f -= D.b * q_q[i + (size_t)N_q * N_c * (n + (size_t)d * k)];
+                    fq[i] = f;
+                }
+            }
+            volume_term(k, n, fq, r, t1, t2, tf);
+        }
+        for (int e = 0; e < N_c; ++e)
+            for (int j = 0; j < N_f; ++j) f_f[j + (size_t)N_f * e] *= BJf[j + (size_t)N_f * k];
+        applyRt(f_f, t2);
+        for (size_t i = 0; i < (size_t)N_q * N_c; ++i) r[i] -= t2[i];
+        T* dk = dudt + (size_t)N_p * N_c * k;
+        applyVt(r, dk);
+        mass_solve(k, dk, t1);
+    }
+
+    void residual_standard(const T* u, T* dudt) {
+#pragma omp parallel
+        {
+            vector<T> w_f((size_t)N_f * N_c);
+#pragma omp for schedule(static)
+            for (int k = 0; k < N_e; ++k) nodal_values_standard(u, k, w_f.data());
+        }
+        if (second_order) {
+#pragma omp parallel
+            {
+                vector<T> r((size_t)N_q * N_c), t1((size_t)N_q * N_c), t2((size_t)N_q * N_c),
+                    tf((size_t)N_f * N_c), x((size_t)N_p * N_c);
+#pragma omp for schedule(static)
+                for (int k = 0; k < N_e; ++k) auxiliary_variable(k, r.data(), t1.data(), t2.data(), tf.data(), x.data());
+            }
+        }
+#pragma omp parallel
+        {
+            vector<T> f_f((size_t)N_f * N_c), r((size_t)N_q * N_c), t1((size_t)N_q * N_c), t2((size_t)N_q * N_c),
+                tf((size_t)N_f * N_c), fq((size_t)N_q * N_c);
+#pragma omp for schedule(static)
+            for (int k = 0; k < N_e; ++k)
+                time_derivative_standard(dudt, k, f_f.data(), r.data(), t1.data(), t2.data(), tf.data(), fq.data());
+        }
+    }
+
     void residual(const T* u, T* dudt, int nthreads) {
 #ifdef _OPENMP
         if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
+        if (D.form == SSE_STANDARD) {
+            residual_standard(u, dudt);
+            return;
+        }
 #pragma omp parallel
         {
             vector<T> w_q((size_t)N_q * N_c), w_f((size_t)N_f * N_c), w((size_t)N_p * N_c);

@@ -10,7 +10,9 @@ import math
 import numpy as np
 import pytest
 
-from cases import (density_wave_3d_config, euler_1d_gauss, euler_vortex_2d_modal, l2_error,
+from cases import (advection_2d_modal_tensor_tri, advection_3d_modal_tensor_tet,
+                   advection_diffusion_1d_modal_multi, advection_diffusion_3d_config,
+                   density_wave_3d_config, euler_1d_gauss, euler_vortex_2d_modal, l2_error,
                    vortex_2d_config)
 from oracle import oracle as O
 
@@ -210,3 +212,68 @@ def test_full_size_entropy_conservation_with_ec_interface_flux():
     u = _perturbed(ode)
     rate, scale = _entropy_rate(sd, law, u, _gpu_residual(ode, u))
     assert abs(rate) <= 1e-12 * scale, (rate, scale)
+
+
+
+# ----------------------------------------------------------------------------- StandardForm
+def _euler_tri_standard():
+    from sse.solvers import SkewSymmetricMapping, StandardForm
+    return vortex_2d_config(M=4, p=3, form=StandardForm(mapping_form=SkewSymmetricMapping()))[4]
+
+
+def _advdiff_standard_mapping():
+    from sse.conservation_laws import BR1, LaxFriedrichsNumericalFlux, LinearAdvectionDiffusionEquation
+    from sse.solvers import PhysicalOperator, StandardForm, StandardMapping, semidiscretize
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=2, p=3, b=2e-2, sigma=0.2)
+    form = StandardForm(mapping_form=StandardMapping(), inviscid_numerical_flux=LaxFriedrichsNumericalFlux(),
+                        viscous_numerical_flux=BR1())
+    return semidiscretize(law, sd, ex.initial_data, form, (0.0, 1.0), PhysicalOperator())
+
+
+STD_CASES = {
+    "advdiff1d_modal_multi_br1": lambda: advection_diffusion_1d_modal_multi()[4],
+    "adv2d_tri_skew_reference_operator": lambda: advection_2d_modal_tensor_tri()[4],
+    "adv3d_tet_skew_central": lambda: advection_3d_modal_tensor_tet()[4],
+    "advdiff3d_tet_p5_br1": lambda: advection_diffusion_3d_config(M=2, p=5)[4],
+    "advdiff3d_tet_p3_standard_mapping": _advdiff_standard_mapping,
+    "euler2d_tri_standard_skew": _euler_tri_standard,
+}
+
+
+@pytest.mark.parametrize("name", sorted(STD_CASES))
+def test_standard_form_residual_matches_oracle(name):
+    ode = STD_CASES[name]()
+    u = _perturbed(ode, amp=1e-3 if name.startswith("euler") else 1e-2)  # Euler states stay physical
+    got = _gpu_residual(ode, u)
+    ref = O.residual(ode.p.arrays, u)
+    assert _rel(got, ref) <= RES_TOL, (name, _rel(got, ref))
+    assert ode.p.describe().startswith("standard"), ode.p.describe()
+
+
+def test_standard_form_br1_rk_steps_match_oracle():
+    from sse.time_integration import rk_steps, to_device, to_host
+    ode = advection_diffusion_3d_config(M=2, p=4)[4]
+    dt = 1e-3
+    ud = to_device(ode.u0)
+    rk_steps(ode.p, ud, dt, 50)
+    got = to_host(ud)
+    ref = O.rk_steps(ode.p.arrays, ode.u0, dt, 50)
+    assert _rel(got, ref) <= RK_TOL, _rel(got, ref)
+
+
+def test_gpu_reproduces_reference_golden_advection_diffusion_1d():
+    # test/runtests.jl:13-36, whole trajectory on the GPU
+    from sse.time_integration import CarpenterKennedy2N54, solve
+    law, sd, exact, form, ode = advection_diffusion_1d_modal_multi()
+    sol = solve(ode, CarpenterKennedy2N54(), dt=1.0 / 100)
+    err = l2_error(sd, sol.u, exact, 1.0)
+    np.testing.assert_allclose(err, [6.988216111882884e-6], rtol=0, atol=1e-10)
+
+
+def test_gpu_reproduces_reference_golden_advection_2d_tri():
+    # test/runtests.jl:38-61, whole trajectory on the GPU
+    from sse.time_integration import CarpenterKennedy2N54, solve
+    law, sd, exact, form, ode = advection_2d_modal_tensor_tri()
+    sol = solve(ode, CarpenterKennedy2N54(), dt=1.0 / 100)
+    err = l2_error(sd, sol.u, exact, 1.0)
+    np.testing.assert_allclose(err, [0.2660013939427627], rtol=0, atol=1e-10)

@@ -6,7 +6,8 @@ Both scenarios are whole trajectories (mesh -> discretisation -> semidiscretize
 
 import numpy as np
 
-from cases import euler_1d_gauss, euler_vortex_2d_modal, l2_error
+from cases import (advection_2d_modal_tensor_tri, advection_diffusion_1d_modal_multi,
+                   euler_1d_gauss, euler_vortex_2d_modal, l2_error)
 from oracle import oracle as O
 
 TOL = 1.0e-10
@@ -41,3 +42,20 @@ def test_euler_vortex_2d_modal_conservation():
         return np.einsum("qk,qp,pck->c", WJ, ref.V, u)
 
     np.testing.assert_allclose(mass(uT) - mass(ode.u0), 0.0, atol=TOL)
+
+
+def test_advection_diffusion_1d_modal_multi_br1_golden():
+    # test/runtests.jl:13-36: StandardForm, StandardMapping, LF + BR1, PhysicalOperator
+    law, sd, exact, form, ode = advection_diffusion_1d_modal_multi()
+    uT = O.rk_steps(ode.p.arrays, ode.u0, 1.0 / 100, 100)
+    err = l2_error(sd, uT, exact, 1.0)
+    np.testing.assert_allclose(err, [6.988216111882884e-6], rtol=0, atol=TOL)
+
+
+def test_advection_2d_modal_tensor_tri_energy_conservative_golden():
+    # test/runtests.jl:38-61: StandardForm, skew-symmetric, central, ReferenceOperator,
+    # DelRey-warped isoparametric degree-4 mesh with exact metrics
+    law, sd, exact, form, ode = advection_2d_modal_tensor_tri()
+    uT = O.rk_steps(ode.p.arrays, ode.u0, 1.0 / 100, 100)
+    err = l2_error(sd, uT, exact, 1.0)
+    np.testing.assert_allclose(err, [0.2660013939427627], rtol=0, atol=TOL)

@@ -28,7 +28,9 @@ import mpmath
 import numpy as np
 import pytest
 
-from cases import density_wave_3d_config, l2_error, vortex_2d_config
+from cases import (advection_2d_modal_tensor_tri, advection_3d_modal_tensor_tet,
+                   advection_diffusion_3d_config, density_wave_3d_config, energy_rate, l2_error,
+                   vortex_2d_config)
 from oracle import oracle as O
 from sse.conservation_laws import (EntropyConservativeNumericalFlux, EulerEquations,
                                    LaxFriedrichsNumericalFlux, conservative_to_entropy,
@@ -299,3 +301,43 @@ def test_tri_entropy_conservation_with_ec_interface_flux():
     u = _perturbed(ode.u0)
     rate, scale = _entropy_rate(sd, law, u, O.residual(ode.p.arrays, u))
     assert abs(rate) < 1e-12 * scale
+
+
+# ----------------------------------------------------------------------------- StandardForm
+@pytest.mark.parametrize("case", ["tri", "tet"])
+def test_standard_form_skew_central_conserves_energy(case):
+    # the skew-symmetric split form with the central flux conserves the discrete
+    # energy on curved meshes (the reference's "energy ~ 0" checks,
+    # test/runtests.jl:58-60, test/advection_3d.jl)
+    law, sd, ex, form, ode = (advection_2d_modal_tensor_tri() if case == "tri"
+                              else advection_3d_modal_tensor_tet())
+    u = np.asfortranarray(ode.u0 + 0.01 * np.random.default_rng(3).standard_normal(ode.u0.shape))
+    rate, scale = energy_rate(sd, u, O.residual(ode.p.arrays, u))
+    assert abs(rate) < 1e-12 * scale, (rate, scale)
+
+
+def test_standard_form_br1_dissipates_energy_and_conserves_mass():
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=2, p=3, b=5e-2, sigma=0.25)
+    ref = sd.reference_approximation
+    u = ode.u0
+    dudt = O.residual(ode.p.arrays, u)
+    rate, scale = energy_rate(sd, u, dudt)
+    assert rate < -1e-6 * scale
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q
+    mass = np.einsum("qk,qp,pk->", WJ, ref.Vdense(), dudt[:, 0, :])
+    mscale = np.einsum("qk,qp,pk->", WJ, np.abs(ref.Vdense()), np.abs(dudt[:, 0, :]))
+    assert abs(mass) < 1e-13 * mscale
+
+
+def test_standard_form_br1_converges_to_exact_gaussian():
+    # advection-diffusion of a Gaussian on curved tets against the closed form of
+    # src/ConservationLaws/linear_advection_diffusion.jl:146-163 (pulse away from the
+    # periodic boundary over the short horizon)
+    T = 0.05
+    errs = []
+    for M in (2, 3):
+        law, sd, ex, form, ode = advection_diffusion_3d_config(M=M, p=4, b=1e-2, sigma=0.15)
+        n = 10 * M
+        uT = O.rk_steps(ode.p.arrays, ode.u0, T / n, n)
+        errs.append(l2_error(sd, uT, ex, T)[0])
+    assert errs[1] < 0.7 * errs[0] and errs[1] < 0.02, errs
