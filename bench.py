@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (os.path.join(ROOT, "cloud.jl_amd"), ROOT):
     if _p not in sys.path:
@@ -65,6 +67,62 @@ def load_traffic(workload: str, kernel: str):
         return None
 
 
+def bench_batched(args):
+    """256 independent 2D Euler vortex problems of uneven size (M in 4..32, p in 2..6),
+    200 CK2N54 steps each: one batched, graph-captured run against a loop of
+    per-problem GPU solves. Prints one JSON line."""
+    import torch
+
+    from sse.batch import BatchedSolver
+    from sse.time_integration import rk_steps, to_device
+    from sse.workloads import batched_vortex_problems
+
+    nsteps = args.batch_steps
+    t0 = time.perf_counter()
+    probs = batched_vortex_problems(n=args.batch_problems, progress=log)
+    dof = sum(int(np.prod(ode.p.size())) for ode, _, _ in probs)
+    log(f"prep: {time.perf_counter() - t0:.1f} s, {len(probs)} problems, {dof} DOF")
+    dts = [dt for _, dt, _ in probs]
+    # loop of plain per-problem solves (handles created and warmed up first)
+    loop_states = [to_device(ode.u0) for ode, _, _ in probs]
+    for (ode, dt, _), u in zip(probs, loop_states):
+        rk_steps(ode.p, u, dt, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (ode, dt, _), u in zip(probs, loop_states):  # one solve after the other, each to completion
+        rk_steps(ode.p, u, dt, nsteps)
+        ode.p.lib.sse_synchronize(ode.p.handle)
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    log(f"loop of {len(probs)} solves: {t_loop:.3f} s")
+    batch = BatchedSolver([ode.p for ode, _, _ in probs])
+    log(batch.describe()[:400])
+    states = [to_device(ode.u0) for ode, _, _ in probs]
+    batch.rk_steps(states, dts, 2)  # warm-up: graph capture
+    states = [to_device(ode.u0) for ode, _, _ in probs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.rk_steps(states, dts, nsteps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log(f"batched: {t_batch:.3f} s")
+    # both paths advanced the same problems from the same states (loop: 1 + nsteps steps)
+    line = {
+        "metric": "batched vs loop speedup (256 2D Euler vortex problems, 200 RK steps)",
+        "value": t_loop / t_batch, "unit": "x", "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": 1e3 * t_batch / nsteps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 independent 2D Euler Shu-vortex problems, periodic [0,10]^2, "
+                               "M x M squares x 2 tris with M in 4..32, ModalTensor p in 2..6, "
+                               "FluxDifferencingForm EC + LF, CK2N54", "problems": len(probs), "DOF": dof,
+                   "t_loop_s": t_loop, "t_batched_s": t_batch,
+                   "solves_per_s_batched": len(probs) / t_batch, "solves_per_s_loop": len(probs) / t_loop,
+                   "rk_steps_per_s_batched": nsteps / t_batch,
+                   "dof_residuals_per_s_batched": dof * 5 * nsteps / t_batch},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +132,11 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="residual", choices=["residual", "batched"],
+                    help="residual: the curved-tet Euler p=4 residual (default); batched: the "
+                         "batched-vs-loop comparison over 256 vortex problems")
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, run 5 copy kernels (dst = 1.0 * src) over this many MiB "
                          "(a kernel of known traffic for calibrating HBM byte counters)")
@@ -84,7 +147,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.config == "batched":
+        return bench_batched(args)
+
     import torch
 
     from sse import abi, semi_discrete_residual

@@ -12,6 +12,8 @@
 #include <array>
 #include <cstdlib>
 #include <cstring>
+#include <cstddef>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -851,6 +853,267 @@ void sse_destroy(sse_handle* h) {
     for (void* p : h->allocations) (void)hipFree(p);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched problems
+struct sse_batch {
+    std::vector<sse_handle*> groups;
+    std::vector<std::vector<int>> members;          // problem indices of each group, in element order
+    std::vector<std::vector<long long>> offsets;    // element offset of each member inside its group
+    std::vector<size_t> elems;                      // N_e of each problem
+    std::vector<size_t> per_elem;                   // N_p * N_c of each problem
+    std::vector<double*> state, tmp, dt_e;          // per group, device
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+int bfail(sse_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    else g_batch_create_error = msg;
+    return code;
+}
+
+bool same_array(const void* a, const void* b, size_t bytes) {
+    if (!a || !b) return a == b;
+    return std::memcmp(a, b, bytes) == 0;
+}
+
+// true when problems a and b can share one handle (same reference element and
+// operators, law, form and fluxes; geometry may differ)
+bool same_discretisation(const sse_desc* a, const sse_desc* b) {
+    const int32_t* ia = &a->abi_version;
+    const int32_t* ib = &b->abi_version;
+    // integer header fields up to viscous_flux, except N_e
+    const int n_int = (int)(offsetof(sse_desc, gamma) / sizeof(int32_t));
+    for (int t = 0; t < n_int; ++t) {
+        if ((const char*)(ia + t) == (const char*)&a->N_e) continue;
+        if (ia[t] != ib[t]) return false;
+    }
+    if (a->gamma != b->gamma || a->half_lambda != b->half_lambda || a->b != b->b) return false;
+    for (int m = 0; m < 3; ++m)
+        if (a->a[m] != b->a[m]) return false;
+    if (a->V_p != b->V_p) return false;
+    for (int m = 0; m < 3; ++m)
+        if (a->V_M[m] != b->V_M[m]) return false;
+    const size_t Nq = a->N_q, Np = a->N_p, Nf = a->N_f, d = a->d, q1 = a->V_p + 1;
+    const size_t dd = sizeof(double);
+    return same_array(a->V, b->V, Nq * Np * dd) && same_array(a->R, b->R, Nf * Nq * dd) &&
+           same_array(a->R_selection, b->R_selection, Nf * sizeof(int64_t)) && same_array(a->W, b->W, Nq * dd) &&
+           same_array(a->B, b->B, Nf * dd) && same_array(a->S, b->S, Nq * Nq * d * dd) &&
+           same_array(a->C, b->C, Nq * Nf * dd) && same_array(a->Minv, b->Minv, Np * Np * dd) &&
+           same_array(a->D, b->D, Nq * Nq * d * dd) &&
+           same_array(a->V_A, b->V_A, (size_t)a->V_M[0] * q1 * dd) &&
+           same_array(a->V_B, b->V_B, (size_t)a->V_M[1] * q1 * q1 * dd) &&
+           same_array(a->V_C, b->V_C, (size_t)a->V_M[2] * q1 * q1 * q1 * dd) &&
+           same_array(a->face_normals, b->face_normals, d * a->N_fac * dd) &&
+           ((a->nJq == nullptr) == (b->nJq == nullptr));
+}
+
+template <typename T>
+void append(std::vector<T>& dst, const T* src, size_t n) {
+    dst.insert(dst.end(), src, src + n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sse_batch_create(const sse_desc* const* descs, int32_t n, sse_batch** out) {
+    if (!out) return bfail(nullptr, SSE_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!descs || n <= 0) return bfail(nullptr, SSE_ERR_INVALID_ARGUMENT, "need at least one descriptor");
+    for (int i = 0; i < n; ++i) {
+        std::string msg;
+        int code = validate(descs[i], msg);
+        if (code) return bfail(nullptr, code, "problem " + std::to_string(i) + ": " + msg);
+    }
+    sse_batch* b = new sse_batch();
+    auto cleanup = [&](int code, const std::string& msg) {
+        g_batch_create_error = msg;
+        sse_batch_destroy(b);
+        return code;
+    };
+    // group problems by discretisation, in order of first appearance
+    std::vector<int> rep;  // representative problem of each group
+    for (int i = 0; i < n; ++i) {
+        int g = -1;
+        for (size_t t = 0; t < rep.size(); ++t)
+            if (same_discretisation(descs[rep[t]], descs[i])) { g = (int)t; break; }
+        if (g < 0) {
+            rep.push_back(i);
+            b->members.emplace_back();
+            b->offsets.emplace_back();
+            g = (int)rep.size() - 1;
+        }
+        b->members[g].push_back(i);
+    }
+    b->elems.resize(n);
+    b->per_elem.resize(n);
+    for (int i = 0; i < n; ++i) {
+        b->elems[i] = descs[i]->N_e;
+        b->per_elem[i] = (size_t)descs[i]->N_p * descs[i]->N_c;
+    }
+    if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup(SSE_ERR_HIP, "hipStreamCreate failed");
+    for (size_t g = 0; g < rep.size(); ++g) {
+        const sse_desc* R = descs[rep[g]];
+        sse_desc C = *R;
+        const size_t Nq = R->N_q, Nf = R->N_f, d = R->d, Nfac = R->N_fac;
+        std::vector<double> Jq, Lam, nJf, Jf, nJq;
+        std::vector<int64_t> mapP;
+        long long off = 0;
+        for (int i : b->members[g]) {
+            const sse_desc* D = descs[i];
+            const size_t Ne = D->N_e;
+            b->offsets[g].push_back(off);
+            append(Jq, D->J_q, Nq * Ne);
+            append(Lam, D->Lambda_q, Nq * d * d * Ne);
+            append(nJf, D->nJf, d * Nf * Ne);
+            append(Jf, D->J_f, Nf * Ne);
+            if (D->nJq) append(nJq, D->nJq, d * Nfac * Nq * Ne);
+            for (size_t t = 0; t < Nf * Ne; ++t) mapP.push_back(D->mapP[t] + (int64_t)Nf * off);
+            off += (long long)Ne;
+        }
+        if (off > 2147483647LL) return cleanup(SSE_ERR_UNSUPPORTED, "group too large");
+        C.N_e = (int32_t)off;
+        C.J_q = Jq.data();
+        C.Lambda_q = Lam.data();
+        C.nJf = nJf.data();
+        C.J_f = Jf.data();
+        C.nJq = R->nJq ? nJq.data() : nullptr;
+        C.mapP = mapP.data();
+        sse_handle* h = nullptr;
+        int code = sse_create(&C, &h);
+        if (code) return cleanup(code, "group " + std::to_string(g) + ": " + g_create_error);
+        b->groups.push_back(h);
+        sse_set_stream(h, b->stream);
+        const size_t nstate = (size_t)C.N_p * C.N_c * C.N_e;
+        double *st = nullptr, *tm = nullptr, *de = nullptr;
+        if (hipMalloc(&st, nstate * sizeof(double)) != hipSuccess ||
+            hipMalloc(&tm, nstate * sizeof(double)) != hipSuccess ||
+            hipMalloc(&de, (size_t)C.N_e * sizeof(double)) != hipSuccess) {
+            if (st) (void)hipFree(st);
+            if (tm) (void)hipFree(tm);
+            return cleanup(SSE_ERR_OUT_OF_MEMORY, "hipMalloc failed for the group state");
+        }
+        b->state.push_back(st);
+        b->tmp.push_back(tm);
+        b->dt_e.push_back(de);
+    }
+    *out = b;
+    return 0;
+}
+
+int sse_batch_rk_steps(sse_batch* b, double* const* u, const double* dt, int32_t nsteps, int32_t where,
+                       int32_t use_graph) {
+    if (!b) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !dt || nsteps < 0) return bfail(b, SSE_ERR_INVALID_ARGUMENT, "u or dt is NULL, or nsteps < 0");
+    if (where != SSE_DEVICE && where != SSE_HOST) return bfail(b, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
+    const hipMemcpyKind in = where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind outk = where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return 0;
+        return bfail(b, SSE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    int rc;
+    // gather states and per-element time steps into the group buffers
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        std::vector<double> de;
+        for (size_t t = 0; t < b->members[g].size(); ++t) {
+            const int i = b->members[g][t];
+            if (!u[i]) return bfail(b, SSE_ERR_INVALID_ARGUMENT, "u[" + std::to_string(i) + "] is NULL");
+            const size_t n_i = b->per_elem[i] * b->elems[i];
+            if ((rc = hip_ok(hipMemcpyAsync(b->state[g] + b->per_elem[i] * b->offsets[g][t], u[i],
+                                            n_i * sizeof(double), in, b->stream), "state gather")))
+                return rc;
+            de.insert(de.end(), b->elems[i], dt[i]);
+        }
+        if ((rc = hip_ok(hipMemcpyAsync(b->dt_e[g], de.data(), de.size() * sizeof(double), hipMemcpyHostToDevice,
+                                        b->stream), "dt upload")))
+            return rc;
+        // the host vector must outlive the copy
+        if ((rc = hip_ok(hipStreamSynchronize(b->stream), "sync"))) return rc;
+    }
+    auto one_step = [&]() -> int {
+        for (int s = 0; s < 5; ++s)
+            for (size_t g = 0; g < b->groups.size(); ++g) {
+                RKStage rk{1, CK_A[s], CK_B[s], 0.0, b->tmp[g], b->state[g], b->dt_e[g]};
+                int r = launch(b->groups[g], b->state[g], nullptr, rk, 3);
+                if (r) return bfail(b, r, b->groups[g]->err);
+            }
+        return 0;
+    };
+    int step = 0;
+    if (nsteps > 0) {  // the first step runs eagerly (it also sets kernel attributes)
+        if ((rc = one_step())) return rc;
+        step = 1;
+    }
+    if (use_graph && step < nsteps) {
+        if (!b->exec) {
+            if ((rc = hip_ok(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
+                return rc;
+            int r = one_step();
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            if (r) return r;
+            if ((rc = hip_ok(e, "end capture"))) return rc;
+            b->graph = graph;
+            if ((rc = hip_ok(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0), "graph instantiate")))
+                return rc;
+        }
+        for (; step < nsteps; ++step)
+            if ((rc = hip_ok(hipGraphLaunch(b->exec, b->stream), "graph launch"))) return rc;
+    }
+    for (; step < nsteps; ++step)
+        if ((rc = one_step())) return rc;
+    // scatter back
+    for (size_t g = 0; g < b->groups.size(); ++g)
+        for (size_t t = 0; t < b->members[g].size(); ++t) {
+            const int i = b->members[g][t];
+            const size_t n_i = b->per_elem[i] * b->elems[i];
+            if ((rc = hip_ok(hipMemcpyAsync(u[i], b->state[g] + b->per_elem[i] * b->offsets[g][t],
+                                            n_i * sizeof(double), outk, b->stream), "state scatter")))
+                return rc;
+        }
+    if ((rc = hip_ok(hipStreamSynchronize(b->stream), "sync"))) return rc;
+    for (sse_handle* h : b->groups)
+        if ((rc = check_status(h))) return bfail(b, rc, h->err);
+    return 0;
+}
+
+int sse_batch_describe(const sse_batch* b, char* buf, int32_t len) {
+    if (!b || !buf || len <= 0) return SSE_ERR_INVALID_ARGUMENT;
+    std::string out = "batch groups=" + std::to_string(b->groups.size());
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        char one[256];
+        sse_describe(b->groups[g], one, sizeof(one));
+        out += " | " + std::to_string(b->members[g].size()) + " problems, N_e=" +
+               std::to_string(b->groups[g]->P.N_e) + ": " + one;
+    }
+    snprintf(buf, len, "%s", out.c_str());
+    return 0;
+}
+
+const char* sse_batch_last_error(const sse_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
+
+void sse_batch_destroy(sse_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (sse_handle* h : b->groups) sse_destroy(h);
+    for (double* p : b->state) (void)hipFree(p);
+    for (double* p : b->tmp) (void)hipFree(p);
+    for (double* p : b->dt_e) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"

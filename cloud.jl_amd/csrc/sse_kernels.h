@@ -86,6 +86,8 @@ struct RKStage {
     double A, B, dt;
     double* tmp;    // low-storage register, (N_p, N_c, N_e)
     double* u;      // state updated in place
+    const double* dt_e;  // per-element time step (batched independent problems), or null for dt
+    __device__ __forceinline__ double dt_of(size_t k) const { return dt_e ? dt_e[k] : dt; }
 };
 
 }  // namespace sse

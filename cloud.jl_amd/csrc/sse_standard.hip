@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_standard(DevParams p, d
         for (int t = threadIdx.x; t < Np * NC; t += blockDim.x) dudt[base + t] = s_o[t];
     } else {
         for (int t = threadIdx.x; t < Np * NC; t += blockDim.x) {
-            const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt * s_o[t];
+            const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt_of(k) * s_o[t];
             rk.tmp[base + t] = kt;
             rk.u[base + t] += rk.B * kt;
         }

@@ -351,7 +351,7 @@ k_time_derivative_tensor(TensorParams tp, double* __restrict__ dudt, RKStage rk)
         for (int t = tl; t < NP * NC; t += TS) dudt[base + t] = sy[t];
     } else {
         for (int t = tl; t < NP * NC; t += TS) {
-            const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt * sy[t];
+            const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt_of(kk) * sy[t];
             rk.tmp[base + t] = kt;
             rk.u[base + t] += rk.B * kt;
         }

@@ -54,6 +54,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_destroy.restype = None
     lib.sse_describe.argtypes = [vp, C.c_char_p, C.c_int32]
     lib.sse_describe.restype = C.c_int
+    lib.sse_batch_create.argtypes = [C.POINTER(C.POINTER(SseDesc)), C.c_int32, C.POINTER(vp)]
+    lib.sse_batch_create.restype = C.c_int
+    lib.sse_batch_rk_steps.argtypes = [vp, C.POINTER(vp), dp, C.c_int32, C.c_int32, C.c_int32]
+    lib.sse_batch_rk_steps.restype = C.c_int
+    lib.sse_batch_describe.argtypes = [vp, C.c_char_p, C.c_int32]
+    lib.sse_batch_describe.restype = C.c_int
+    lib.sse_batch_last_error.argtypes = [vp]
+    lib.sse_batch_last_error.restype = C.c_char_p
+    lib.sse_batch_destroy.argtypes = [vp]
+    lib.sse_batch_destroy.restype = None
     lib.sse_abi_version.argtypes = []
     lib.sse_abi_version.restype = C.c_int32
     for name in EXPORTED_SYMBOLS:

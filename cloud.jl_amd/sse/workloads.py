@@ -142,3 +142,37 @@ def advection_diffusion_3d_config(M=4, p=5, b=5.0e-3, sigma=0.1, factor=0.05):
                         viscous_numerical_flux=BR1())
     ode = semidiscretize(law, sd, ic, form, (0.0, 1.0), PhysicalOperator())
     return law, sd, ExactSolution(law, ic), form, ode
+
+
+def batched_vortex_problems(n=256, seed=20261202, M_range=(4, 32), p_range=(2, 6), cfl=0.1,
+                            progress=None):
+    """n independent 2D Euler Shu-vortex problems of uneven size: periodic [0,10]^2
+    with M x M squares split into triangles (M uniform in M_range), ModalTensor(p)
+    with p uniform in p_range, vortex strength uniform in [1,5] and centre uniform
+    in the domain (seeded), free stream (1,1), EC + LF flux differencing.
+    Each problem's time step is dt = cfl * h / ((2p + 1) * 2.6), h = 10 / M, with
+    2.6 >= |v| + c of the free stream. Problems with the same (M, p) share their
+    mesh and geometry. Returns a list of (ode, dt, (M, p))."""
+    rng = np.random.default_rng(seed)
+    Ms = rng.integers(M_range[0], M_range[1] + 1, n)
+    ps = rng.integers(p_range[0], p_range[1] + 1, n)
+    strengths = rng.uniform(1.0, 5.0, n)
+    centres = rng.uniform(0.0, 10.0, (n, 2))
+    law = EulerEquations(2, 1.4)
+    form = FluxDifferencingForm(inviscid_numerical_flux=LaxFriedrichsNumericalFlux())
+    cache = {}
+    out = []
+    for i in range(n):
+        M, p = int(Ms[i]), int(ps[i])
+        if (M, p) not in cache:
+            ref = ReferenceApproximationTri(ModalTensor(p), mapping_degree=1)
+            mesh = uniform_periodic_mesh(ref, ((0.0, 10.0), (0.0, 10.0)), (M, M))
+            cache[(M, p)] = make_spatial_discretization(mesh, ref)
+        sd = cache[(M, p)]
+        ic = ShuVortex(1.4, float(strengths[i]), (float(centres[i, 0]), float(centres[i, 1])), (1.0, 1.0),
+                       (10.0, 10.0))
+        ode = semidiscretize(law, sd, ic, form, (0.0, 1.0))
+        out.append((ode, cfl * (10.0 / M) / ((2 * p + 1) * 2.6), (M, p)))
+        if progress and (i + 1) % 32 == 0:
+            progress(f"batched problems: {i + 1}/{n} set up")
+    return out

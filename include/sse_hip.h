@@ -190,6 +190,37 @@ void sse_destroy(sse_handle* h);
 /* ABI version the library was built with. */
 int32_t sse_abi_version(void);
 
+/*
+ * Batched independent problems. No counterpart in the reference, which runs one
+ * problem per Solver; a caller would otherwise loop over per-problem solves.
+ * Problems whose descriptors share the element type, reference operators, law,
+ * form and fluxes are concatenated into one group (one kernel launch per phase
+ * for all of them); a batch holds one group per distinct discretisation.
+ * Geometry and mapP are per problem, exactly as for sse_create.
+ */
+typedef struct sse_batch sse_batch;
+
+/* descs[i] describes problem i (i < n). On failure *out is NULL and
+ * sse_batch_last_error(NULL) explains. */
+int sse_batch_create(const sse_desc* const* descs, int32_t n, sse_batch** out);
+
+/* Advances every problem i by nsteps CarpenterKennedy2N54 steps of its own
+ * time step dt[i]. u[i] points to problem i's (N_p, N_c, N_e_i) state, device or
+ * host memory per `where`; it is read at the start and written at the end. With
+ * use_graph != 0 the RK step of all groups is captured once in a HIP graph and
+ * replayed; otherwise the kernels are launched step by step. Blocks until done. */
+int sse_batch_rk_steps(sse_batch* b, double* const* u, const double* dt, int32_t nsteps, int32_t where,
+                       int32_t use_graph);
+
+/* One-line description of the groups (count, elements, kernel family). */
+int sse_batch_describe(const sse_batch* b, char* buf, int32_t len);
+
+/* Message of the last failure on b (or of the last failed sse_batch_create when b is NULL). */
+const char* sse_batch_last_error(const sse_batch* b);
+
+/* Frees the batch, its groups and its HIP graph. */
+void sse_batch_destroy(sse_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
