@@ -27,6 +27,9 @@ size_t lds_nodal_values_generic(const DevParams& p);
 size_t lds_time_derivative_generic(const DevParams& p);
 hipError_t launch_standard(const DevParams& p, double* dudt, const RKStage& rk, hipStream_t s, int which);
 size_t lds_standard(const DevParams& p);
+hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
+                              double* out, hipStream_t s);
+size_t lds_functionals(const DevParams& p);
 struct TensorConfig {
     int TS, EPB, R, lds1, lds2, NQ, NF, NP;
 };
@@ -69,6 +72,8 @@ struct sse_handle {
     double* rk_tmp = nullptr;
     double* stage_u = nullptr;
     double* stage_dudt = nullptr;
+    double* fpartial = nullptr;  // functionals: per-element partial sums and the result
+    double* fout = nullptr;
     std::vector<void*> allocations;
     std::string err;
 };
@@ -760,6 +765,37 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     } else {
         snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d", P.d, P.N_p, P.N_q, P.N_f, P.N_c, P.N_e);
     }
+    return 0;
+}
+
+int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t which, int32_t where, double* out) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u or out is NULL");
+    if (which < SSE_MASS || which > SSE_ENTROPY) return fail(h, SSE_ERR_INVALID_ARGUMENT, "unknown functional");
+    if (where != SSE_DEVICE && where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
+    if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
+    const size_t n = state_size(h);
+    int rc;
+    if (!h->fpartial && ((rc = alloc(h, (size_t)h->P.N_e * h->P.N_c, &h->fpartial)) ||
+                         (rc = alloc(h, (size_t)h->P.N_c, &h->fout))))
+        return rc;
+    const double* ud = u;
+    const double* vd = dudt;
+    if (where == SSE_HOST) {
+        if (!h->stage_u && ((rc = alloc(h, n, &h->stage_u)) || (rc = alloc(h, n, &h->stage_dudt)))) return rc;
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_u, u, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ud = h->stage_u;
+        if (dudt) {
+            SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_dudt, dudt, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+            vd = h->stage_dudt;
+        }
+    }
+    const int kind = which + (dudt ? 3 : 0);
+    hipError_t e = sse::launch_functionals(h->P, ud, vd, kind, h->fpartial, h->fout, h->stream);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    const int nout = which == SSE_ENTROPY ? 1 : h->P.N_c;
+    SSE_HIP_TRY(h, hipMemcpyAsync(out, h->fout, nout * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
     return 0;
 }
 

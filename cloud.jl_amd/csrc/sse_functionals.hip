@@ -1,0 +1,228 @@
+// sse_functionals.hip -- conservation and invariant functionals on the device
+// (the reference's src/Analysis/conservation.jl:113-189):
+//   mass       sum_k 1' WJ_k V u_k                      (per variable)
+//   energy     1/2 sum_k u_k' M_k u_k                   (per variable)
+//   entropy    sum_k sum_i WJ_k,i S((V u_k)_i)
+// and their rates along dudt:
+//   mass       sum_k 1' WJ_k V dudt_k
+//   energy     sum_k u_k' M_k dudt_k
+//   entropy    sum_k (P_k w_q)' M_k dudt_k = sum_k w_q' WJ_k V dudt_k,  P_k = M_k^{-1} V' WJ_k
+// M_k is the mass matrix of the handle's solver: (V' diag(W / J_k) V)^{-1} for the
+// weight-adjusted solver with M^{-1} = I (src/Solvers/mass_matrix.jl:150-153), W J_k
+// for the diagonal one. One workgroup per element writes its partial sums; a
+// second kernel adds them in a fixed order (deterministic, no atomics).
+#include <hip/hip_runtime.h>
+
+#include "sse_kernels.h"
+
+namespace sse {
+
+namespace {
+
+constexpr int FT = 128;  // threads per element
+
+__device__ double block_sum(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = FT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// S(u) (euler_navierstokes.jl:299-304; linear_advection_diffusion.jl:273) and the
+// entropy variables
+template <int D, int LAW>
+__device__ double entropy_of(const LawParams& P, const double* u) {
+    if constexpr (LAW == LAW_EULER) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) s += u[m + 1] * u[m + 1];
+        const double p = P.gm1 * (u[D + 1] - (0.5 / u[0]) * s);
+        return -u[0] * log(p / pow(u[0], P.gamma)) * P.inv_gm1;
+    } else {
+        return 0.5 * u[0] * u[0];
+    }
+}
+
+}  // namespace
+
+// which: 0 mass, 1 energy, 2 entropy, 3 mass rate, 4 energy rate, 5 entropy rate.
+// partial has N_e x ncomp entries.
+template <int D, int LAW>
+__global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* __restrict__ u,
+                                                    const double* __restrict__ dudt, int which,
+                                                    double* __restrict__ partial) {
+    using Ph = Physics<D, LAW>;
+    constexpr int NC = Ph::NC;
+    extern __shared__ double sm[];
+    const int k = blockIdx.x;
+    const int Np = p.N_p, Nq = p.N_q;
+    double* red = sm;                // FT
+    double* s_u = red + FT;          // Np x NC
+    double* s_v = s_u + Np * NC;     // Np x NC (dudt)
+    double* s_uq = s_v + Np * NC;    // Nq x NC
+    double* s_vq = s_uq + Nq * NC;   // Nq x NC
+    double* s_M = s_vq + Nq * NC;    // Np x Np (M_k^{-1}, then its Cholesky factor)
+    double* s_y = s_M + Np * Np;     // Np x NC
+    const double* Jk = p.Jq + (size_t)k * Nq;
+    const size_t base = (size_t)k * Np * NC;
+    const bool rate = which >= 3;
+    const int kind = which % 3;
+    for (int t = threadIdx.x; t < Np * NC; t += FT) {
+        s_u[t] = u[base + t];
+        s_v[t] = rate ? dudt[base + t] : 0.0;
+    }
+    __syncthreads();
+    double* out = partial + (size_t)k * NC;
+    if (kind != 1) {  // mass and entropy work at the volume nodes
+        for (int i = threadIdx.x; i < Nq; i += FT)
+            for (int e = 0; e < NC; ++e) {
+                double a = 0.0, b = 0.0;
+                if (p.has_V) {
+                    for (int j = 0; j < Np; ++j) {
+                        const double vij = p.V[i + (size_t)Nq * j];
+                        a += vij * s_u[j + Np * e];
+                        b += vij * s_v[j + Np * e];
+                    }
+                } else {
+                    a = s_u[i + Np * e];
+                    b = s_v[i + Np * e];
+                }
+                s_uq[i + Nq * e] = a;
+                s_vq[i + Nq * e] = b;
+            }
+        __syncthreads();
+        if (kind == 0) {
+            for (int e = 0; e < NC; ++e) {
+                double s = 0.0;
+                for (int i = threadIdx.x; i < Nq; i += FT) s += p.W[i] * Jk[i] * (rate ? s_vq : s_uq)[i + Nq * e];
+                const double tot = block_sum(s, red);
+                if (threadIdx.x == 0) out[e] = tot;
+            }
+        } else {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < Nq; i += FT) {
+                double ui[NC];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) ui[e] = s_uq[i + Nq * e];
+                const double wj = p.W[i] * Jk[i];
+                if (!rate) {
+                    s += wj * entropy_of<D, LAW>(p.law, ui);
+                } else {
+                    double wi[NC];
+                    if constexpr (LAW == LAW_EULER) {
+                        Ph::c2e(p.law, ui, wi);
+                    } else {
+                        wi[0] = ui[0];
+                    }
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) s += wj * wi[e] * s_vq[i + Nq * e];
+                }
+            }
+            const double tot = block_sum(s, red);
+            if (threadIdx.x == 0) {
+                out[0] = tot;
+                for (int e = 1; e < NC; ++e) out[e] = 0.0;
+            }
+        }
+        return;
+    }
+    // energy: y = M_k x by solving (M_k^{-1}) y = x, x = u (energy) or dudt (rate)
+    if (p.has_V) {
+        for (int t = threadIdx.x; t < Np * Np; t += FT) {  // M_k^{-1} = V' diag(W/J) V
+            const int a = t % Np, b = t / Np;
+            double s = 0.0;
+            for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * a] * (p.W[i] / Jk[i]) * p.V[i + (size_t)Nq * b];
+            s_M[t] = s;
+        }
+    } else {
+        for (int t = threadIdx.x; t < Np * Np; t += FT) {
+            const int a = t % Np, b = t / Np;
+            s_M[t] = a == b ? 1.0 / (p.W[a] * Jk[a]) : 0.0;
+        }
+    }
+    for (int t = threadIdx.x; t < Np * NC; t += FT) s_y[t] = rate ? s_v[t] : s_u[t];
+    __syncthreads();
+    // Cholesky (lower, in place, column by column)
+    for (int j = 0; j < Np; ++j) {
+        if (threadIdx.x == 0) s_M[j + Np * j] = sqrt(s_M[j + Np * j]);
+        __syncthreads();
+        const double djj = s_M[j + Np * j];
+        for (int a = j + 1 + threadIdx.x; a < Np; a += FT) s_M[a + Np * j] /= djj;
+        __syncthreads();
+        for (int t = threadIdx.x; t < (Np - j - 1) * (Np - j - 1); t += FT) {
+            const int a = j + 1 + t % (Np - j - 1), b = j + 1 + t / (Np - j - 1);
+            if (b <= a) s_M[a + Np * b] -= s_M[a + Np * j] * s_M[b + Np * j];
+        }
+        __syncthreads();
+    }
+    // forward and back substitution, one variable per thread
+    for (int e = threadIdx.x; e < NC; e += FT) {
+        double* y = s_y + Np * e;
+        for (int a = 0; a < Np; ++a) {
+            double s = y[a];
+            for (int b = 0; b < a; ++b) s -= s_M[a + Np * b] * y[b];
+            y[a] = s / s_M[a + Np * a];
+        }
+        for (int a = Np - 1; a >= 0; --a) {
+            double s = y[a];
+            for (int b = a + 1; b < Np; ++b) s -= s_M[b + Np * a] * y[b];
+            y[a] = s / s_M[a + Np * a];
+        }
+    }
+    __syncthreads();
+    for (int e = 0; e < NC; ++e) {
+        double s = 0.0;
+        for (int a = threadIdx.x; a < Np; a += FT) s += s_u[a + Np * e] * s_y[a + Np * e];
+        const double tot = block_sum(s, red);
+        if (threadIdx.x == 0) out[e] = rate ? tot : 0.5 * tot;
+    }
+}
+
+// out[c] = sum_k partial[k, c] in a fixed order
+__global__ void __launch_bounds__(FT) k_sum_partials(const double* __restrict__ partial, int N_e, int ncomp,
+                                                     double* __restrict__ out) {
+    __shared__ double red[FT];
+    for (int c = 0; c < ncomp; ++c) {
+        double s = 0.0;
+        for (int k = threadIdx.x; k < N_e; k += FT) s += partial[(size_t)k * ncomp + c];
+        const double tot = block_sum(s, red);
+        if (threadIdx.x == 0) out[c] = tot;
+    }
+}
+
+size_t lds_functionals(const DevParams& p) {
+    return sizeof(double) * ((size_t)FT + 4 * (size_t)p.N_p * p.N_c + 2 * (size_t)p.N_q * p.N_c +
+                             (size_t)p.N_p * p.N_p);
+}
+
+template <int D, int LAW>
+hipError_t launch_functionals_t(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
+                                double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_functionals<D, LAW>), dim3(p.N_e), dim3(FT), lds_functionals(p), s, p, u, dudt, which,
+                       partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FT), 0, s, partial, p.N_e, p.N_c, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
+                              double* out, hipStream_t s) {
+    const bool euler = (p.N_c != 1);
+    switch (p.d) {
+        case 1: return euler ? launch_functionals_t<1, LAW_EULER>(p, u, dudt, which, partial, out, s)
+                             : launch_functionals_t<1, LAW_ADVECTION>(p, u, dudt, which, partial, out, s);
+        case 2: return euler ? launch_functionals_t<2, LAW_EULER>(p, u, dudt, which, partial, out, s)
+                             : launch_functionals_t<2, LAW_ADVECTION>(p, u, dudt, which, partial, out, s);
+        case 3: return euler ? launch_functionals_t<3, LAW_EULER>(p, u, dudt, which, partial, out, s)
+                             : launch_functionals_t<3, LAW_ADVECTION>(p, u, dudt, which, partial, out, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace sse

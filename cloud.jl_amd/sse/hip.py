@@ -64,6 +64,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_batch_last_error.restype = C.c_char_p
     lib.sse_batch_destroy.argtypes = [vp]
     lib.sse_batch_destroy.restype = None
+    lib.sse_functional.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, dp]
+    lib.sse_functional.restype = C.c_int
     lib.sse_abi_version.argtypes = []
     lib.sse_abi_version.restype = C.c_int32
     for name in EXPORTED_SYMBOLS:

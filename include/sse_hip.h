@@ -175,6 +175,21 @@ int sse_profile_residual(sse_handle* h, const double* u_dev, double* dudt_dev, i
  * bytes). No counterpart in the reference; for tests and benchmarks. */
 int sse_describe(const sse_handle* h, char* buf, int32_t len);
 
+/* Functionals of src/Analysis/conservation.jl:113-189 */
+typedef enum {
+    SSE_MASS = 0,    /* sum_k 1' WJ_k V u_k, N_c values (PrimaryConservationAnalysis) */
+    SSE_ENERGY = 1,  /* 1/2 sum_k u_k' M_k u_k, N_c values (EnergyConservationAnalysis) */
+    SSE_ENTROPY = 2  /* sum_k sum_i WJ_k,i S((V u_k)_i), 1 value (EntropyConservationAnalysis) */
+} sse_functional_kind;
+
+/* Replaces evaluate_conservation (dudt == NULL) and evaluate_conservation_residual
+ * (dudt != NULL: the functional's rate along dudt) of src/Analysis/conservation.jl:113-189,
+ * with M_k the handle's mass matrix (src/Solvers/mass_matrix.jl:140-153). u and dudt
+ * are (N_p, N_c, N_e), device or host memory per `where`; out (host) receives N_c
+ * values for SSE_MASS and SSE_ENERGY, 1 for SSE_ENTROPY. Sums run in a fixed order.
+ * Blocks until the values are in out. */
+int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t which, int32_t where, double* out);
+
 /* Uses the given hipStream_t (0 = the handle's own stream) for later calls. */
 int sse_set_stream(sse_handle* h, void* hip_stream);
 

@@ -301,3 +301,55 @@ def test_batched_problems_match_individual_solves():
         ref = to_host(ud)
         assert _rel(to_host(got), ref) <= 1e-13, key
         assert _rel(to_host(got_eager), ref) <= 1e-13, key
+
+
+# ----------------------------------------------------------------------------- functionals, LinearResidual
+FUNC_CASES = {
+    "wave3d_tet_p3_curved": lambda: density_wave_3d_config(M=2, p=3, ic_strength=0.2),
+    "euler1d_gauss_nodal": lambda: euler_1d_gauss(),
+    "advdiff3d_tet_p4": lambda: advection_diffusion_3d_config(M=2, p=4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FUNC_CASES))
+def test_functionals_match_oracle(name):
+    from oracle import functionals as F
+    from sse.analysis import (EnergyConservationAnalysis, EntropyConservationAnalysis,
+                              PrimaryConservationAnalysis)
+    from sse.time_integration import to_device
+    law, sd, ic, form, ode = FUNC_CASES[name]()
+    u = ode.u0
+    dudt = O.residual(ode.p.arrays, u)
+    ud, dd = to_device(u), to_device(dudt)
+    expect = {
+        PrimaryConservationAnalysis: (F.mass(sd, u), F.mass_rate(sd, u, dudt)),
+        EnergyConservationAnalysis: (F.energy(sd, u), F.energy_rate(sd, u, dudt)),
+        EntropyConservationAnalysis: (F.entropy(sd, law, u), F.entropy_rate(sd, law, u, dudt)),
+    }
+    for cls, (val, rate) in expect.items():
+        a = cls(ode.p)
+        for got in (a.evaluate_conservation(ud), a.evaluate_conservation(u)):
+            np.testing.assert_allclose(got, val, rtol=1e-12, atol=1e-12 * np.abs(val).max(), err_msg=cls.__name__)
+        scale = np.abs(F.energy(sd, np.abs(dudt))).max() + np.abs(val).max()
+        for got in (a.evaluate_conservation_residual(ud, dd), a.evaluate_conservation_residual(u, dudt)):
+            np.testing.assert_allclose(got, rate, rtol=1e-10, atol=1e-12 * scale, err_msg=cls.__name__)
+
+
+def test_linear_residual_is_linear_and_skew_adjoint():
+    # central skew-symmetric advection: M R is skew-symmetric, <x, M R y> = -<y, M R x>
+    from oracle import functionals as F
+    from sse.linear import LinearResidual
+    law, sd, ex, form, ode = advection_2d_modal_tensor_tri()
+    L = LinearResidual(ode.p)
+    rng = np.random.default_rng(11)
+    x, y = rng.standard_normal(L.shape[0]), rng.standard_normal(L.shape[0])
+    Rx, Ry, Rxy = L @ x, L @ y, L @ (2.0 * x - 3.0 * y)
+    np.testing.assert_allclose(Rxy, 2.0 * Rx - 3.0 * Ry, rtol=0, atol=1e-12 * np.abs(Rxy).max())
+    shape = ode.p.size()
+
+    def minner(a, b):
+        A, B = a.reshape(shape, order="F"), b.reshape(shape, order="F")
+        return sum(A[:, 0, k] @ F.mass_matrix(sd, k) @ B[:, 0, k] for k in range(sd.N_e))
+
+    lhs, rhs = minner(x, Ry), -minner(y, Rx)
+    assert abs(lhs - rhs) < 1e-11 * (abs(lhs) + 1.0), (lhs, rhs)

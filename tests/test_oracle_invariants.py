@@ -341,3 +341,33 @@ def test_standard_form_br1_converges_to_exact_gaussian():
         uT = O.rk_steps(ode.p.arrays, ode.u0, T / n, n)
         errs.append(l2_error(sd, uT, ex, T)[0])
     assert errs[1] < 0.7 * errs[0] and errs[1] < 0.02, errs
+
+
+# ----------------------------------------------------------------------------- functionals
+def test_functionals_closed_forms():
+    from oracle import functionals as F
+    law, sd, ic, form, ode = density_wave_3d_config(M=2, p=3, factor=0.05, ic_strength=0.2)
+    ref = sd.reference_approximation
+    # a constant state: mass = state x volume (8), entropy = S(state) x volume
+    rho, v, p = 1.3, np.array([0.2, -0.1, 0.4]), 0.8
+    uc = np.array([rho, *(rho * v), p / (GAMMA - 1) + 0.5 * rho * v @ v])
+    coef0 = np.linalg.lstsq(ref.Vdense(), np.ones(ref.N_q), rcond=None)[0]
+    u = np.asfortranarray(coef0[:, None, None] * uc[None, :, None] * np.ones((1, 1, sd.N_e)))
+    np.testing.assert_allclose(F.mass(sd, u), 8.0 * uc, rtol=1e-12)
+    S = -rho * math.log(p / rho ** GAMMA) / (GAMMA - 1)
+    np.testing.assert_allclose(F.entropy(sd, law, u), [8.0 * S], rtol=1e-12)
+    # entropy rate: the literal (P w)' M dudt equals w_q' WJ V dudt
+    dudt = O.residual(ode.p.arrays, ode.u0)
+    rate, _ = _entropy_rate(sd, law, ode.u0, dudt)
+    np.testing.assert_allclose(F.entropy_rate(sd, law, ode.u0, dudt), [rate], rtol=1e-9, atol=1e-12)
+
+
+def test_energy_functional_on_affine_mesh_is_half_l2_norm():
+    # with constant J the weight-adjusted mass matrix is V' W J V exactly, so the
+    # energy is half the quadrature L2 norm of the polynomial
+    from oracle import functionals as F
+    law, sd, ic, form, ode = vortex_2d_config(M=2, p=3)
+    ref = sd.reference_approximation
+    uq = np.einsum("qp,pck->qck", ref.Vdense(), ode.u0)
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q
+    np.testing.assert_allclose(F.energy(sd, ode.u0), 0.5 * np.einsum("qk,qck->c", WJ, uq ** 2), rtol=1e-12)
