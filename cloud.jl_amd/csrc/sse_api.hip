@@ -30,6 +30,12 @@ size_t lds_standard(const DevParams& p);
 hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
                               double* out, hipStream_t s);
 size_t lds_functionals(const DevParams& p);
+hipError_t launch_vjp(const DevParams& p, const double* u, const double* ybar, double* ubar, double* rbar,
+                      double* fbar, double* lam_partial, double* lam_out, double lam_scale, int accumulate,
+                      hipStream_t s);
+size_t lds_vjp_v2(const DevParams& p);
+hipError_t launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t s);
+hipError_t launch_entropy_grad(const DevParams& p, const double* u, double* g, hipStream_t s);
 struct TensorConfig {
     int TS, EPB, R, lds1, lds2, NQ, NF, NP;
 };
@@ -74,6 +80,7 @@ struct sse_handle {
     double* stage_dudt = nullptr;
     double* fpartial = nullptr;  // functionals: per-element partial sums and the result
     double* fout = nullptr;
+    double *vjp_rbar = nullptr, *vjp_fbar = nullptr, *vjp_lam = nullptr, *vjp_lamout = nullptr;  // VJP scratch
     std::vector<void*> allocations;
     std::string err;
 };
@@ -797,6 +804,166 @@ int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t w
     SSE_HIP_TRY(h, hipMemcpyAsync(out, h->fout, nout * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
     return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+int vjp_check(sse_handle* h) {
+    const DevParams& P = h->P;
+    if (P.standard || P.N_c == 1) return fail(h, SSE_ERR_UNSUPPORTED, "the VJP covers the Euler flux-differencing residual");
+    if (P.law.two_point_flux != SSE_TWO_POINT_EC)
+        return fail(h, SSE_ERR_UNSUPPORTED, "the VJP covers the entropy-conservative two-point flux");
+    if (sse::lds_vjp_v2(P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the VJP kernels");
+    const size_t Ne = P.N_e;
+    int rc;
+    if (!h->vjp_rbar && ((rc = alloc(h, (size_t)P.N_q * P.N_c * Ne, &h->vjp_rbar)) ||
+                         (rc = alloc(h, (size_t)P.N_f * P.N_c * Ne, &h->vjp_fbar)) || (rc = alloc(h, Ne, &h->vjp_lam)) ||
+                         (rc = alloc(h, 1, &h->vjp_lamout))))
+        return rc;
+    return 0;
+}
+
+// ubar = (dR/du)' ybar on device buffers; lam_out (device) gets or accumulates scale * (dR/dλ)' ybar
+int vjp_device(sse_handle* h, const double* u, const double* ybar, double* ubar, double lam_scale, int accumulate) {
+    hipError_t e = sse::launch_generic_dispatch(h->P, u, nullptr, RKStage{}, h->stream, 1);  // u tilde
+    if (e == hipSuccess)
+        e = sse::launch_vjp(h->P, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_lam, h->vjp_lamout, lam_scale,
+                            accumulate, h->stream);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sse_residual_vjp(sse_handle* h, const double* u, const double* ybar, double* ubar, double* lambda_bar,
+                     int32_t where) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !ybar || !ubar || !lambda_bar) return fail(h, SSE_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (where != SSE_DEVICE && where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
+    int rc;
+    if ((rc = vjp_check(h))) return rc;
+    const size_t n = state_size(h);
+    const double *ud = u, *yd = ybar;
+    double* od = ubar;
+    if (where == SSE_HOST) {
+        if (!h->stage_u && ((rc = alloc(h, n, &h->stage_u)) || (rc = alloc(h, n, &h->stage_dudt)))) return rc;
+        if (!h->rk_tmp && (rc = alloc(h, n, &h->rk_tmp))) return rc;
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_u, u, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_dudt, ybar, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ud = h->stage_u;
+        yd = h->stage_dudt;
+        od = h->rk_tmp;
+    }
+    if ((rc = vjp_device(h, ud, yd, od, 1.0, 0))) return rc;
+    if (where == SSE_HOST)
+        SSE_HIP_TRY(h, hipMemcpyAsync(ubar, od, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SSE_HIP_TRY(h, hipMemcpyAsync(lambda_bar, h->vjp_lamout, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return check_status(h);
+}
+
+int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, int32_t checkpoint_every,
+                   int32_t objective, double* J, double* grad_u0, double* grad_lambda, int32_t where) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u0 || !J || !grad_u0 || !grad_lambda) return fail(h, SSE_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nsteps < 0 || checkpoint_every <= 0) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad nsteps or checkpoint_every");
+    if (objective != SSE_ENTROPY) return fail(h, SSE_ERR_UNSUPPORTED, "the adjoint objective is the entropy integral");
+    if (where != SSE_DEVICE && where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
+    int rc;
+    if ((rc = vjp_check(h))) return rc;
+    const size_t n = state_size(h);
+    const int nseg = (nsteps + checkpoint_every - 1) / checkpoint_every;
+    const int seglen = nsteps < checkpoint_every ? nsteps : checkpoint_every;
+    // work buffers, freed at the end of the call
+    std::vector<void*> mine;
+    auto dalloc = [&](size_t count, double** p) -> int {
+        void* q = nullptr;
+        if (hipMalloc(&q, count * sizeof(double)) != hipSuccess) return fail(h, SSE_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+        mine.push_back(q);
+        *p = static_cast<double*>(q);
+        return 0;
+    };
+    auto release = [&]() {
+        (void)hipStreamSynchronize(h->stream);
+        for (void* q : mine) (void)hipFree(q);
+    };
+    double *u = nullptr, *ktmp = nullptr, *ub = nullptr, *kb = nullptr, *vb = nullptr, *ckpt = nullptr,
+           *stages = nullptr;
+    if ((rc = dalloc(n, &u)) || (rc = dalloc(n, &ktmp)) || (rc = dalloc(n, &ub)) || (rc = dalloc(n, &kb)) ||
+        (rc = dalloc(n, &vb)) || (rc = dalloc(n * (size_t)(nseg > 0 ? nseg : 1), &ckpt)) ||
+        (rc = dalloc(n * 5 * (size_t)(seglen > 0 ? seglen : 1), &stages))) {
+        release();
+        return rc;
+    }
+    auto hip_step = [&](hipError_t e, const char* what) {
+        return e == hipSuccess ? 0 : fail(h, SSE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+#define ADJ_TRY(expr)      \
+    do {                    \
+        if ((rc = (expr))) { \
+            release();      \
+            return rc;      \
+        }                   \
+    } while (0)
+    ADJ_TRY(hip_step(hipMemcpyAsync(u, u0, n * sizeof(double),
+                                    where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream),
+                     "copy u0"));
+    // forward sweep with checkpoints at the start of each segment
+    for (int step = 0; step < nsteps; ++step) {
+        if (step % checkpoint_every == 0)
+            ADJ_TRY(hip_step(hipMemcpyAsync(ckpt + n * (size_t)(step / checkpoint_every), u, n * sizeof(double),
+                                            hipMemcpyDeviceToDevice, h->stream), "checkpoint"));
+        for (int s = 0; s < 5; ++s) {
+            RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
+            ADJ_TRY(launch(h, u, nullptr, rk, 3));
+        }
+    }
+    // objective and its gradient at the final state
+    ADJ_TRY(hip_step(sse::launch_functionals(h->P, u, nullptr, SSE_ENTROPY, h->fpartial ? h->fpartial : ktmp,
+                                             vb, h->stream), "entropy"));
+    ADJ_TRY(hip_step(hipMemcpyAsync(J, vb, sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy J"));
+    ADJ_TRY(hip_step(hipStreamSynchronize(h->stream), "sync"));
+    ADJ_TRY(hip_step(sse::launch_entropy_grad(h->P, u, ub, h->stream), "entropy gradient"));
+    ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lamout, 0, sizeof(double), h->stream), "zero"));
+    // reverse sweep, one segment at a time
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int first = seg * checkpoint_every;
+        const int last = std::min(nsteps, first + checkpoint_every);
+        ADJ_TRY(hip_step(hipMemcpyAsync(u, ckpt + n * (size_t)seg, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                        h->stream), "restore"));
+        for (int step = first; step < last; ++step)
+            for (int s = 0; s < 5; ++s) {
+                double* keep = stages + n * (size_t)(5 * (step - first) + s);
+                ADJ_TRY(hip_step(hipMemcpyAsync(keep, u, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
+                                 "stage state"));
+                RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
+                ADJ_TRY(launch(h, u, nullptr, rk, 3));
+            }
+        // adjoint of the 2N stages: kbar += B_s ubar; ubar += dt R'(u_s)' kbar; kbar *= A_s
+        for (int step = last - 1; step >= first; --step) {
+            ADJ_TRY(hip_step(hipMemsetAsync(kb, 0, n * sizeof(double), h->stream), "zero"));
+            for (int s = 4; s >= 0; --s) {
+                const double* us = stages + n * (size_t)(5 * (step - first) + s);
+                ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[s], ub, 1.0, kb, h->stream), "axpby"));
+                ADJ_TRY(vjp_device(h, us, kb, vb, dt, 1));
+                ADJ_TRY(hip_step(sse::launch_axpby(n, dt, vb, 1.0, ub, h->stream), "axpby"));
+                if (s > 0) ADJ_TRY(hip_step(sse::launch_axpby(n, 0.0, kb, CK_A[s], kb, h->stream), "scale"));
+            }
+        }
+    }
+    ADJ_TRY(hip_step(hipMemcpyAsync(grad_u0, ub, n * sizeof(double),
+                                    where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream),
+                     "copy gradient"));
+    ADJ_TRY(hip_step(hipMemcpyAsync(grad_lambda, h->vjp_lamout, sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                     "copy lambda gradient"));
+    ADJ_TRY(hip_step(hipStreamSynchronize(h->stream), "sync"));
+#undef ADJ_TRY
+    release();
+    return check_status(h);
 }
 
 int sse_set_stream(sse_handle* h, void* stream) {

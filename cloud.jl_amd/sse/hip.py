@@ -66,6 +66,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_batch_destroy.restype = None
     lib.sse_functional.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, dp]
     lib.sse_functional.restype = C.c_int
+    lib.sse_residual_vjp.argtypes = [vp, vp, vp, vp, dp, C.c_int32]
+    lib.sse_residual_vjp.restype = C.c_int
+    lib.sse_rk_adjoint.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, C.c_int32, dp, vp, dp, C.c_int32]
+    lib.sse_rk_adjoint.restype = C.c_int
     lib.sse_abi_version.argtypes = []
     lib.sse_abi_version.restype = C.c_int32
     for name in EXPORTED_SYMBOLS:

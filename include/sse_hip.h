@@ -175,6 +175,26 @@ int sse_profile_residual(sse_handle* h, const double* u_dev, double* dudt_dev, i
  * bytes). No counterpart in the reference; for tests and benchmarks. */
 int sse_describe(const sse_handle* h, char* buf, int32_t len);
 
+/* Vector-Jacobian product of the residual (the pull-back a ChainRulesCore.rrule of
+ * semi_discrete_residual! returns; no counterpart in the reference, which has no
+ * derivatives): ubar = (dR/du)' ybar and *lambda_bar = (dR/dλ)' ybar, λ the
+ * LaxFriedrichsNumericalFlux coefficient (halfλ = λ / 2). u, ybar, ubar are
+ * (N_p, N_c, N_e), device or host per `where`; lambda_bar is a host scalar.
+ * Implemented for the Euler equations in FluxDifferencingForm with the
+ * entropy-conservative two-point flux (any element, modal or nodal); returns
+ * SSE_ERR_UNSUPPORTED otherwise. Blocks until done. */
+int sse_residual_vjp(sse_handle* h, const double* u, const double* ybar, double* ubar, double* lambda_bar,
+                     int32_t where);
+
+/* Discrete adjoint of nsteps CarpenterKennedy2N54 steps of size dt from u0 for the
+ * objective J = entropy integral of the final state (SSE_ENTROPY; the only
+ * objective so far): writes J, grad_u0 = dJ/du0 (N_p, N_c, N_e) and *grad_lambda =
+ * dJ/dλ. The forward states are checkpointed every `checkpoint_every` steps and
+ * each segment is recomputed once, keeping its stage states. No counterpart in
+ * the reference. u0 and grad_u0 are device or host per `where`. Blocks. */
+int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, int32_t checkpoint_every,
+                   int32_t objective, double* J, double* grad_u0, double* grad_lambda, int32_t where);
+
 /* Functionals of src/Analysis/conservation.jl:113-189 */
 typedef enum {
     SSE_MASS = 0,    /* sum_k 1' WJ_k V u_k, N_c values (PrimaryConservationAnalysis) */

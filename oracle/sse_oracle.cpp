@@ -49,6 +49,51 @@ namespace {
 
 using std::vector;
 
+using std::exp;
+using std::fabs;
+using std::log;
+using std::pow;
+using std::sqrt;
+
+// Forward-mode dual number (value, one tangent): instantiating the oracle on it
+// differentiates the restated residual exactly (the role ForwardDiff plays for
+// the Julia reference). Comparisons look at the value only.
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double x) : v(x), d(0.0) {}
+    Dual(double x, double t) : v(x), d(t) {}
+    Dual& operator+=(const Dual& o) { v += o.v; d += o.d; return *this; }
+    Dual& operator-=(const Dual& o) { v -= o.v; d -= o.d; return *this; }
+    Dual& operator*=(const Dual& o) { d = d * o.v + v * o.d; v *= o.v; return *this; }
+    Dual& operator/=(const Dual& o) { d = (d * o.v - v * o.d) / (o.v * o.v); v /= o.v; return *this; }
+};
+inline Dual operator+(Dual a, const Dual& b) { return a += b; }
+inline Dual operator-(Dual a, const Dual& b) { return a -= b; }
+inline Dual operator*(Dual a, const Dual& b) { return a *= b; }
+inline Dual operator/(Dual a, const Dual& b) { return a /= b; }
+inline Dual operator+(Dual a, double b) { return a += Dual(b); }
+inline Dual operator+(double a, const Dual& b) { return Dual(a) += b; }
+inline Dual operator-(Dual a, double b) { return a -= Dual(b); }
+inline Dual operator-(double a, const Dual& b) { return Dual(a) -= b; }
+inline Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+inline Dual operator*(double a, const Dual& b) { return Dual(a * b.v, a * b.d); }
+inline Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
+inline Dual operator/(double a, const Dual& b) { return Dual(a) /= b; }
+inline Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+inline bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+inline bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+inline bool operator<(const Dual& a, double b) { return a.v < b; }
+inline bool operator>(const Dual& a, double b) { return a.v > b; }
+inline Dual log(const Dual& a) { return Dual(std::log(a.v), a.d / a.v); }
+inline Dual exp(const Dual& a) { const double e = std::exp(a.v); return Dual(e, e * a.d); }
+inline Dual sqrt(const Dual& a) { const double r = std::sqrt(a.v); return Dual(r, 0.5 * a.d / r); }
+inline Dual fabs(const Dual& a) { return a.v < 0.0 ? -a : a; }
+inline Dual pow(const Dual& a, double b) {
+    const double r = std::pow(a.v, b);
+    return Dual(r, b * std::pow(a.v, b - 1.0) * a.d);
+}
+
 template <typename T>
 struct Law {
     int kind, d, N_c;
@@ -213,6 +258,7 @@ struct Oracle {
     vector<T> u_f;            // N_f x N_e x N_c ("switched order")
     vector<int64_t> mapP0;    // 0-based
     bool second_order = false;
+    T hl = 0.0;               // LaxFriedrichsNumericalFlux.halfλ (a T so that it can carry a tangent)
     vector<T> q_q;            // N_q x N_c x d x N_e (auxiliary variable, SecondOrder)
     vector<T> q_f;            // N_f x N_e x N_c x d
 
@@ -221,6 +267,7 @@ struct Oracle {
         d = D.d; N_p = D.N_p; N_q = D.N_q; N_f = D.N_f; N_fac = D.N_fac; N_c = D.N_c; N_e = D.N_e;
         if (N_fac <= 0 || N_f % N_fac != 0) return "N_f must be a multiple of N_fac";
         npf = N_f / N_fac;
+        hl = D.half_lambda;
         law.kind = D.law; law.d = d; law.N_c = N_c;
         law.gamma = D.gamma; law.gm1 = D.gamma - 1.0; law.inv_gm1 = 1.0 / (D.gamma - 1.0);
         for (int m = 0; m < 3; ++m) law.a[m] = D.a[m];
@@ -417,7 +464,7 @@ struct Oracle {
             law.two_point(D.two_point_flux, uin, uout, F);
             const double* nf = n_f.data() + d * (j + (size_t)N_f * k);
             T a = 0.0;
-            if (D.inviscid_flux == SSE_LAX_FRIEDRICHS) a = D.half_lambda * law.wave_speed(uin, uout, nf);
+            if (D.inviscid_flux == SSE_LAX_FRIEDRICHS) a = hl * law.wave_speed(uin, uout, nf);
             for (int e = 0; e < N_c; ++e) {
                 T fn = 0.0;
                 for (int m = 0; m < d; ++m) fn += F[e + N_c * m] * nf[m];
@@ -564,7 +611,7 @@ struct Oracle {
             law.two_point(D.two_point_flux, uin, uout, F);
             const double* nf = n_f.data() + d * (j + (size_t)N_f * k);
             T a = 0.0;
-            if (D.inviscid_flux == SSE_LAX_FRIEDRICHS) a = D.half_lambda * law.wave_speed(uin, uout, nf);
+            if (D.inviscid_flux == SSE_LAX_FRIEDRICHS) a = hl * law.wave_speed(uin, uout, nf);
             for (int e = 0; e < N_c; ++e) {
                 T fn = 0.0;
                 for (int m = 0; m < d; ++m) fn += F[e + N_c * m] * nf[m];
@@ -766,6 +813,44 @@ int sse_oracle_rk_steps(const sse_desc* desc, double* u, double dt, int nsteps) 
             }
         }
     }
+    return 0;
+}
+
+// Directional derivative of the residual along (du, dlambda), lambda the
+// LaxFriedrichsNumericalFlux coefficient (halfλ = λ / 2): dudt = R(u), ddudt = R'(u) du +
+// dR/dλ dlambda. Forward-mode dual numbers through the same restatement.
+int sse_oracle_residual_jvp(const sse_desc* desc, const double* u, const double* du, double dlambda, double* dudt,
+                            double* ddudt) {
+    Oracle<Dual> o;
+    g_err = o.init(desc);
+    if (!g_err.empty()) return SSE_ERR_UNSUPPORTED;
+    o.hl = Dual(desc->half_lambda, 0.5 * dlambda);
+    const size_t n = (size_t)desc->N_p * desc->N_c * desc->N_e;
+    vector<Dual> ud(n), rd(n);
+    for (size_t i = 0; i < n; ++i) ud[i] = Dual(u[i], du[i]);
+    o.residual(ud.data(), rd.data(), g_threads);
+    for (size_t i = 0; i < n; ++i) { dudt[i] = rd[i].v; ddudt[i] = rd[i].d; }
+    return 0;
+}
+
+// nsteps CK2N54 steps of (u, du) together: du becomes d u_N along (du_0, dlambda).
+int sse_oracle_rk_jvp(const sse_desc* desc, double* u, double* du, double dlambda, double dt, int nsteps) {
+    Oracle<Dual> o;
+    g_err = o.init(desc);
+    if (!g_err.empty()) return SSE_ERR_UNSUPPORTED;
+    o.hl = Dual(desc->half_lambda, 0.5 * dlambda);
+    const size_t n = (size_t)desc->N_p * desc->N_c * desc->N_e;
+    vector<Dual> ud(n), k(n), tmp(n, Dual(0.0));
+    for (size_t i = 0; i < n; ++i) ud[i] = Dual(u[i], du[i]);
+    for (int step = 0; step < nsteps; ++step)
+        for (int s = 0; s < 5; ++s) {
+            o.residual(ud.data(), k.data(), g_threads);
+            for (size_t i = 0; i < n; ++i) {
+                tmp[i] = (s == 0 ? Dual(0.0) : CK_A[s] * tmp[i]) + dt * k[i];
+                ud[i] += CK_B[s] * tmp[i];
+            }
+        }
+    for (size_t i = 0; i < n; ++i) { u[i] = ud[i].v; du[i] = ud[i].d; }
     return 0;
 }
 

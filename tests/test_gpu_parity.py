@@ -353,3 +353,67 @@ def test_linear_residual_is_linear_and_skew_adjoint():
 
     lhs, rhs = minner(x, Ry), -minner(y, Rx)
     assert abs(lhs - rhs) < 1e-11 * (abs(lhs) + 1.0), (lhs, rhs)
+
+
+# ----------------------------------------------------------------------------- adjoint
+def _ec_interface(mk):
+    from sse.conservation_laws import EntropyConservativeNumericalFlux
+    from sse.solvers import FluxDifferencingForm
+    return mk(FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux()))
+
+
+VJP_CASES = {
+    "wave3d_tet_p3_lf": lambda: density_wave_3d_config(M=2, p=3, ic_strength=0.2)[4],
+    "wave3d_tet_p3_ec": lambda: _ec_interface(lambda f: density_wave_3d_config(M=2, p=3, ic_strength=0.2, form=f))[4],
+    "vortex2d_tri_p4_lf": lambda: vortex_2d_config(M=3, p=4)[4],
+    "euler1d_gauss_nodal_ec": lambda: euler_1d_gauss()[4],
+}
+
+
+@pytest.mark.parametrize("name", sorted(VJP_CASES))
+def test_residual_vjp_dot_product_against_oracle_jvp(name):
+    """<ybar, R'(u) du + dR/dλ dλ> (oracle, forward-mode dual numbers) equals
+    <VJP(ybar), du> + lambda_bar dλ (GPU)."""
+    from sse.adjoint import residual_vjp
+    ode = VJP_CASES[name]()
+    u = _perturbed(ode, amp=1e-3)
+    rng = np.random.default_rng(17)
+    du = np.asfortranarray(rng.standard_normal(u.shape))
+    ybar = np.asfortranarray(rng.standard_normal(u.shape))
+    dlam = 0.7
+    r, dr = O.residual_jvp(ode.p.arrays, u, du, dlam)
+    ubar, lam_bar = residual_vjp(ode.p, u, ybar)
+    lhs = float(np.sum(ybar * dr))
+    rhs = float(np.sum(ubar * du)) + lam_bar * dlam
+    scale = float(np.sum(np.abs(ybar * dr)))
+    assert abs(lhs - rhs) <= 1e-11 * scale, (name, lhs, rhs, scale)
+    # the device-memory path gives the same pull-back
+    from sse.time_integration import to_device, to_host
+    ub_dev, lam_dev = residual_vjp(ode.p, to_device(u), to_device(ybar))
+    assert _rel(to_host(ub_dev), ubar) <= 1e-14 and abs(lam_dev - lam_bar) <= 1e-12 * (1 + abs(lam_bar))
+
+
+def test_rk_adjoint_matches_oracle_forward_mode():
+    """dJ along (du0, dλ) from the discrete adjoint (GPU, checkpoints every 3 of 8 steps)
+    equals the oracle's forward-mode derivative through the same 8 CK2N54 steps."""
+    from oracle import functionals as F
+    from sse.adjoint import rk_adjoint
+    law, sd, ic, form, ode = density_wave_3d_config(M=2, p=3, ic_strength=0.2)
+    u0 = _perturbed(ode, amp=1e-3)
+    dt, nsteps = 2e-3, 8
+    J, g, glam = rk_adjoint(ode.p, u0, dt, nsteps, checkpoint_every=3)
+    uN = O.rk_steps(ode.p.arrays, u0, dt, nsteps)
+    assert abs(J - F.entropy(sd, law, uN)[0]) <= 1e-12 * abs(J)
+    rng = np.random.default_rng(23)
+    du = np.asfortranarray(rng.standard_normal(u0.shape))
+    dlam = 0.5
+    uN2, duN = O.rk_jvp(ode.p.arrays, u0, du, dt, nsteps, dlam)
+    # dJ = <dJ/du_N, du_N>, dJ/du_N = V'(WJ w(V u_N)) per element
+    ref = sd.reference_approximation
+    V = ref.Vdense()
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q
+    from sse.conservation_laws import conservative_to_entropy
+    wq = conservative_to_entropy(law, np.einsum("qp,pck->qkc", V, uN2))
+    dJ_fwd = float(np.einsum("qk,qkc,qp,pck->", WJ, wq, V, duN))
+    dJ_adj = float(np.sum(g * du)) + glam * dlam
+    assert abs(dJ_fwd - dJ_adj) <= 1e-10 * (abs(dJ_fwd) + 1e-3), (dJ_fwd, dJ_adj)
