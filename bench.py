@@ -123,6 +123,86 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_adjoint(args):
+    """Discrete adjoint of 200 CK2N54 steps, 3D Euler p=3 on the curved-tet mesh at
+    12^3 hexes x 6 tets (10,368 elements): gradient of the final entropy integral
+    w.r.t. the initial state and the Lax-Friedrichs coefficient, checkpoints every
+    20 steps. Reports adjoint/forward cost and the central-FD agreement on 32 seeded
+    directions. Prints one JSON line."""
+    import torch
+
+    from sse.adjoint import rk_adjoint
+    from sse.time_integration import rk_steps, to_device, to_host
+
+    M, p, nsteps, every = args.adj_mesh, 3, args.adj_steps, 20
+    t0 = time.perf_counter()
+    law, sd, ic, form, ode = build_workload(M, p)
+    solver = ode.p
+    rng = np.random.default_rng(20261202)
+    u0 = np.asfortranarray(ode.u0 + 1e-3 * rng.standard_normal(ode.u0.shape))
+    dt = 0.2 * (2.0 / M) / ((2 * p + 1) * 2.9)
+    N_p, N_c, N_e = solver.size()
+    log(f"prep: {time.perf_counter() - t0:.1f} s, N_e={N_e}, dt={dt:.3e}")
+    ud0 = to_device(u0)
+    # warm-up both paths
+    u = ud0.clone()
+    rk_steps(solver, u, dt, 2)
+    rk_adjoint(solver, ud0, dt, 2, every)
+    torch.cuda.synchronize()
+    u = ud0.clone()
+    t0 = time.perf_counter()
+    rk_steps(solver, u, dt, nsteps)
+    solver.lib.sse_synchronize(solver.handle)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, g, glam = rk_adjoint(solver, ud0, dt, nsteps, every)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    log(f"forward {t_fwd:.3f} s, adjoint (forward sweep + reverse with recompute) {t_adj:.3f} s")
+    g = to_host(g)
+    # central finite differences of J on seeded random directions, h = 1e-6
+    from sse.analysis import EntropyConservationAnalysis
+    ent = EntropyConservationAnalysis(solver)
+
+    def J_of(x, lam_shift=0.0):
+        hl = solver.arrays.desc.half_lambda
+        if lam_shift:
+            solver.close()
+            solver.arrays.desc.half_lambda = hl + 0.5 * lam_shift
+        y = to_device(x)
+        rk_steps(solver, y, dt, nsteps)
+        val = float(ent.evaluate_conservation(y)[0])
+        if lam_shift:
+            solver.close()
+            solver.arrays.desc.half_lambda = hl
+        return val
+
+    h = 1e-6
+    errs = []
+    for t in range(args.adj_directions):
+        d = np.asfortranarray(rng.standard_normal(u0.shape))  # O(1) entries: each coefficient moves by ~h
+        fd = (J_of(u0 + h * d) - J_of(u0 - h * d)) / (2 * h)
+        ad = float(np.sum(g * d))
+        errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+    fd_lam = (J_of(u0, h) - J_of(u0, -h)) / (2 * h)
+    lam_err = abs(fd_lam - glam) / max(abs(glam), 1e-300)
+    log(f"FD check: max rel err over {len(errs)} directions {max(errs):.2e}; dJ/dλ {glam:.6e} vs FD {fd_lam:.6e}")
+    line = {
+        "metric": "adjoint/forward cost (3D Euler p=3 curved tets, 200 RK steps, entropy objective)",
+        "value": (t_adj - t_fwd) / t_fwd, "unit": "x", "n_gpus": 1, "steps": nsteps, "warmup": 2,
+        "ms_per_step": 1e3 * t_adj / nsteps, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Euler, curved tets, {M}^3 hexes x 6 tets, ModalTensor p={p}, "
+                               "EC + LF flux differencing, CK2N54, J = entropy integral at the final time, "
+                               f"checkpoints every {every} steps", "N_e": N_e, "dt": dt,
+                   "t_forward_s": t_fwd, "t_adjoint_total_s": t_adj,
+                   "adjoint_including_forward_sweep_over_forward": t_adj / t_fwd,
+                   "J": J, "dJ_dlambda": glam,
+                   "fd_max_rel_err": max(errs), "fd_directions": len(errs), "fd_lambda_rel_err": lam_err},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,10 +212,13 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="residual", choices=["residual", "batched"],
+    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint"],
                     help="residual: the curved-tet Euler p=4 residual (default); batched: the "
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--adj-mesh", type=int, default=12)
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-directions", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, run 5 copy kernels (dst = 1.0 * src) over this many MiB "
@@ -149,6 +232,8 @@ def main():
 
     if args.config == "batched":
         return bench_batched(args)
+    if args.config == "adjoint":
+        return bench_adjoint(args)
 
     import torch
 

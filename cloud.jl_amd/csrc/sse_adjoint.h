@@ -54,9 +54,136 @@ __device__ __forceinline__ void inv_logmean_grad(double x, double y, double& dx,
     }
 }
 
+// logmean(x, y) and its partial derivatives, from x, y and their logarithms
+// (no logarithm evaluated); the branch test is the forward one.
+__device__ __forceinline__ double logmean_vg(double x, double y, double lx, double ly, double ix, double iy,
+                                             double& dx, double& dy) {
+    const double f2 = (x * (x - 2.0 * y) + y * y) / (x * (x + 2.0 * y) + y * y);
+    if (f2 < 1.0e-4) {
+        const double r = 1.0 / (x + y);
+        const double q = (x - y) * r;
+        const double den = 210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0));
+        const double dden = 70.0 + f2 * (84.0 + f2 * 90.0);
+        const double g = 105.0 / den;
+        const double slope = -(x + y) * g * g * (1.0 / 105.0) * dden * 2.0 * q * 2.0 * r * r;
+        dx = g + slope * y;
+        dy = g - slope * x;
+        return (x + y) * g;
+    }
+    const double L = ly - lx;
+    const double iL = 1.0 / L;
+    const double lm = (y - x) * iL;
+    dx = (-1.0 + (y - x) * ix * iL) * iL;
+    dy = (1.0 - (y - x) * iy * iL) * iL;
+    return lm;
+}
+
+// inv_logmean(x, y) and its partial derivatives, from x, y and their logarithms
+__device__ __forceinline__ double inv_logmean_vg(double x, double y, double lx, double ly, double ix, double iy,
+                                                 double& dx, double& dy) {
+    const double f2 = (x * (x - 2.0 * y) + y * y) / (x * (x + 2.0 * y) + y * y);
+    if (f2 < 1.0e-4) {
+        const double r = 1.0 / (x + y);
+        const double q = (x - y) * r;
+        const double den = 210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0));
+        const double dden = 70.0 + f2 * (84.0 + f2 * 90.0);
+        const double c = r * (1.0 / 105.0);
+        const double a = dden * 2.0 * q * c * 2.0 * r * r;  // times y (x) gives the q-chain term
+        const double b = -den * c * r;
+        dx = a * y + b;
+        dy = -a * x + b;
+        return den * c;
+    }
+    const double L = ly - lx;
+    const double r = 1.0 / (y - x);
+    dx = (-ix + L * r) * r;
+    dy = (iy - L * r) * r;
+    return L * r;
+}
+
 template <int D>
 struct EulerAdj {
     static constexpr int NC = D + 2;
+    // hoisted per-node quantities: rho, v (D), p, log rho, beta = rho / p, log beta, 1 / rho
+    static constexpr int NH = D + 6;
+    enum { H_RHO = 0, H_V = 1, H_P = D + 1, H_LR = D + 2, H_B = D + 3, H_LB = D + 4, H_IR = D + 5 };
+
+    __device__ static void hoist_full(const LawParams& P, const double* u, double* h) {
+        const double ir = 1.0 / u[0];
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            h[H_V + m] = u[m + 1] * ir;
+            s += h[H_V + m] * u[m + 1];
+        }
+        const double p = P.gm1 * (u[D + 1] - 0.5 * s);
+        h[H_RHO] = u[0];
+        h[H_P] = p;
+        h[H_LR] = log(u[0]);
+        h[H_B] = u[0] / p;
+        h[H_LB] = log(h[H_B]);
+        h[H_IR] = ir;
+    }
+
+    // ec_dot_adj_side from hoisted states: adds the pull-back of Gb through
+    // G = sum_n a_n F_n(L, R) into the kept side's conservative cotangent ub.
+    __device__ static void ec_dot_adj_h(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                        const double* Gb, bool left, double* ub) {
+        double Vb[D];
+        double vv = 0.0, av = 0.0, avL = 0.0, avR = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            const double vl = hL[H_V + m], vr = hR[H_V + m];
+            vv += vl * vr;
+            Vb[m] = 0.5 * (vl + vr);
+            av += a[m] * Vb[m];
+            avL += a[m] * vl;
+            avR += a[m] * vr;
+        }
+        const double pL = hL[H_P], pR = hR[H_P];
+        double drx, dry, dbx, dby;
+        const double rho_avg = logmean_vg(hL[H_RHO], hR[H_RHO], hL[H_LR], hR[H_LR], hL[H_IR], hR[H_IR], drx, dry);
+        const double ilm = inv_logmean_vg(hL[H_B], hR[H_B], hL[H_LB], hR[H_LB], pL * hL[H_IR], pR * hR[H_IR], dbx, dby);
+        const double Cc = 0.5 * vv + P.inv_gm1 * ilm;
+        const double f = rho_avg * av;
+        double fb = Gb[0] + Gb[D + 1] * Cc;
+        double pavgb = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            fb += Gb[m + 1] * Vb[m];
+            pavgb += Gb[m + 1] * a[m];
+        }
+        const double Ccb = Gb[D + 1] * f;
+        const double rhoavgb = fb * av, avb = fb * rho_avg;
+        const double* hs = left ? hL : hR;
+        const double* ho = left ? hR : hL;
+        const double po = left ? pR : pL;
+        const double ps = left ? pL : pR;
+        const double avo = left ? avR : avL;
+        double vb[D];
+#pragma unroll
+        for (int m = 0; m < D; ++m)
+            vb[m] = 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * f + avb * a[m]) + 0.5 * Ccb * ho[H_V + m];
+        double pb = 0.5 * Gb[D + 1] * avo + 0.5 * pavgb;
+        const double bb = P.inv_gm1 * Ccb * (left ? dbx : dby);
+        const double ips = 1.0 / ps;
+        double rb = rhoavgb * (left ? drx : dry) + bb * ips;
+        pb += -bb * hs[H_B] * ips;
+        // (rho, v, p) -> (rho, m, E), p = gm1 (E - 1/2 rho |v|^2), v = m / rho
+        const double rho = hs[H_RHO], ir = hs[H_IR];
+        double s = 0.0, vdot = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) s += hs[H_V + m] * hs[H_V + m];
+        ub[D + 1] += P.gm1 * pb;
+        rb += -0.5 * P.gm1 * s * pb;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            const double vbt = vb[m] - P.gm1 * rho * hs[H_V + m] * pb;
+            ub[m + 1] += vbt * ir;
+            vdot += vbt * hs[H_V + m];
+        }
+        ub[0] += rb - vdot * ir;
+    }
 
     // (rho, v, p) cotangents -> conservative cotangents, added into ub
     __device__ static void prim_to_cons(const LawParams& P, const double* u, double rb, const double* vb, double pb,
@@ -141,6 +268,114 @@ struct EulerAdj {
         pRb += -bRb * uR[0] / (pR * pR);
         prim_to_cons(P, uL, rLb, vLb, pLb, uLb);
         prim_to_cons(P, uR, rRb, vRb, pRb, uRb);
+    }
+
+    // As ec_dot_adj, but adds the pull-back of one side only (left: uL, else uR)
+    // into ub: a single code path for every two-point term of a gathering node.
+    __device__ static void ec_dot_adj_side(const LawParams& P, const double* uL, const double* uR, const double* a,
+                                           const double* Gb, bool left, double* ub) {
+        double vL[D], vR[D], Vb[D];
+        double sL = 0.0, sR = 0.0, vv = 0.0, av = 0.0, avL = 0.0, avR = 0.0;
+        const double iL = 1.0 / uL[0], iR = 1.0 / uR[0];
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            vL[m] = uL[m + 1] * iL;
+            vR[m] = uR[m + 1] * iR;
+        }
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            sL += vL[m] * vL[m];
+            sR += vR[m] * vR[m];
+            vv += vL[m] * vR[m];
+            Vb[m] = 0.5 * (vL[m] + vR[m]);
+            av += a[m] * Vb[m];
+            avL += a[m] * vL[m];
+            avR += a[m] * vR[m];
+        }
+        const double pL = P.gm1 * (uL[D + 1] - 0.5 * uL[0] * sL);
+        const double pR = P.gm1 * (uR[D + 1] - 0.5 * uR[0] * sR);
+        const double bL = uL[0] / pL, bR = uR[0] / pR;
+        const double rho_avg = logmean(uL[0], uR[0]);
+        const double Cc = 0.5 * vv + P.inv_gm1 * inv_logmean(bL, bR);
+        const double f = rho_avg * av;
+        double fb = Gb[0] + Gb[D + 1] * Cc;
+        double pavgb = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            fb += Gb[m + 1] * Vb[m];
+            pavgb += Gb[m + 1] * a[m];
+        }
+        const double Ccb = Gb[D + 1] * f;
+        const double rhoavgb = fb * av, avb = fb * rho_avg;
+        // the kept side
+        const double* us = left ? uL : uR;
+        const double* vs = left ? vL : vR;      // this side's velocity
+        const double* vo = left ? vR : vL;      // the other side's velocity
+        const double po = left ? pR : pL;       // the other side's pressure
+        const double ps = left ? pL : pR;
+        const double avo = left ? avR : avL;    // a . v of the other side
+        double vb[D];
+#pragma unroll
+        for (int m = 0; m < D; ++m)
+            vb[m] = 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * f + avb * a[m]) + 0.5 * Ccb * vo[m];
+        double pb = 0.5 * Gb[D + 1] * avo + 0.5 * pavgb;
+        double gx, gy;
+        inv_logmean_grad(bL, bR, gx, gy);
+        const double bb = P.inv_gm1 * Ccb * (left ? gx : gy);
+        logmean_grad(uL[0], uR[0], gx, gy);
+        const double rb = rhoavgb * (left ? gx : gy) + bb / ps;
+        pb += -bb * us[0] / (ps * ps);
+        (void)vs;
+        prim_to_cons(P, us, rb, vb, pb, ub);
+    }
+
+    // Dissipation part of the Lax-Friedrichs flux, a (uin - uout) with
+    // a = halfλ λmax(uin, uout, n): adds the pull-back of fb into the kept side's
+    // cotangent ub (left: uin) and returns d/dhalfλ (fb . a (uin - uout)).
+    __device__ static double lf_diss_adj(const LawParams& P, const double* ui, const double* uo, const double* n,
+                                         const double* fb, bool left, double* ub) {
+        double si = 0.0, so = 0.0, vni = 0.0, vno = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            si += ui[m + 1] * ui[m + 1];
+            so += uo[m + 1] * uo[m + 1];
+            vni += (ui[m + 1] / ui[0]) * n[m];
+            vno += (uo[m + 1] / uo[0]) * n[m];
+        }
+        const double p_in = P.gm1 * (ui[D + 1] - (0.5 / ui[0]) * si);
+        const double p_out = P.gm1 * (uo[D + 1] - (0.5 / uo[0]) * so);
+        const double c_in = sqrt(P.gamma * p_in / ui[0]);
+        const double c_out = sqrt(P.gamma * p_out / uo[0]);
+        const bool vin_max = fabs(vni) > fabs(vno);
+        const bool cin_max = c_in > c_out;
+        const double lmax = (vin_max ? fabs(vni) : fabs(vno)) + (cin_max ? c_in : c_out);
+        const double a = P.half_lambda * lmax;
+        double ab = 0.0;
+        const double sgn = left ? 1.0 : -1.0;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            ub[e] += sgn * a * fb[e];
+            ab += fb[e] * (ui[e] - uo[e]);
+        }
+        const double lb = ab * P.half_lambda;
+        const double* us = left ? ui : uo;
+        if (vin_max == left) {  // the kept side sets the velocity part of λmax
+            const double vn = left ? vni : vno;
+            const double vnb = lb * (vn < 0.0 ? -1.0 : 1.0);
+#pragma unroll
+            for (int m = 0; m < D; ++m) ub[m + 1] += vnb * n[m] / us[0];
+            ub[0] += -vnb * vn / us[0];
+        }
+        if (cin_max == left) {  // and the sound-speed part
+            const double c = left ? c_in : c_out;
+            const double sc = left ? si : so;
+            const double pb = lb * P.gamma / (2.0 * c * us[0]);
+            ub[0] += -lb * c / (2.0 * us[0]) + P.gm1 * pb * 0.5 * sc / (us[0] * us[0]);
+#pragma unroll
+            for (int m = 0; m < D; ++m) ub[m + 1] += -P.gm1 * pb * us[m + 1] / us[0];
+            ub[D + 1] += P.gm1 * pb;
+        }
+        return ab * lmax;
     }
 
     // f* = F(uin, uout) . n + halfλ λmax (uin - uout) (Lax-Friedrichs, or only the

@@ -815,7 +815,7 @@ int vjp_check(sse_handle* h) {
     if (P.standard || P.N_c == 1) return fail(h, SSE_ERR_UNSUPPORTED, "the VJP covers the Euler flux-differencing residual");
     if (P.law.two_point_flux != SSE_TWO_POINT_EC)
         return fail(h, SSE_ERR_UNSUPPORTED, "the VJP covers the entropy-conservative two-point flux");
-    if (sse::lds_vjp_v2(P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the VJP kernels");
+    if (sse::lds_vjp_v2(P) > 160 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the VJP kernels");
     const size_t Ne = P.N_e;
     int rc;
     if (!h->vjp_rbar && ((rc = alloc(h, (size_t)P.N_q * P.N_c * Ne, &h->vjp_rbar)) ||
@@ -827,7 +827,9 @@ int vjp_check(sse_handle* h) {
 
 // ubar = (dR/du)' ybar on device buffers; lam_out (device) gets or accumulates scale * (dR/dλ)' ybar
 int vjp_device(sse_handle* h, const double* u, const double* ybar, double* ubar, double lam_scale, int accumulate) {
-    hipError_t e = sse::launch_generic_dispatch(h->P, u, nullptr, RKStage{}, h->stream, 1);  // u tilde
+    int rc = launch(h, u, nullptr, RKStage{}, 1);  // u tilde: the handle's own phase 1
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
     if (e == hipSuccess)
         e = sse::launch_vjp(h->P, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_lam, h->vjp_lamout, lam_scale,
                             accumulate, h->stream);

@@ -27,14 +27,14 @@ namespace {
 constexpr int VT = 256;
 
 // s_o (N_p x NC) = M^{-1} x for x (N_p x NC) in s_o; s_t (N_q x NC) scratch.
-__device__ void wadg_inplace(const DevParams& p, int k, double* s_o, double* s_t, int NC) {
+__device__ void wadg_inplace(const DevParams& p, const double* Vm, int k, double* s_o, double* s_t, int NC) {
     const int Nq = p.N_q, Np = p.N_p;
     const double* Jk = p.Jq + (size_t)k * Nq;
     for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
         const double jw = p.W[i] / Jk[i];
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
-            for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j + Np * e];
+            for (int j = 0; j < Np; ++j) s += Vm[i + Nq * j] * s_o[j + Np * e];
             s_t[i + Nq * e] = s * jw;
         }
     }
@@ -42,10 +42,17 @@ __device__ void wadg_inplace(const DevParams& p, int k, double* s_o, double* s_t
     for (int j = threadIdx.x; j < Np; j += blockDim.x)
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
-            for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_t[i + Nq * e];
+            for (int i = 0; i < Nq; ++i) s += Vm[i + Nq * j] * s_t[i + Nq * e];
             s_o[j + Np * e] = s;
         }
     __syncthreads();
+}
+
+// parts per node of the two-point term lists in phase V2
+__host__ __device__ inline int vjp_split(const DevParams& p, int threads) {
+    const int n = p.N_q > p.N_f ? p.N_q : p.N_f;
+    const int s = threads / n;
+    return s < 1 ? 1 : (s > 8 ? 8 : s);
 }
 
 __device__ double block_sum_vjp(double v, double* red) {
@@ -73,16 +80,19 @@ __global__ void __launch_bounds__(VT) k_vjp_v1(DevParams p, const double* __rest
     double* s_o = sm;               // Np x NC
     double* s_t = s_o + Np * NC;    // Nq x NC
     double* s_r = s_t + Nq * NC;    // Nq x NC
+    double* sV = s_r + Nq * NC;     // Nq x Np (V, when modal)
     const double* yk = ybar + (size_t)k * Np * NC;
     for (int t = threadIdx.x; t < Np * NC; t += blockDim.x) s_o[t] = yk[t];
+    if (p.has_V)
+        for (int t = threadIdx.x; t < Nq * Np; t += blockDim.x) sV[t] = p.V[t];
     __syncthreads();
     const double* Jk = p.Jq + (size_t)k * Nq;
     if (p.has_V) {
-        wadg_inplace(p, k, s_o, s_t, NC);
+        wadg_inplace(p, sV, k, s_o, s_t, NC);
         for (int i = threadIdx.x; i < Nq; i += blockDim.x)
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
-                for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j + Np * e];
+                for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
                 s_r[i + Nq * e] = s;
             }
     } else {
@@ -124,6 +134,9 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
     double* s_wf = s_wq + Nq * NC;        // Nf x NC   w_f = R w_q'
     double* s_t = s_wf + Nf * NC;         // Nq x NC   scratch
     double* s_o = s_t + Nq * NC;          // Np x NC   scratch
+    double* sV = s_o + Np * NC;           // Nq x Np   V (modal schemes)
+    const int nV = p.has_V ? Nq * Np : 0;
+    for (int t = threadIdx.x; t < nV; t += blockDim.x) sV[t] = p.V[t];
     const size_t kq = (size_t)k * Nq * NC, kf = (size_t)k * Nf * NC;
     for (int t = threadIdx.x; t < Nq * NC; t += blockDim.x) {
         s_uq[t] = p.uq[kq + t];
@@ -149,7 +162,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
 #pragma unroll
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
-                for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j + Np * e];
+                for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
                 ub[e] = s;
                 s_u0[i + Nq * e] = s;
             }
@@ -162,15 +175,15 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
         for (int j = threadIdx.x; j < Np; j += blockDim.x)
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
-                for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_t[i + Nq * e];
+                for (int i = 0; i < Nq; ++i) s += sV[i + Nq * j] * s_t[i + Nq * e];
                 s_o[j + Np * e] = s;
             }
         __syncthreads();
-        wadg_inplace(p, k, s_o, s_t, NC);
+        wadg_inplace(p, sV, k, s_o, s_t, NC);
         for (int i = threadIdx.x; i < Nq; i += blockDim.x)
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
-                for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j + Np * e];
+                for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
                 s_wq[i + Nq * e] = s;
             }
     } else {
@@ -194,122 +207,142 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
             s_wf[j + Nf * e] = s;
         }
 
-    // interface flux (own side and the neighbour's evaluation at the shared node),
-    // and the facet side of the facet correction
+    // Cotangents of the projected states. Work items: facet node j or volume node
+    // i, each split into `nsplit` parts over its list of two-point terms; parts
+    // meet in s_part and are added in a fixed order.
     const double* nJfk = p.nJf + (size_t)k * D * Nf;
     const double* Jfk = p.Jf + (size_t)k * Nf;
     const bool lf = (p.law.inviscid_flux == 1);
+    const int nsplit = vjp_split(p, blockDim.x);
+    double* s_part = sV + nV;                              // nsplit x (Nf + Nq) x NC
+    constexpr int NH = A::NH;
+    double* s_h = s_part + (size_t)nsplit * (Nf + Nq) * NC;  // (Nf + Nq) x NH: facet nodes, then volume nodes
+    for (int t = threadIdx.x; t < Nf + Nq; t += blockDim.x) {
+        double ut[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ut[e] = t < Nf ? s_uf[t + Nf * e] : s_uq[(t - Nf) + Nq * e];
+        A::hoist_full(p.law, ut, s_h + (size_t)t * NH);
+    }
+    __syncthreads();
     double lam = 0.0;
-    for (int j = threadIdx.x; j < Nf; j += blockDim.x) {
-        double uin[NC], uout[NC], nrm[D], nrm2[D], c[NC], ub[NC], junk[NC];
-        const int g = p.mapP[j + (size_t)Nf * k];
-        const int kn = g / Nf, jn = g - kn * Nf;
-        const double* ufn = p.uf + (size_t)kn * Nf * NC;
-        const double* fbn = fbar + (size_t)kn * Nf * NC;
+    for (int w = threadIdx.x; w < (Nf + Nq) * nsplit; w += blockDim.x) {
+        double ub[NC];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            uin[e] = s_uf[j + Nf * e];
-            uout[e] = ufn[jn + Nf * e];
-            ub[e] = 0.0;
-        }
-        const double jf = Jfk[j];
+        for (int e = 0; e < NC; ++e) ub[e] = 0.0;
+        const bool facet = w < Nf * nsplit;
+        const int part = facet ? w / Nf : (w - Nf * nsplit) / Nq;
+        const int node = facet ? w - part * Nf : w - Nf * nsplit - part * Nq;
+        // the gathering node's state and its list of two-point terms
+        double un[NC];
 #pragma unroll
-        for (int m = 0; m < D; ++m) nrm[m] = nJfk[m + D * j] / jf;
-        const double bj = p.B[j] * jf;
+        for (int e = 0; e < NC; ++e) un[e] = facet ? s_uf[node + Nf * e] : s_uq[node + Nq * e];
+        int nt;
+        if (facet) nt = 2 + (p.C_kind ? p.Cc_ptr[node + 1] - p.Cc_ptr[node] : 0);
+        else nt = (p.S_ptr[node + 1] - p.S_ptr[node]) + (p.C_kind ? p.Cr_ptr[node + 1] - p.Cr_ptr[node] : 0);
+        const int nS = facet ? 0 : p.S_ptr[node + 1] - p.S_ptr[node];
+        const double* hn = s_h + (size_t)(facet ? node : Nf + node) * NH;
+        for (int q = part; q < nt; q += nsplit) {
+            double xo[NC], a[D], gb[NC], hx[NH];
+            const double* ho = hx;  // partner's hoisted state
+            bool left = true;
+            int lfmode = 0;  // 1: this node's interface flux, 2: the neighbour's at the shared node
+            if (facet && q < 2) {
+                // interface flux: f*(u_in, u_out, n) at this node (keep u_in), and the
+                // neighbour's f*(u_out', u_in', n') at its node (keep its exterior state, ours)
+                const int j = node;
+                const int g = p.mapP[j + (size_t)Nf * k];
+                const int kn = g / Nf, jn = g - kn * Nf;
+                const double* ufn = p.uf + (size_t)kn * Nf * NC;
+                const int kk = q == 0 ? k : kn, jj = q == 0 ? j : jn;
+                const double jf = p.Jf[jj + (size_t)Nf * kk];
+                const double* nJ = p.nJf + (size_t)kk * D * Nf;
 #pragma unroll
-        for (int e = 0; e < NC; ++e) c[e] = bj * s_fb[j + Nf * e];
-        const double dh = A::lf_adj(p.law, lf, uin, uout, nrm, c, ub, junk);
-        lam += 0.5 * dh;  // halfλ = λ / 2
-        // the neighbour evaluates f*(u_out, u_in, n') at its node jn; our state is its exterior
-        const double* nJfn = p.nJf + (size_t)kn * D * Nf;
-        const double jfn = p.Jf[jn + (size_t)Nf * kn];
-#pragma unroll
-        for (int m = 0; m < D; ++m) nrm2[m] = nJfn[m + D * jn] / jfn;
-        const double bjn = p.B[jn] * jfn;
-#pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            c[e] = bjn * fbn[jn + Nf * e];
-            junk[e] = 0.0;
-        }
-        A::lf_adj(p.law, lf, uout, uin, nrm2, c, junk, ub);
-        // facet correction, facet side: -C_ij G(u_i, u_fj) enters r_q[i] and f_f[j]
-        if (p.C_kind) {
-            const int f = j / npf;
-            for (int t = p.Cc_ptr[j]; t < p.Cc_ptr[j + 1]; ++t) {
-                const int i = p.Cc_idx[t];
-                const double cij = p.Cc_val[t];
-                double ui[NC], nJ[D], gb[NC];
+                for (int m = 0; m < D; ++m) a[m] = nJ[m + D * jj] / jf;
+                const double bj = p.B[jj] * jf;
+                const double* fbs = q == 0 ? s_fb + j : fbar + (size_t)kn * Nf * NC + jn;
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
-                    ui[e] = s_uq[i + Nq * e];
-                    gb[e] = -cij * (s_rb[i + Nq * e] + s_fb[j + Nf * e]);
-                    junk[e] = 0.0;
+                    xo[e] = ufn[jn + Nf * e];
+                    gb[e] = bj * fbs[Nf * e];
                 }
+                left = (q == 0);
+                lfmode = q + 1;
+                A::hoist_full(p.law, xo, hx);
+            } else if (facet) {  // facet correction seen from facet node j (keep u_fj)
+                const int j = node, t = p.Cc_ptr[j] + q - 2;
+                const int i = p.Cc_idx[t];
+                const double cij = p.Cc_val[t];
+                const int f = j / npf;
 #pragma unroll
                 for (int m = 0; m < D; ++m) {
                     double s = 0.0;
 #pragma unroll
                     for (int l = 0; l < D; ++l) s += s_L[i + Nq * (l + D * m)] * p.fn[l + D * f];
-                    nJ[m] = 0.5 * nJfk[m + D * j] + 0.5 * s;
+                    a[m] = 0.5 * nJfk[m + D * j] + 0.5 * s;
                 }
-                A::ec_dot_adj(p.law, ui, uin, nJ, gb, junk, ub);
-            }
-        }
 #pragma unroll
-        for (int e = 0; e < NC; ++e) s_ubf[j + Nf * e] = ub[e];
-    }
-
-    // volume nodes: Hadamard sum and the volume side of the facet correction
-    for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
-        double ui[NC], uj[NC], ub[NC], junk[NC], gb[NC], an[D];
+                for (int e = 0; e < NC; ++e) {
+                    xo[e] = s_uq[i + Nq * e];
+                    gb[e] = -cij * (s_rb[i + Nq * e] + s_fb[j + Nf * e]);
+                }
+                ho = s_h + (size_t)(Nf + i) * NH;
+                left = false;
+            } else if (q < nS) {  // Hadamard sum term (i, j): cotangent (dT_ij/du_L)' (rbar_j - rbar_i)
+                const int i = node, t = p.S_ptr[i] + q;
+                const int j = p.S_idx[t];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            ui[e] = s_uq[i + Nq * e];
-            ub[e] = 0.0;
-        }
-        for (int t = p.S_ptr[i]; t < p.S_ptr[i + 1]; ++t) {
-            const int j = p.S_idx[t];
+                for (int n = 0; n < D; ++n) {
+                    double s = 0.0;
 #pragma unroll
-            for (int n = 0; n < D; ++n) {
-                double s = 0.0;
+                    for (int m = 0; m < D; ++m)
+                        s += p.S_val[t * D + m] * (s_L[i + Nq * (m + D * n)] + s_L[j + Nq * (m + D * n)]);
+                    a[n] = s;
+                }
 #pragma unroll
-                for (int m = 0; m < D; ++m)
-                    s += p.S_val[t * D + m] * (s_L[i + Nq * (m + D * n)] + s_L[j + Nq * (m + D * n)]);
-                an[n] = s;
-            }
-            // r_i -= T_ij, r_j -= T_ji = +T_ij: u_i's cotangent from both is (dT_ij/du_L)' (rbar_j - rbar_i)
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                uj[e] = s_uq[j + Nq * e];
-                gb[e] = s_rb[j + Nq * e] - s_rb[i + Nq * e];
-                junk[e] = 0.0;
-            }
-            A::ec_dot_adj(p.law, ui, uj, an, gb, ub, junk);
-        }
-        if (p.C_kind) {
-            for (int t = p.Cr_ptr[i]; t < p.Cr_ptr[i + 1]; ++t) {
+                for (int e = 0; e < NC; ++e) {
+                    xo[e] = s_uq[j + Nq * e];
+                    gb[e] = s_rb[j + Nq * e] - s_rb[i + Nq * e];
+                }
+                ho = s_h + (size_t)(Nf + j) * NH;
+            } else {  // facet correction seen from volume node i
+                const int i = node, t = p.Cr_ptr[i] + q - nS;
                 const int j = p.Cr_idx[t];
                 const double cij = p.Cr_val[t];
                 const int f = j / npf;
-                double nJ[D];
 #pragma unroll
                 for (int m = 0; m < D; ++m) {
                     double s = 0.0;
 #pragma unroll
                     for (int l = 0; l < D; ++l) s += s_L[i + Nq * (l + D * m)] * p.fn[l + D * f];
-                    nJ[m] = 0.5 * nJfk[m + D * j] + 0.5 * s;
+                    a[m] = 0.5 * nJfk[m + D * j] + 0.5 * s;
                 }
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
-                    uj[e] = s_uf[j + Nf * e];
+                    xo[e] = s_uf[j + Nf * e];
                     gb[e] = -cij * (s_rb[i + Nq * e] + s_fb[j + Nf * e]);
-                    junk[e] = 0.0;
                 }
-                A::ec_dot_adj(p.law, ui, uj, nJ, gb, ub, junk);
+                ho = s_h + (size_t)j * NH;
+            }
+            const double* xL = left ? un : xo;
+            const double* xR = left ? xo : un;
+            A::ec_dot_adj_h(p.law, left ? hn : ho, left ? ho : hn, a, gb, left, ub);
+            if (lfmode && lf) {
+                const double dh = A::lf_diss_adj(p.law, xL, xR, a, gb, left, ub);
+                if (lfmode == 1) lam += 0.5 * dh;  // halfλ = λ / 2
             }
         }
+        const int slot = facet ? node : Nf + node;
 #pragma unroll
-        for (int e = 0; e < NC; ++e) s_ubq[i + Nq * e] = ub[e];
+        for (int e = 0; e < NC; ++e) s_part[(part * (Nf + Nq) + slot) * NC + e] = ub[e];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < Nf + Nq; t += blockDim.x) {
+        for (int e = 0; e < NC; ++e) {
+            double s = 0.0;
+            for (int part = 0; part < nsplit; ++part) s += s_part[(part * (Nf + Nq) + t) * NC + e];
+            if (t < Nf) s_ubf[t + Nf * e] = s;
+            else s_ubq[(t - Nf) + Nq * e] = s;
+        }
     }
     __syncthreads();
 
@@ -356,18 +389,18 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
             for (int j = threadIdx.x; j < Np; j += blockDim.x)
                 for (int e = 0; e < NC; ++e) {
                     double s = 0.0;
-                    for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_t[i + Nq * e];
+                    for (int i = 0; i < Nq; ++i) s += sV[i + Nq * j] * s_t[i + Nq * e];
                     s_o[j + Np * e] = s;
                 }
             __syncthreads();
-            wadg_inplace(p, k, s_o, s_wq, NC);  // s_wq no longer needed
+            wadg_inplace(p, sV, k, s_o, s_wq, NC);  // s_wq no longer needed
             for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
                 const double wj = p.W[i] * Jk[i];
                 double wb[NC], u0[NC], ubv[NC];
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
                     double s = 0.0;
-                    for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j + Np * e];
+                    for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
                     wb[e] = s * wj;
                     u0[e] = s_u0[i + Nq * e];
                     ubv[e] = 0.0;
@@ -381,7 +414,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
             for (int j = threadIdx.x; j < Np; j += blockDim.x)
                 for (int e = 0; e < NC; ++e) {
                     double s = 0.0;
-                    for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_t[i + Nq * e];
+                    for (int i = 0; i < Nq; ++i) s += sV[i + Nq * j] * s_t[i + Nq * e];
                     outk[j + Np * e] = s;
                 }
         } else {
@@ -425,18 +458,28 @@ __global__ void __launch_bounds__(VT) k_vjp_sum(const double* __restrict__ parti
 }
 
 size_t lds_vjp_v1(const DevParams& p) {
-    return sizeof(double) * (size_t)(p.N_p + 2 * p.N_q) * p.N_c;
+    return sizeof(double) * ((size_t)(p.N_p + 2 * p.N_q) * p.N_c + (p.has_V ? (size_t)p.N_q * p.N_p : 0));
 }
 
 size_t lds_vjp_v2(const DevParams& p) {
     return sizeof(double) * ((size_t)VT + (size_t)(6 * p.N_q + 5 * p.N_f) * p.N_c + (size_t)p.N_q * p.d * p.d +
-                             (size_t)p.N_p * p.N_c);
+                             (size_t)p.N_p * p.N_c + (size_t)vjp_split(p, VT) * (p.N_q + p.N_f) * p.N_c +
+                             (size_t)(p.N_q + p.N_f) * (p.d + 6) + (p.has_V ? (size_t)p.N_q * p.N_p : 0));
 }
 
 template <int D>
 hipError_t launch_vjp_t(const DevParams& p, const double* u, const double* ybar, double* ubar, double* rbar,
                         double* fbar, double* lam_partial, double* lam_out, double lam_scale, int accumulate,
                         hipStream_t s) {
+    static bool attr = false;  // allow the full LDS of a CU for phase V2
+    if (!attr) {
+        hipError_t ea = hipFuncSetAttribute((const void*)k_vjp_v2<D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            160 * 1024);
+        if (ea == hipSuccess)
+            ea = hipFuncSetAttribute((const void*)k_vjp_v1<D>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (ea != hipSuccess) return ea;
+        attr = true;
+    }
     hipLaunchKernelGGL((k_vjp_v1<D>), dim3(p.N_e), dim3(VT), lds_vjp_v1(p), s, p, ybar, rbar, fbar);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
