@@ -27,24 +27,58 @@ namespace {
 constexpr int VT = 256;
 
 // s_o (N_p x NC) = M^{-1} x for x (N_p x NC) in s_o; s_t (N_q x NC) scratch.
-__device__ void wadg_inplace(const DevParams& p, const double* Vm, int k, double* s_o, double* s_t, int NC) {
+// LDS copy of V with an odd leading dimension, so that threads over i (V x) and
+// threads over j (V' x) both read without bank conflicts
+__host__ __device__ inline int vjp_ldv(const DevParams& p) { return p.N_q | 1; }
+
+// y (N_q x NC) = diag(scale) V x for x (N_p x NC); threads over (i, e), two
+// accumulators per dot product. scale: 0 none, 1 W / J_k, 2 W J_k.
+__device__ void v_apply(const DevParams& p, const double* Vm, int ldv, const double* Jk, int mode, int NC,
+                        const double* x, double* y) {
     const int Nq = p.N_q, Np = p.N_p;
-    const double* Jk = p.Jq + (size_t)k * Nq;
-    for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
-        const double jw = p.W[i] / Jk[i];
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int j = 0; j < Np; ++j) s += Vm[i + Nq * j] * s_o[j + Np * e];
-            s_t[i + Nq * e] = s * jw;
+    for (int t = threadIdx.x; t < Nq * NC; t += blockDim.x) {
+        const int i = t % Nq, e = t / Nq;
+        const double* xe = x + Np * e;
+        double s0 = 0.0, s1 = 0.0;
+        int j = 0;
+        for (; j + 1 < Np; j += 2) {
+            s0 += Vm[i + ldv * j] * xe[j];
+            s1 += Vm[i + ldv * (j + 1)] * xe[j + 1];
         }
+        if (j < Np) s0 += Vm[i + ldv * j] * xe[j];
+        double sc = 1.0;
+        if (mode == 1) sc = p.W[i] / Jk[i];
+        else if (mode == 2) sc = p.W[i] * Jk[i];
+        y[i + Nq * e] = (s0 + s1) * sc;
     }
-    __syncthreads();
-    for (int j = threadIdx.x; j < Np; j += blockDim.x)
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int i = 0; i < Nq; ++i) s += Vm[i + Nq * j] * s_t[i + Nq * e];
-            s_o[j + Np * e] = s;
+}
+
+// y (N_p x NC) = V' x for x (N_q x NC); threads over (j, e)
+__device__ void vt_apply(const DevParams& p, const double* Vm, int ldv, int NC, const double* x, double* y) {
+    const int Nq = p.N_q, Np = p.N_p;
+    for (int t = threadIdx.x; t < Np * NC; t += blockDim.x) {
+        const int j = t % Np, e = t / Np;
+        const double* xe = x + Nq * e;
+        const double* vj = Vm + ldv * j;
+        double s0 = 0.0, s1 = 0.0;
+        int i = 0;
+        for (; i + 1 < Nq; i += 2) {
+            s0 += vj[i] * xe[i];
+            s1 += vj[i + 1] * xe[i + 1];
         }
+        if (i < Nq) s0 += vj[i] * xe[i];
+        y[j + Np * e] = s0 + s1;
+    }
+}
+
+// s_o (N_p x NC) = M^{-1} s_o = V' (W / J_k (V s_o)) (weight-adjusted, M^{-1} = I);
+// s_t (N_q x NC) scratch. Vm: V in LDS, Vm[i + ldv j] = V[i, j].
+__device__ void wadg_inplace(const DevParams& p, const double* Vm, int ldv, int k, double* s_o, double* s_t,
+                             int NC) {
+    const double* Jk = p.Jq + (size_t)k * p.N_q;
+    v_apply(p, Vm, ldv, Jk, 1, NC, s_o, s_t);
+    __syncthreads();
+    vt_apply(p, Vm, ldv, NC, s_t, s_o);
     __syncthreads();
 }
 
@@ -80,21 +114,17 @@ __global__ void __launch_bounds__(VT) k_vjp_v1(DevParams p, const double* __rest
     double* s_o = sm;               // Np x NC
     double* s_t = s_o + Np * NC;    // Nq x NC
     double* s_r = s_t + Nq * NC;    // Nq x NC
-    double* sV = s_r + Nq * NC;     // Nq x Np (V, when modal)
+    double* sV = s_r + Nq * NC;     // V (when modal), leading dimension ldv
+    const int ldv = vjp_ldv(p);
     const double* yk = ybar + (size_t)k * Np * NC;
     for (int t = threadIdx.x; t < Np * NC; t += blockDim.x) s_o[t] = yk[t];
     if (p.has_V)
-        for (int t = threadIdx.x; t < Nq * Np; t += blockDim.x) sV[t] = p.V[t];
+        for (int t = threadIdx.x; t < Nq * Np; t += blockDim.x) sV[(t % Nq) + ldv * (t / Nq)] = p.V[t];
     __syncthreads();
     const double* Jk = p.Jq + (size_t)k * Nq;
     if (p.has_V) {
-        wadg_inplace(p, sV, k, s_o, s_t, NC);
-        for (int i = threadIdx.x; i < Nq; i += blockDim.x)
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
-                s_r[i + Nq * e] = s;
-            }
+        wadg_inplace(p, sV, ldv, k, s_o, s_t, NC);
+        v_apply(p, sV, ldv, Jk, 0, NC, s_o, s_r);
     } else {
         for (int i = threadIdx.x; i < Nq; i += blockDim.x)
             for (int e = 0; e < NC; ++e) s_r[i + Nq * e] = s_o[i + Np * e] / (p.W[i] * Jk[i]);
@@ -121,8 +151,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
     extern __shared__ double sm[];
     const int k = blockIdx.x;
     const int Np = p.N_p, Nq = p.N_q, Nf = p.N_f, npf = p.npf;
-    double* red = sm;                     // VT
-    double* s_uq = red + VT;              // Nq x NC   projected volume states (u tilde)
+    double* s_uq = sm;                    // Nq x NC   projected volume states (u tilde)
     double* s_uf = s_uq + Nq * NC;        // Nf x NC   projected facet states
     double* s_rb = s_uf + Nf * NC;        // Nq x NC   rbar
     double* s_fb = s_rb + Nq * NC;        // Nf x NC   fbar
@@ -134,9 +163,10 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
     double* s_wf = s_wq + Nq * NC;        // Nf x NC   w_f = R w_q'
     double* s_t = s_wf + Nf * NC;         // Nq x NC   scratch
     double* s_o = s_t + Nq * NC;          // Np x NC   scratch
-    double* sV = s_o + Np * NC;           // Nq x Np   V (modal schemes)
-    const int nV = p.has_V ? Nq * Np : 0;
-    for (int t = threadIdx.x; t < nV; t += blockDim.x) sV[t] = p.V[t];
+    double* sV = s_o + Np * NC;           // V (modal schemes), leading dimension ldv
+    const int ldv = vjp_ldv(p);
+    const int nV = p.has_V ? ldv * Np : 0;
+    for (int t = threadIdx.x; t < (p.has_V ? Nq * Np : 0); t += blockDim.x) sV[(t % Nq) + ldv * (t / Nq)] = p.V[t];
     const size_t kq = (size_t)k * Nq * NC, kf = (size_t)k * Nf * NC;
     for (int t = threadIdx.x; t < Nq * NC; t += blockDim.x) {
         s_uq[t] = p.uq[kq + t];
@@ -157,35 +187,22 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
     // recompute the entropy projection's intermediates
     const double* Jk = p.Jq + (size_t)k * Nq;
     if (p.has_V) {
+        v_apply(p, sV, ldv, Jk, 0, NC, s_o, s_u0);  // V u
+        __syncthreads();
         for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
             double ub[NC], wb[NC];
 #pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
-                ub[e] = s;
-                s_u0[i + Nq * e] = s;
-            }
+            for (int e = 0; e < NC; ++e) ub[e] = s_u0[i + Nq * e];
             Ph::c2e(p.law, ub, wb);
             const double wj = p.W[i] * Jk[i];
 #pragma unroll
             for (int e = 0; e < NC; ++e) s_t[i + Nq * e] = wb[e] * wj;
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < Np; j += blockDim.x)
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int i = 0; i < Nq; ++i) s += sV[i + Nq * j] * s_t[i + Nq * e];
-                s_o[j + Np * e] = s;
-            }
+        vt_apply(p, sV, ldv, NC, s_t, s_o);
         __syncthreads();
-        wadg_inplace(p, sV, k, s_o, s_t, NC);
-        for (int i = threadIdx.x; i < Nq; i += blockDim.x)
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
-                s_wq[i + Nq * e] = s;
-            }
+        wadg_inplace(p, sV, ldv, k, s_o, s_t, NC);
+        v_apply(p, sV, ldv, Jk, 0, NC, s_o, s_wq);
     } else {
         for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
             double ub[NC], wb[NC];
@@ -386,22 +403,16 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
         __syncthreads();
         if (p.has_V) {
             // w_q' = V w ; w = M^{-1} wt ; wt = V' (WJ w_q) ; w_q = c2e(V u)
-            for (int j = threadIdx.x; j < Np; j += blockDim.x)
-                for (int e = 0; e < NC; ++e) {
-                    double s = 0.0;
-                    for (int i = 0; i < Nq; ++i) s += sV[i + Nq * j] * s_t[i + Nq * e];
-                    s_o[j + Np * e] = s;
-                }
+            vt_apply(p, sV, ldv, NC, s_t, s_o);
             __syncthreads();
-            wadg_inplace(p, sV, k, s_o, s_wq, NC);  // s_wq no longer needed
+            wadg_inplace(p, sV, ldv, k, s_o, s_wq, NC);  // s_wq no longer needed
+            v_apply(p, sV, ldv, Jk, 2, NC, s_o, s_wq);   // WJ (V wt_bar)
+            __syncthreads();
             for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
-                const double wj = p.W[i] * Jk[i];
                 double wb[NC], u0[NC], ubv[NC];
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
-                    double s = 0.0;
-                    for (int j = 0; j < Np; ++j) s += sV[i + Nq * j] * s_o[j + Np * e];
-                    wb[e] = s * wj;
+                    wb[e] = s_wq[i + Nq * e];
                     u0[e] = s_u0[i + Nq * e];
                     ubv[e] = 0.0;
                 }
@@ -410,13 +421,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
                 for (int e = 0; e < NC; ++e) s_t[i + Nq * e] = ubv[e];
             }
             __syncthreads();
-            double* outk = ubar + (size_t)k * Np * NC;
-            for (int j = threadIdx.x; j < Np; j += blockDim.x)
-                for (int e = 0; e < NC; ++e) {
-                    double s = 0.0;
-                    for (int i = 0; i < Nq; ++i) s += sV[i + Nq * j] * s_t[i + Nq * e];
-                    outk[j + Np * e] = s;
-                }
+            vt_apply(p, sV, ldv, NC, s_t, ubar + (size_t)k * Np * NC);
         } else {
             // nodal, general R: u tilde_q = u, w_q = c2e(u)
             double* outk = ubar + (size_t)k * Np * NC;
@@ -443,6 +448,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
                 outk[i + Np * e] = s;
             }
     }
+    double* red = s_part;  // free again: VT doubles for the block sum
     const double tot = block_sum_vjp(lam, red);
     if (threadIdx.x == 0) lam_partial[k] = tot;
 }
@@ -458,13 +464,15 @@ __global__ void __launch_bounds__(VT) k_vjp_sum(const double* __restrict__ parti
 }
 
 size_t lds_vjp_v1(const DevParams& p) {
-    return sizeof(double) * ((size_t)(p.N_p + 2 * p.N_q) * p.N_c + (p.has_V ? (size_t)p.N_q * p.N_p : 0));
+    return sizeof(double) * ((size_t)(p.N_p + 2 * p.N_q) * p.N_c + (p.has_V ? (size_t)vjp_ldv(p) * p.N_p : 0));
 }
 
 size_t lds_vjp_v2(const DevParams& p) {
-    return sizeof(double) * ((size_t)VT + (size_t)(6 * p.N_q + 5 * p.N_f) * p.N_c + (size_t)p.N_q * p.d * p.d +
-                             (size_t)p.N_p * p.N_c + (size_t)vjp_split(p, VT) * (p.N_q + p.N_f) * p.N_c +
-                             (size_t)(p.N_q + p.N_f) * (p.d + 6) + (p.has_V ? (size_t)p.N_q * p.N_p : 0));
+    size_t part = (size_t)vjp_split(p, VT) * (p.N_q + p.N_f) * p.N_c;
+    if (part < (size_t)VT) part = VT;  // also the block-sum buffer
+    return sizeof(double) * ((size_t)(6 * p.N_q + 5 * p.N_f) * p.N_c + (size_t)p.N_q * p.d * p.d +
+                             (size_t)p.N_p * p.N_c + part + (size_t)(p.N_q + p.N_f) * (p.d + 6) +
+                             (p.has_V ? (size_t)vjp_ldv(p) * p.N_p : 0));
 }
 
 template <int D>
