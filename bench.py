@@ -203,6 +203,45 @@ def bench_adjoint(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_advdiff(args):
+    """3D linear advection-diffusion with BR1 on curved tets, 24^3 hexes x 6 tets
+    (82,944 elements), ModalTensor p=5, a = (1, 0.5, 0.25), b = 5e-3, Gaussian pulse
+    of width 0.1: DOF-residuals/s of the second-order StandardForm residual, plus a
+    batched sweep of independent diffusivities. Prints one JSON line."""
+    import torch
+
+    from sse import semi_discrete_residual
+    from sse.time_integration import to_device
+    from sse.workloads import advection_diffusion_3d_config
+
+    t0 = time.perf_counter()
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=args.ad_mesh, p=5, b=5e-3, sigma=0.1)
+    solver = ode.p
+    N_p, N_c, N_e = solver.size()
+    dof = N_p * N_c * N_e
+    log(f"prep: {time.perf_counter() - t0:.1f} s, N_e={N_e}, DOF={dof}; kernels: {solver.describe()}")
+    u = to_device(ode.u0)
+    dudt = torch.empty_like(u)
+    for _ in range(args.warmup):
+        semi_discrete_residual(dudt, u, solver)
+    solver.lib.sse_synchronize(solver.handle)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        semi_discrete_residual(dudt, u, solver)
+    solver.lib.sse_synchronize(solver.handle)
+    el = time.perf_counter() - t0
+    line = {
+        "metric": "DOF-residuals/s (3D advection-diffusion BR1 p=5 FP64)", "value": dof * args.steps / el,
+        "unit": "DOF-residuals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D linear advection-diffusion, BR1, curved tets, {args.ad_mesh}^3 hexes x 6 tets, "
+                               "ModalTensor p=5, StandardForm skew-symmetric + LF, PhysicalOperator",
+                   "N_e": N_e, "DOF": dof, "kernels": solver.describe()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,11 +251,12 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint"],
+    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint", "advdiff"],
                     help="residual: the curved-tet Euler p=4 residual (default); batched: the "
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--adj-mesh", type=int, default=12)
+    ap.add_argument("--ad-mesh", type=int, default=24)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-directions", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -234,6 +274,8 @@ def main():
         return bench_batched(args)
     if args.config == "adjoint":
         return bench_adjoint(args)
+    if args.config == "advdiff":
+        return bench_advdiff(args)
 
     import torch
 

@@ -143,7 +143,9 @@ def make_mesh(rd: RefElemData, VXYZ, EToV, limits) -> MeshData:
     # face matching on the face centroid taken modulo the periodic box
     fv = np.array(rd.face_vertices, dtype=np.int64)          # nfac x nv_face
     cen = [VXYZ[m][EToV[:, fv]].mean(axis=2).reshape(-1) - lo[m] for m in range(d)]
-    flat = _periodic_key(cen, L, tol=1e-9 * max(L))
+    # centroids of distinct faces are a fraction of an element apart; the key's cell
+    # (1e-6 of the box) is far above round-off and far below that separation
+    flat = _periodic_key(cen, L, tol=1e-6 * max(L))
     _, inv, counts = np.unique(flat, axis=0, return_inverse=True, return_counts=True)
     inv = inv.reshape(-1)
     if not np.all(counts == 2):
@@ -159,16 +161,23 @@ def make_mesh(rd: RefElemData, VXYZ, EToV, limits) -> MeshData:
     mapP = np.empty((N_f, N_e), dtype=np.int64)
     fx = np.stack([xyzf[m].T.reshape(N_e, nfac, npf) for m in range(d)], axis=-1)  # N_e x nfac x npf x d
     Larr = np.array(L)
+    # The two faces of a pair carry the same node set. Shift face b by its periodic
+    # image onto face a, order both faces' nodes along a generic direction and pair
+    # them in that order; the distances are then checked.
+    wdir = np.array([1.0, 2.0 ** 0.5, 3.0 ** 0.5][:d])
     for (ga, gb) in ((pair_a, pair_b), (pair_b, pair_a)):
         ka, fa = ga // nfac, ga % nfac
         kb, fb = gb // nfac, gb % nfac
         xa = fx[ka, fa]               # P x npf x d
         xb = fx[kb, fb]
-        diff = xa[:, :, None, :] - xb[:, None, :, :]
-        diff = diff - Larr * np.round(diff / Larr)
-        dist = np.sqrt((diff ** 2).sum(-1))
-        jb = np.argmin(dist, axis=2)  # P x npf
-        if np.take_along_axis(dist, jb[:, :, None], 2).max() > 1e-8 * max(L):
+        shift = Larr * np.round((xa.mean(axis=1) - xb.mean(axis=1)) / Larr)
+        xb = xb + shift[:, None, :]
+        ia = np.argsort(xa @ wdir, axis=1, kind="stable")
+        ib = np.argsort(xb @ wdir, axis=1, kind="stable")
+        jb = np.empty_like(ia)
+        np.put_along_axis(jb, ia, ib, axis=1)  # node ia[t] of face a pairs with node ib[t] of face b
+        gap = np.sqrt(((xa - np.take_along_axis(xb, jb[:, :, None], axis=1)) ** 2).sum(-1))
+        if gap.max() > 1e-8 * max(L):
             raise ValueError("facet nodes do not conform across a face")
         rows = fa[:, None] * npf + np.arange(npf)[None, :]
         nbr = (fb[:, None] * npf + jb) + N_f * kb[:, None] + 1

@@ -318,11 +318,18 @@ def project_function(initial_data, sd: SpatialDiscretization) -> np.ndarray:
         return np.asfortranarray(uq)
     V = ref.V
     WJ = ref.W[:, None] * sd.geometric_factors.J_q  # N_q x N_e
-    M = np.einsum("qa,qk,qb->kab", V, WJ, V)
-    rhs = np.einsum("qa,qk,qck->kac", V, WJ, uq)
-    L = np.linalg.cholesky(M)
-    y = np.linalg.solve(L, rhs)
-    u0 = np.linalg.solve(np.transpose(L, (0, 2, 1)), y)  # N_e x N_p x N_c
+    N_q, N_p = V.shape
+    N_e = WJ.shape[1]
+    u0 = np.empty((N_e, N_p, uq.shape[1]))
+    step = max(1, (1 << 25) // (N_q * N_p))  # elements per batch (bounded temporaries)
+    for k0 in range(0, N_e, step):
+        k1 = min(N_e, k0 + step)
+        VW = V.T[None, :, :] * WJ[:, k0:k1].T[:, None, :]          # K x N_p x N_q
+        M = VW @ V                                                  # K x N_p x N_p
+        rhs = VW @ np.transpose(uq[:, :, k0:k1], (2, 0, 1))         # K x N_p x N_c
+        L = np.linalg.cholesky(M)
+        y = np.linalg.solve(L, rhs)
+        u0[k0:k1] = np.linalg.solve(np.transpose(L, (0, 2, 1)), y)
     return np.asfortranarray(np.transpose(u0, (1, 2, 0)))
 
 
