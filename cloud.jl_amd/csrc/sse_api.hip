@@ -652,8 +652,10 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     const int N_q = D->N_q, N_f = D->N_f, N_p = D->N_p, N_e = D->N_e;
     // operators
     if (P.has_V) {
-        std::vector<double> V(D->V, D->V + (size_t)N_q * N_p);
-        if ((rc = upload(h, V, &P.V))) return cleanup_fail(rc);
+        std::vector<double> V(D->V, D->V + (size_t)N_q * N_p), Vt((size_t)N_q * N_p);
+        for (int i = 0; i < N_q; ++i)
+            for (int j = 0; j < N_p; ++j) Vt[j + (size_t)N_p * i] = V[i + (size_t)N_q * j];
+        if ((rc = upload(h, V, &P.V)) || (rc = upload(h, Vt, &P.Vt))) return cleanup_fail(rc);
     }
     Rows R, Rt;
     if (D->R_selection) {

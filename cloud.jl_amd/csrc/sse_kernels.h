@@ -23,6 +23,7 @@ struct DevParams {
     double b;          // diffusion coefficient
     LawParams law;
     const double* V;  // N_q x N_p column-major
+    const double* Vt; // V' (N_p x N_q column-major): coalesced reads for threads over modes
     // R rows: facet node j -> (volume node, value)
     const int* R_ptr; const int* R_idx; const double* R_val;
     // R^T rows: volume node i -> (facet node, value)

@@ -97,6 +97,41 @@ __device__ void volume_term(const DevParams& p, int k, int n, const double* s_x,
     __syncthreads();
 }
 
+// y (N_p x NC) = V' x (x: N_q x NC), threads over (j, e), reading V' from global
+// memory with consecutive j (coalesced), two accumulators per dot product
+__device__ void std_vt(const DevParams& p, int NC, const double* x, double* y) {
+    const int Nq = p.N_q, Np = p.N_p;
+    for (int t = threadIdx.x; t < Np * NC; t += blockDim.x) {
+        const int j = t % Np, e = t / Np;
+        const double* xe = x + Nq * e;
+        double s0 = 0.0, s1 = 0.0;
+        int i = 0;
+        for (; i + 1 < Nq; i += 2) {
+            s0 += p.Vt[j + (size_t)Np * i] * xe[i];
+            s1 += p.Vt[j + (size_t)Np * (i + 1)] * xe[i + 1];
+        }
+        if (i < Nq) s0 += p.Vt[j + (size_t)Np * i] * xe[i];
+        y[j + Np * e] = s0 + s1;
+    }
+}
+
+// y (N_q x NC) = scale V x (x: N_p x NC), threads over (i, e); scale = W / J_k if wj
+__device__ void std_v(const DevParams& p, const double* Jk, bool wj, int NC, const double* x, double* y) {
+    const int Nq = p.N_q, Np = p.N_p;
+    for (int t = threadIdx.x; t < Nq * NC; t += blockDim.x) {
+        const int i = t % Nq, e = t / Nq;
+        const double* xe = x + Np * e;
+        double s0 = 0.0, s1 = 0.0;
+        int j = 0;
+        for (; j + 1 < Np; j += 2) {
+            s0 += p.V[i + (size_t)Nq * j] * xe[j];
+            s1 += p.V[i + (size_t)Nq * (j + 1)] * xe[j + 1];
+        }
+        if (j < Np) s0 += p.V[i + (size_t)Nq * j] * xe[j];
+        y[i + Nq * e] = (s0 + s1) * (wj ? p.W[i] / Jk[i] : 1.0);
+    }
+}
+
 // s_o = M^{-1} V' s_r (weight-adjusted with M^{-1} = I, or diagonal W J); s_t is scratch.
 __device__ void mass_solve_vt(const DevParams& p, int k, const double* s_r, double* s_t, double* s_o) {
     const int NC = p.N_c, Nq = p.N_q, Np = p.N_p;
@@ -109,28 +144,11 @@ __device__ void mass_solve_vt(const DevParams& p, int k, const double* s_r, doub
         __syncthreads();
         return;
     }
-    for (int j = threadIdx.x; j < Np; j += blockDim.x)  // V' r
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_r[i + Nq * e];
-            s_o[j + Np * e] = s;
-        }
+    std_vt(p, NC, s_r, s_o);  // V' r
     __syncthreads();
-    for (int i = threadIdx.x; i < Nq; i += blockDim.x) {  // J^{-1} W (V rhs)
-        const double jw = p.W[i] / Jk[i];
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j + Np * e];
-            s_t[i + Nq * e] = s * jw;
-        }
-    }
+    std_v(p, Jk, true, NC, s_o, s_t);  // J^{-1} W (V rhs)
     __syncthreads();
-    for (int j = threadIdx.x; j < Np; j += blockDim.x)
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_t[i + Nq * e];
-            s_o[j + Np * e] = s;
-        }
+    std_vt(p, NC, s_t, s_o);
     __syncthreads();
 }
 
@@ -175,16 +193,15 @@ __global__ void __launch_bounds__(256) k_auxiliary_standard(DevParams p) {
         // q_q = V q_m, q_f = R q_q
         double* qqk = p.qq + ((size_t)k * D + m) * Nq;
         double* qfk = p.qf + ((size_t)k * D + m) * Nf;
-        for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
-            double s;
-            if (p.has_V) {
-                s = 0.0;
-                for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_o[j];
-            } else {
-                s = s_o[i];
+        if (p.has_V) {
+            std_v(p, nullptr, false, 1, s_o, s_x);
+            __syncthreads();
+            for (int i = threadIdx.x; i < Nq; i += blockDim.x) qqk[i] = s_x[i];
+        } else {
+            for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
+                s_x[i] = s_o[i];
+                qqk[i] = s_o[i];
             }
-            s_x[i] = s;
-            qqk[i] = s;
         }
         __syncthreads();
         for (int j = threadIdx.x; j < Nf; j += blockDim.x) {
