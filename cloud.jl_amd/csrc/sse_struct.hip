@@ -185,13 +185,15 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         }
         if (tl < NQ) {
             double hj[NH], av[D], c[NC];
+            // tensor indices of node tl (last index fastest) and the family strides
+            const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
+            const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
+            int f = q0 / (N1 - 1), dlt = q0 - f * (N1 - 1) + 1;
 #pragma unroll 1
-            for (int q = half * QH; q < (half + 1) * QH && q < NPART; ++q) {
-                const int f = q / (N1 - 1), dlt = q - f * (N1 - 1) + 1;
+            for (int q = q0; q < q1; ++q) {
                 {
-                    int stride = 1;
-                    for (int g = f + 1; g < D; ++g) stride *= N1;
-                    const int af = (tl / stride) % N1;  // tensor index of node tl in family f
+                    const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
+                    const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
                     const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
                     const int j = tl + (b - af) * stride;
 #pragma unroll
@@ -207,6 +209,10 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                     ec_dot<D>(p.law, hi, hj, av, c);
 #pragma unroll
                     for (int e = 0; e < NC; ++e) r[e] -= c[e];
+                }
+                if (++dlt == N1) {  // next family
+                    dlt = 1;
+                    ++f;
                 }
             }
             const int wh = (st.wcr + SPLIT - 1) / SPLIT;

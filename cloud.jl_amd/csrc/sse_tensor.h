@@ -51,7 +51,7 @@ SSE_TUNE(ELEM_TRI, 5, 0, 64, 2, 5, 64, 2)
 SSE_TUNE(ELEM_TRI, 6, 0, 64, 2, 7, 64, 2)
 SSE_TUNE(ELEM_TET, 2, 0, 64, 2, 3, 64, 2)
 SSE_TUNE(ELEM_TET, 3, 0, 128, 1, 3, 64, 2)
-SSE_TUNE(ELEM_TET, 4, 0, 128, 1, 2, 128, 1)
+SSE_TUNE(ELEM_TET, 4, 0, 128, 1, 2, 128, 2)
 SSE_TUNE(ELEM_TET, 4, 1, 128, 1, 2, 64, 1)
 SSE_TUNE(ELEM_TET, 4, 2, 128, 1, 2, 128, 2)
 SSE_TUNE(ELEM_TET, 4, 3, 128, 2, 2, 128, 4)
