@@ -60,6 +60,9 @@ bool struct_config(int elem, int P, StructConfig* c);
 int struct_occupancy(int elem, int P);
 hipError_t launch_struct(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
                          hipStream_t s, int n_blocks);
+size_t std_tensor_config(int elem, int P);
+hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, const double* u, double* dudt,
+                             const RKStage& rk, hipStream_t s, int which);
 }  // namespace sse
 
 using sse::DevParams;
@@ -72,6 +75,9 @@ struct sse_handle {
     sse::StructTables ST{};
     bool use_struct = false;
     int struct_blocks = 0;
+    bool use_std_tensor = false;  // StandardForm scalar-law kernels with compile-time shapes
+    int members = 1;              // problems sharing this discretisation and mesh (ensemble)
+    double* b_member = nullptr;   // diffusion coefficient of each member (device), or null
     int element = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
@@ -465,6 +471,66 @@ int build_tensor(sse_handle* h, const sse_desc* D) {
     return 0;
 }
 
+// Set up the compile-time-shaped StandardForm kernels of scalar linear laws
+// (ModalTensor Tri/Tet, weight-adjusted mass solve); returns 0 and leaves
+// use_std_tensor false when the discretisation is not one of the compiled shapes.
+int build_std_tensor(sse_handle* h, const sse_desc* D) {
+    const DevParams& P = h->P;
+    if (!P.standard || D->N_c != 1) return 0;
+    if (D->law != SSE_LINEAR_ADVECTION && D->law != SSE_LINEAR_ADVECTION_DIFFUSION) return 0;
+    if (!(D->element == SSE_TRI || D->element == SSE_TET) || !D->V || D->V_p <= 0 || !D->V_A || !D->V_B) return 0;
+    if (D->element == SSE_TET && !D->V_C) return 0;
+    if (D->mass_solver != SSE_MASS_WEIGHT_ADJUSTED || D->Minv || D->R_selection || !D->R) return 0;
+    const int p = D->V_p, n1 = p + 1;
+    for (int m = 0; m < D->d; ++m)
+        if (D->V_M[m] != n1) return 0;
+    const bool tet = D->element == SSE_TET;
+    const int nq = tet ? n1 * n1 * n1 : n1 * n1, nf = tet ? 4 * n1 * n1 : 3 * n1;
+    const int np = tet ? n1 * (p + 2) * (p + 3) / 6 : n1 * (p + 2) / 2;
+    if (D->N_q != nq || D->N_f != nf || D->N_p != np) return 0;
+    const size_t lds = sse::std_tensor_config(D->element, p);
+    if (lds == 0 || lds > 160 * 1024) return 0;
+    {  // the dense V must be the product of the factors
+        std::vector<double> Vw = warped_dense(D);
+        if ((int)Vw.size() != nq * np) return 0;
+        double scale = 0.0, worst = 0.0;
+        for (size_t t = 0; t < Vw.size(); ++t) {
+            scale = std::max(scale, std::fabs(D->V[t]));
+            worst = std::max(worst, std::fabs(D->V[t] - Vw[t]));
+        }
+        if (worst > 1e-12 * std::max(scale, 1.0)) return 0;
+    }
+    sse::TensorParams& T = h->TP;
+    T.base = P;
+    T.elem = D->element;
+    T.P = p;
+    T.variant = 0;
+    std::vector<std::vector<std::pair<int, double>>> rrows(nf), rtrows(nq);
+    for (int j = 0; j < nf; ++j)
+        for (int i = 0; i < nq; ++i) {
+            const double v = D->R[j + (size_t)nf * i];
+            if (v == 0.0) continue;
+            rrows[j].push_back({i, v});
+            rtrows[i].push_back({j, v});
+        }
+    std::vector<int> r_idx, rt_idx;
+    std::vector<double> r_val, rt_val;
+    ell(rrows, T.wr, r_idx, r_val);
+    ell(rtrows, T.wrt, rt_idx, rt_val);
+    std::vector<double> VA(D->V_A, D->V_A + (size_t)n1 * n1);
+    std::vector<double> VB(D->V_B, D->V_B + (size_t)n1 * n1 * n1);
+    std::vector<double> VC;
+    if (tet) VC.assign(D->V_C, D->V_C + (size_t)n1 * n1 * n1 * n1);
+    int rc;
+    if ((rc = upload(h, r_idx, &T.r_idx)) || (rc = upload(h, r_val, &T.r_val)) || (rc = upload(h, rt_idx, &T.rt_idx)) ||
+        (rc = upload(h, rt_val, &T.rt_val)) || (rc = upload(h, VA, &T.VA)) || (rc = upload(h, VB, &T.VB)))
+        return rc;
+    if (tet && (rc = upload(h, VC, &T.VC))) return rc;
+    T.xcd_remap = 1;
+    h->use_std_tensor = true;
+    return 0;
+}
+
 // Tables of the line-structured time_derivative kernel (requires the tensor
 // kernels to be set up). Returns 0 and leaves use_struct false when the
 // operators do not have the tensor-line structure.
@@ -577,7 +643,9 @@ int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int 
         return 0;
     }
     hipError_t e;
-    if (h->P.standard) {  // phase 1 without entropy projection, then the StandardForm kernels
+    if (h->use_std_tensor) {
+        e = sse::launch_std_tensor(h->TP, h->members, h->b_member, u, dudt, rk, h->stream, which);
+    } else if (h->P.standard) {  // phase 1 without entropy projection, then the StandardForm kernels
         e = (which & 1) ? sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, 1) : hipSuccess;
         if (e == hipSuccess) e = sse::launch_standard(h->P, dudt, rk, h->stream, which);
     } else {
@@ -610,7 +678,13 @@ int32_t sse_abi_version(void) { return SSE_ABI_VERSION; }
 
 const char* sse_last_error(const sse_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
-int sse_create(const sse_desc* D, sse_handle** out) {
+}  // extern "C"
+
+namespace {
+
+// A handle for `members` problems that share the descriptor's discretisation and
+// mesh (members > 1 needs the compile-time-shaped StandardForm kernels).
+int create_handle(const sse_desc* D, int members, sse_handle** out) {
     if (!out) return fail(nullptr, SSE_ERR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
     std::string msg;
@@ -723,15 +797,17 @@ int sse_create(const sse_desc* D, sse_handle** out) {
             (rc = upload(h, Jf, &P.Jf)) || (rc = upload(h, mapP, &P.mapP)))
             return cleanup_fail(rc);
     }
-    if ((rc = alloc(h, (size_t)N_q * D->N_c * N_e, &P.uq)) || (rc = alloc(h, (size_t)N_f * D->N_c * N_e, &P.uf)) ||
-        (rc = alloc(h, 1, &P.status)))
+    const size_t ens = (size_t)members;
+    h->members = members;
+    if ((rc = alloc(h, (size_t)N_q * D->N_c * N_e * ens, &P.uq)) ||
+        (rc = alloc(h, (size_t)N_f * D->N_c * N_e * ens, &P.uf)) || (rc = alloc(h, 1, &P.status)))
         return cleanup_fail(rc);
     if (hipMemset(P.status, 0, sizeof(int)) != hipSuccess) {
         h->err = "hipMemset failed";
         return cleanup_fail(SSE_ERR_HIP);
     }
     if (P.second_order &&
-        ((rc = alloc(h, (size_t)N_q * d * N_e, &P.qq)) || (rc = alloc(h, (size_t)N_f * d * N_e, &P.qf))))
+        ((rc = alloc(h, (size_t)N_q * d * N_e * ens, &P.qq)) || (rc = alloc(h, (size_t)N_f * d * N_e * ens, &P.qf))))
         return cleanup_fail(rc);
     if (sse::lds_time_derivative_generic(P) > 64 * 1024 || sse::lds_nodal_values_generic(P) > 64 * 1024 ||
         (P.standard && sse::lds_standard(P) > 64 * 1024)) {
@@ -739,14 +815,19 @@ int sse_create(const sse_desc* D, sse_handle** out) {
         return cleanup_fail(SSE_ERR_UNSUPPORTED);
     }
     if (!P.standard && (rc = build_tensor(h, D))) return cleanup_fail(rc);
+    if (P.standard && (rc = build_std_tensor(h, D))) return cleanup_fail(rc);
     const char* kenv = getenv("SSE_KERNELS");  // "generic", "tensor" (table-driven pairs) or unset
     if (!(kenv && (!strcmp(kenv, "generic") || !strcmp(kenv, "tensor"))) && (rc = build_struct(h, D)))
         return cleanup_fail(rc);
     if (const char* env = getenv("SSE_FORCE_GENERIC")) {
         if (env[0] == '1') h->use_tensor = false;
     }
-    if (kenv && !strcmp(kenv, "generic")) h->use_tensor = false;
+    if (kenv && !strcmp(kenv, "generic")) h->use_tensor = h->use_std_tensor = false;
     if (!h->use_tensor) h->use_struct = false;
+    if (members > 1 && !h->use_std_tensor) {
+        h->err = "an ensemble of problems needs the compiled StandardForm shapes of scalar laws";
+        return cleanup_fail(SSE_ERR_UNSUPPORTED);
+    }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) {
         h->err = "hipStreamCreate failed";
         return cleanup_fail(SSE_ERR_HIP);
@@ -755,6 +836,12 @@ int sse_create(const sse_desc* D, sse_handle** out) {
     *out = h;
     return 0;
 }
+
+}  // namespace
+
+extern "C" {
+
+int sse_create(const sse_desc* D, sse_handle** out) { return create_handle(D, 1, out); }
 
 int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     if (!h || !buf || len <= 0) return SSE_ERR_INVALID_ARGUMENT;
@@ -768,6 +855,10 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
         const sse::TensorParams& T = h->TP;
         snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",
                  T.elem, T.P, T.variant, T.n_pairs, T.n_vol_pairs, T.n_pass, T.pass_size, T.wv, T.wf, T.wrt, T.wr);
+    } else if (h->use_std_tensor) {
+        snprintf(buf, len, "standard tensor elem=%d P=%d N_e=%d members=%d mapping=%s%s wr=%d wrt=%d", h->TP.elem,
+                 h->TP.P, P.N_e, h->members, P.skew ? "skew" : "standard", P.second_order ? " br1" : "", h->TP.wr,
+                 h->TP.wrt);
     } else if (P.standard) {
         snprintf(buf, len, "standard d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d mapping=%s%s", P.d, P.N_p, P.N_q, P.N_f,
                  P.N_c, P.N_e, P.skew ? "skew" : "standard", P.second_order ? " br1" : "");

@@ -1,0 +1,557 @@
+// sse_std_tensor.hip -- StandardForm residual of scalar linear laws (advection,
+// advection-diffusion with BR1) on ModalTensor triangles and tetrahedra, with
+// compile-time element shapes.
+//
+// Same algebra as the general StandardForm kernels (the reference's
+// src/Solvers/standard_form_first_order.jl:16-94 and
+// src/Solvers/standard_form_second_order.jl:3-75 with the operators of
+// src/Solvers/operators.jl:83-160), arranged for throughput:
+//  * a workgroup advances E "slots" at once. A slot is one element of one
+//    problem; with an ensemble of R problems that share the mesh and differ
+//    only in the diffusion coefficient, consecutive slots are the members of
+//    one element, so they also share its geometric factors. Every operator
+//    entry loaded (V factors, rows of D_m, D_m', R, R') serves all E slots
+//    from registers;
+//  * V and V' are sum-factorised (warped-product factors), the mass solve is
+//    the weight-adjusted one with M^{-1} = I (src/Solvers/mass_matrix.jl:59-75);
+//  * the BR1 auxiliary variable of all d directions goes through one
+//    three-direction mass solve, and R q_q reuses the stored facet values;
+//  * with the skew-symmetric mapping, R x at the facets is never recomputed:
+//    R u_q = u_f and R f_n = a_n u_f - b q_f,n (R is linear), so the interface
+//    and surface terms combine pointwise at the facet nodes.
+//
+// Lanes: lane g = k * R + r is member r of element k. Intermediate arrays
+// (u_q, u_f, q_q, q_f) are indexed by lane; states and time steps by
+// r * N_e + k (each member a reference-layout (N_p, 1, N_e) array).
+#include <hip/hip_runtime.h>
+
+#include "sse_tensor.h"
+
+namespace sse {
+
+struct Ensemble {
+    int R;              // members per element (1: a single problem)
+    const double* b;    // diffusion coefficient of each member, or null for the law's b
+};
+
+// threads per workgroup and slots per workgroup of the three kernels
+template <int ELEM, int P>
+struct StdTune;
+#define SSE_STD_TUNE(E_, PP, ts, e1, e2, e3) \
+    template <>                              \
+    struct StdTune<E_, PP> {                 \
+        static constexpr int TS = ts, E1 = e1, E2 = e2, E3 = e3; \
+    };
+SSE_STD_TUNE(ELEM_TRI, 2, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TRI, 3, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TRI, 4, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TRI, 5, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TRI, 6, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TET, 2, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TET, 3, 64, 8, 4, 4)
+SSE_STD_TUNE(ELEM_TET, 4, 128, 8, 2, 4)
+SSE_STD_TUNE(ELEM_TET, 5, 256, 8, 2, 4)
+#undef SSE_STD_TUNE
+
+namespace {
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+__device__ __forceinline__ size_t state_row(long long g, int R, int N_e) {
+    const long long k = g / R, r = g - k * R;
+    return (size_t)r * N_e + k;
+}
+
+__device__ __forceinline__ double member_b(const DevParams& p, const Ensemble& en, long long g) {
+    return en.b ? en.b[g % en.R] : p.b;
+}
+
+// lane of the exterior state of facet node j of lane g
+__device__ __forceinline__ long long neighbour(const DevParams& p, int R, long long g, int NF, int j, int& jn) {
+    const long long k = g / R, r = g - k * R;
+    const int gl = p.mapP[j + (size_t)NF * k];
+    const int kn = gl / NF;
+    jn = gl - kn * NF;
+    return (long long)kn * R + r;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------- phase 1
+// u_q = V u, u_f = R u_q
+template <int ELEM, int P>
+struct StdNodalLayout {
+    using Sh = Shape<ELEM, P>;
+    static constexpr int E = StdTune<ELEM, P>::E1;
+    static constexpr int u = 0, q = u + E * Sh::NP, z = q + E * Sh::NQ, w = z + E * Sh::NZ;
+    static constexpr int size = w + E * cmax(Sh::NW, 1);
+};
+
+template <int ELEM, int P>
+__global__ void __launch_bounds__((StdTune<ELEM, P>::TS))
+k_std_nodal_tensor(TensorParams tp, Ensemble en, const double* __restrict__ u) {
+    using Sh = Shape<ELEM, P>;
+    using T = StdTune<ELEM, P>;
+    using L = StdNodalLayout<ELEM, P>;
+    constexpr int E = T::E1, TS = T::TS, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF;
+    using SF = SumFact<ELEM, P, TS, E>;
+    extern __shared__ double sm[];
+    const DevParams& p = tp.base;
+    const int tl = threadIdx.x;
+    const long long nl = (long long)p.N_e * en.R;
+    const long long g0 = (long long)element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * E;
+    double *su = sm + L::u, *sq = sm + L::q;
+    for (int t = tl; t < E * NP; t += TS) {
+        const int s = t / NP, j = t - s * NP;
+        const long long g = g0 + s;
+        su[t] = g < nl ? u[state_row(g, en.R, p.N_e) * NP + j] : 0.0;
+    }
+    __syncthreads();
+    SF::V(tp, tl, su, sq, sm + L::z, sm + L::w);
+    const int nvalid = (int)(nl - g0 < E ? nl - g0 : E);
+    for (int t = tl; t < nvalid * NQ; t += TS) p.uq[(size_t)g0 * NQ + t] = sq[t];
+    for (int j = tl; j < NF; j += TS) {
+        double acc[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) acc[s] = 0.0;
+        for (int w = 0; w < tp.wr; ++w) {
+            const int i = tp.r_idx[w * NF + j];
+            if (i < 0) break;
+            const double v = tp.r_val[w * NF + j];
+#pragma unroll
+            for (int s = 0; s < E; ++s) acc[s] += v * sq[s * NQ + i];
+        }
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            if (s < nvalid) p.uf[(size_t)(g0 + s) * NF + j] = acc[s];
+    }
+}
+
+// ----------------------------------------------------------------------------- BR1 auxiliary variable
+// q_m = M^{-1} V' [ -VT_m(u_q) + R' (B J_f u*_m) ],  u*_m = 1/2 (u^- + u^+) n_m,
+// VT_n(x) = h sum_m D_m' (W Lambda_mn x) - h W sum_m Lambda_mn (D_m x) + R' (1/2 B nJf_n R x)
+// (h = 1/2, skew-symmetric mapping) or sum_m D_m' (W Lambda_mn x) (standard mapping).
+// With the skew form the facet terms combine to R' (1/2 B nJf_n u^+).
+template <int ELEM, int P>
+struct StdAuxLayout {
+    using Sh = Shape<ELEM, P>;
+    static constexpr int E = StdTune<ELEM, P>::E2, D = Sh::D;
+    static constexpr int A = cmax(D * Sh::NQ, D * (Sh::NZ + Sh::NP));  // D u, then SumFact Z and y
+    static constexpr int B = cmax(D * Sh::NQ, D * Sh::NW);             // g, then SumFact W
+    static constexpr int uq = 0, up = uq + E * Sh::NQ, um = up + E * Sh::NF, a = um + E * Sh::NF;
+    static constexpr int b = a + E * A, r = b + E * B, tf = r + E * D * Sh::NQ;
+    static constexpr int size = tf + E * Sh::NF;
+};
+
+template <int ELEM, int P, bool SKEW>
+__global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(TensorParams tp, Ensemble en) {
+    using Sh = Shape<ELEM, P>;
+    using T = StdTune<ELEM, P>;
+    using L = StdAuxLayout<ELEM, P>;
+    constexpr int E = T::E2, TS = T::TS, D = Sh::D, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF;
+    using SF = SumFact<ELEM, P, TS, D * E>;
+    extern __shared__ double sm[];
+    const DevParams& p = tp.base;
+    const int tl = threadIdx.x;
+    const int R = en.R;
+    const long long nl = (long long)p.N_e * R;
+    const long long g0 = (long long)element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * E;
+    const int nvalid = (int)(nl - g0 < E ? nl - g0 : E);
+    double *suq = sm + L::uq, *sup = sm + L::up, *sum = sm + L::um, *sDu = sm + L::a, *sg = sm + L::b,
+           *sr = sm + L::r, *stf = sm + L::tf;
+    constexpr double h = SKEW ? 0.5 : 1.0;
+
+    for (int t = tl; t < E * NQ; t += TS) suq[t] = t < nvalid * NQ ? p.uq[(size_t)g0 * NQ + t] : 0.0;
+    for (int t = tl; t < E * NF; t += TS) {
+        const int s = t / NF, j = t - s * NF;
+        const long long g = g0 + s;
+        double up = 0.0, um = 0.0;
+        if (s < nvalid) {
+            int jn;
+            const long long gn = neighbour(p, R, g, NF, j, jn);
+            up = p.uf[(size_t)gn * NF + jn];
+            um = p.uf[(size_t)g * NF + j];
+        }
+        sup[t] = up;
+        sum[t] = um;
+    }
+    __syncthreads();
+    if constexpr (SKEW) {  // D_m u_q for all m
+        for (int i = tl; i < NQ; i += TS) {
+            double acc[E][D];
+#pragma unroll
+            for (int s = 0; s < E; ++s)
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc[s][m] = 0.0;
+            for (int t = p.D_ptr[i]; t < p.D_ptr[i + 1]; ++t) {
+                const int j = p.D_idx[t];
+                double dv[D];
+#pragma unroll
+                for (int m = 0; m < D; ++m) dv[m] = p.D_val[t * D + m];
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const double x = suq[s * NQ + j];
+#pragma unroll
+                    for (int m = 0; m < D; ++m) acc[s][m] += dv[m] * x;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < E; ++s)
+#pragma unroll
+                for (int m = 0; m < D; ++m) sDu[(s * D + m) * NQ + i] = acc[s][m];
+        }
+        __syncthreads();
+    }
+    for (int n = 0; n < D; ++n) {
+        // g_m = h W Lambda_mn u_q; r_n starts as h W sum_m Lambda_mn (D_m u_q)
+        for (int t = tl; t < E * NQ; t += TS) {
+            const int s = t / NQ, i = t - s * NQ;
+            const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+            const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+            const double hw = h * p.W[i];
+            const double x = suq[t];
+            double dterm = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; ++m) {
+                const double lam = Lk[i + NQ * (m + D * n)];
+                sg[(s * D + m) * NQ + i] = hw * lam * x;
+                if constexpr (SKEW) dterm += lam * sDu[(s * D + m) * NQ + i];
+            }
+            sr[(s * D + n) * NQ + i] = hw * dterm;
+        }
+        // facet: 1/2 B nJf_n u^+ (skew) or 1/2 B nJf_n (u^- + u^+)
+        for (int t = tl; t < E * NF; t += TS) {
+            const int s = t / NF, j = t - s * NF;
+            const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+            const double nj = p.nJf[(size_t)k * D * NF + n + D * j];
+            stf[t] = 0.5 * p.B[j] * nj * (SKEW ? sup[t] : sup[t] + sum[t]);
+        }
+        __syncthreads();
+        // r_n -= sum_m D_m' g_m;  r_n += R' tf
+        for (int i = tl; i < NQ; i += TS) {
+            double acc[E];
+#pragma unroll
+            for (int s = 0; s < E; ++s) acc[s] = 0.0;
+            for (int t = p.Dt_ptr[i]; t < p.Dt_ptr[i + 1]; ++t) {
+                const int j = p.Dt_idx[t];
+                double dv[D];
+#pragma unroll
+                for (int m = 0; m < D; ++m) dv[m] = p.Dt_val[t * D + m];
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int m = 0; m < D; ++m) a += dv[m] * sg[(s * D + m) * NQ + j];
+                    acc[s] -= a;
+                }
+            }
+            for (int w = 0; w < tp.wrt; ++w) {
+                const int j = tp.rt_idx[w * NQ + i];
+                if (j < 0) break;
+                const double v = tp.rt_val[w * NQ + i];
+#pragma unroll
+                for (int s = 0; s < E; ++s) acc[s] += v * stf[s * NF + j];
+            }
+#pragma unroll
+            for (int s = 0; s < E; ++s) sr[(s * D + n) * NQ + i] += acc[s];
+        }
+        __syncthreads();
+    }
+    // three-direction mass solve: q = V' (W/J o V (V' r)); then q_q = V q
+    double* sZ = sDu;
+    double* sy = sDu + E * D * Sh::NZ;
+    double* sW = sg;
+    SF::Vt(tp, tl, sr, sy, sZ, sW);
+    SF::V(tp, tl, sy, sr, sZ, sW);
+    for (int t = tl; t < E * D * NQ; t += TS) {
+        const int c = t / NQ, i = t - c * NQ, s = c / D;
+        const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+        sr[t] *= p.W[i] / p.Jq[(size_t)k * NQ + i];
+    }
+    __syncthreads();
+    SF::Vt(tp, tl, sr, sy, sZ, sW);
+    SF::V(tp, tl, sy, sr, sZ, sW);
+    for (int t = tl; t < nvalid * D * NQ; t += TS) p.qq[(size_t)g0 * D * NQ + t] = sr[t];
+    for (int j = tl; j < NF; j += TS) {  // q_f = R q_q
+        double acc[E * D];
+#pragma unroll
+        for (int c = 0; c < E * D; ++c) acc[c] = 0.0;
+        for (int w = 0; w < tp.wr; ++w) {
+            const int i = tp.r_idx[w * NF + j];
+            if (i < 0) break;
+            const double v = tp.r_val[w * NF + j];
+#pragma unroll
+            for (int c = 0; c < E * D; ++c) acc[c] += v * sr[c * NQ + i];
+        }
+#pragma unroll
+        for (int c = 0; c < E * D; ++c)
+            if (c < nvalid * D) p.qf[((size_t)g0 * D + c) * NF + j] = acc[c];
+    }
+}
+
+// ----------------------------------------------------------------------------- time derivative
+// dudt = M^{-1} V' [ sum_n VT_n(f_n) - R' (B J_f f*) ],  f_n = a_n u - b q_n,
+// f* = a.n {u} + halfλ |a.n| [u] - b {q}.n (LF; BR1 when second order).
+template <int ELEM, int P>
+struct StdTdLayout {
+    using Sh = Shape<ELEM, P>;
+    static constexpr int E = StdTune<ELEM, P>::E3, D = Sh::D;
+    static constexpr int F = 2 * D * Sh::NQ;  // f_n and G_m, then SumFact Z, W, y, tmp
+    static_assert(F >= Sh::NZ + cmax(Sh::NW, 1) + Sh::NP + Sh::NQ, "mass-solve scratch");
+    static constexpr int f = 0, r = f + E * F, t = r + E * Sh::NQ;
+    static constexpr int size = t + E * Sh::NF;
+};
+
+template <int ELEM, int P, bool SKEW, bool SECOND>
+__global__ void __launch_bounds__((StdTune<ELEM, P>::TS))
+k_std_td_tensor(TensorParams tp, Ensemble en, double* __restrict__ dudt, RKStage rk) {
+    using Sh = Shape<ELEM, P>;
+    using T = StdTune<ELEM, P>;
+    using L = StdTdLayout<ELEM, P>;
+    constexpr int E = T::E3, TS = T::TS, D = Sh::D, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF;
+    using SF = SumFact<ELEM, P, TS, E>;
+    extern __shared__ double sm[];
+    const DevParams& p = tp.base;
+    const int tl = threadIdx.x;
+    const int R = en.R;
+    const long long nl = (long long)p.N_e * R;
+    const long long g0 = (long long)element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * E;
+    const int nvalid = (int)(nl - g0 < E ? nl - g0 : E);
+    double *sf = sm + L::f, *sG = sf + E * D * NQ, *sr = sm + L::r, *st = sm + L::t;
+    constexpr double h = SKEW ? 0.5 : 1.0;
+
+    // volume fluxes f_n and G_m = h W sum_n Lambda_mn f_n
+    for (int t = tl; t < E * NQ; t += TS) {
+        const int s = t / NQ, i = t - s * NQ;
+        const long long g = g0 + (s < nvalid ? s : 0);
+        const long long k = g / R;
+        const double bm = SECOND ? member_b(p, en, g) : 0.0;
+        const double x = s < nvalid ? p.uq[(size_t)g * NQ + i] : 0.0;
+        double f[D];
+#pragma unroll
+        for (int n = 0; n < D; ++n) {
+            f[n] = p.law.a[n] * x;
+            if constexpr (SECOND) f[n] -= bm * (s < nvalid ? p.qq[((size_t)g * D + n) * NQ + i] : 0.0);
+            sf[(s * D + n) * NQ + i] = f[n];
+        }
+        const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+        const double hw = h * p.W[i];
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            double a = 0.0;
+#pragma unroll
+            for (int n = 0; n < D; ++n) a += Lk[i + NQ * (m + D * n)] * f[n];
+            sG[(s * D + m) * NQ + i] = hw * a;
+        }
+    }
+    // facet values: 1/2 B sum_n nJf_n (R f_n) - B J_f f*
+    for (int t = tl; t < E * NF; t += TS) {
+        const int s = t / NF, j = t - s * NF;
+        double val = 0.0;
+        if (s < nvalid) {
+            const long long g = g0 + s;
+            const long long k = g / R;
+            int jn;
+            const long long gn = neighbour(p, R, g, NF, j, jn);
+            const double um = p.uf[(size_t)g * NF + j], up = p.uf[(size_t)gn * NF + jn];
+            const double jf = p.Jf[(size_t)k * NF + j];
+            const double* nJ = p.nJf + (size_t)k * D * NF + D * j;
+            double an = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; ++m) an += p.law.a[m] * nJ[m];
+            an /= jf;  // a . n
+            double fs = 0.5 * an * (um + up);
+            if (p.law.inviscid_flux == 1) fs += p.law.half_lambda * fabs(an) * (um - up);
+            double surf = SKEW ? an * jf * um : 0.0;  // sum_n nJf_n a_n u^-
+            if constexpr (SECOND) {
+                const double bm = member_b(p, en, g);
+                double qn = 0.0, qm = 0.0;
+#pragma unroll
+                for (int m = 0; m < D; ++m) {
+                    const double qi = p.qf[((size_t)g * D + m) * NF + j];
+                    const double qo = p.qf[((size_t)gn * D + m) * NF + jn];
+                    qn += (qi + qo) * nJ[m];
+                    qm += qi * nJ[m];
+                }
+                fs -= bm * 0.5 * qn / jf;
+                if (SKEW) surf -= bm * qm;
+            }
+            val = 0.5 * p.B[j] * surf - p.B[j] * jf * fs;
+        }
+        st[t] = val;
+    }
+    __syncthreads();
+    // r = sum_m D_m' G_m - h W sum_{m,n} Lambda_mn (D_m f_n) + R' t
+    for (int i = tl; i < NQ; i += TS) {
+        double acc[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) acc[s] = 0.0;
+        for (int t = p.Dt_ptr[i]; t < p.Dt_ptr[i + 1]; ++t) {
+            const int j = p.Dt_idx[t];
+            double dv[D];
+#pragma unroll
+            for (int m = 0; m < D; ++m) dv[m] = p.Dt_val[t * D + m];
+#pragma unroll
+            for (int s = 0; s < E; ++s)
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc[s] += dv[m] * sG[(s * D + m) * NQ + j];
+        }
+        if constexpr (SKEW) {
+            for (int s0 = 0; s0 < E; s0 += 2) {  // two slots at a time: 2 D^2 accumulators
+                double dd[2][D][D];
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int m = 0; m < D; ++m)
+#pragma unroll
+                        for (int n = 0; n < D; ++n) dd[s][m][n] = 0.0;
+                for (int t = p.D_ptr[i]; t < p.D_ptr[i + 1]; ++t) {
+                    const int j = p.D_idx[t];
+                    double dv[D];
+#pragma unroll
+                    for (int m = 0; m < D; ++m) dv[m] = p.D_val[t * D + m];
+#pragma unroll
+                    for (int s = 0; s < 2; ++s)
+#pragma unroll
+                        for (int n = 0; n < D; ++n) {
+                            const double x = (s0 + s < E) ? sf[((s0 + s) * D + n) * NQ + j] : 0.0;
+#pragma unroll
+                            for (int m = 0; m < D; ++m) dd[s][m][n] += dv[m] * x;
+                        }
+                }
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    if (s0 + s >= E) break;
+                    const long long k = (g0 + (s0 + s < nvalid ? s0 + s : 0)) / R;
+                    const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+                    double a = 0.0;
+#pragma unroll
+                    for (int m = 0; m < D; ++m)
+#pragma unroll
+                        for (int n = 0; n < D; ++n) a += Lk[i + NQ * (m + D * n)] * dd[s][m][n];
+                    acc[s0 + s] -= h * p.W[i] * a;
+                }
+            }
+        }
+        for (int w = 0; w < tp.wrt; ++w) {
+            const int j = tp.rt_idx[w * NQ + i];
+            if (j < 0) break;
+            const double v = tp.rt_val[w * NQ + i];
+#pragma unroll
+            for (int s = 0; s < E; ++s) acc[s] += v * st[s * NF + j];
+        }
+#pragma unroll
+        for (int s = 0; s < E; ++s) sr[s * NQ + i] = acc[s];
+    }
+    __syncthreads();
+    // mass solve: o = V' (W/J o V (V' r))
+    double* sZ = sf;
+    double* sW = sZ + E * Sh::NZ;
+    double* sy = sW + E * cmax(Sh::NW, 1);
+    double* sx = sy + E * NP;
+    SF::Vt(tp, tl, sr, sy, sZ, sW);
+    SF::V(tp, tl, sy, sx, sZ, sW);
+    for (int t = tl; t < E * NQ; t += TS) {
+        const int s = t / NQ, i = t - s * NQ;
+        const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+        sx[t] *= p.W[i] / p.Jq[(size_t)k * NQ + i];
+    }
+    __syncthreads();
+    SF::Vt(tp, tl, sx, sy, sZ, sW);
+    for (int t = tl; t < nvalid * NP; t += TS) {
+        const int s = t / NP, j = t - s * NP;
+        const size_t o = state_row(g0 + s, R, p.N_e) * NP + j;
+        if (!rk.active) {
+            dudt[o] = sy[t];
+        } else {
+            const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[o]) + (rk.dt_e ? rk.dt_e[o / NP] : rk.dt) * sy[t];
+            rk.tmp[o] = kt;
+            rk.u[o] += rk.B * kt;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- launch
+template <int ELEM, int P>
+size_t std_tensor_lds(int which) {
+    if (which == 0) return sizeof(double) * StdNodalLayout<ELEM, P>::size;
+    if (which == 1) return sizeof(double) * StdAuxLayout<ELEM, P>::size;
+    return sizeof(double) * StdTdLayout<ELEM, P>::size;
+}
+
+template <int ELEM, int P, bool SKEW, bool SECOND>
+hipError_t launch_std_tensor_t(const TensorParams& tp, const Ensemble& en, const double* u, double* dudt,
+                               const RKStage& rk, hipStream_t s, int which) {
+    using T = StdTune<ELEM, P>;
+    static bool attr_set = false;
+    const size_t l0 = std_tensor_lds<ELEM, P>(0), l1 = std_tensor_lds<ELEM, P>(1), l2 = std_tensor_lds<ELEM, P>(2);
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_std_nodal_tensor<ELEM, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l0);
+        if (e == hipSuccess && SECOND)
+            e = hipFuncSetAttribute((const void*)k_std_aux_tensor<ELEM, P, SKEW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_std_td_tensor<ELEM, P, SKEW, SECOND>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const long long nl = (long long)tp.base.N_e * en.R;
+    if (which & 1) {
+        const int nb = (int)((nl + T::E1 - 1) / T::E1);
+        hipLaunchKernelGGL((k_std_nodal_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l0, s, tp, en, u);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (which & 2) {
+        if (SECOND) {
+            const int nb = (int)((nl + T::E2 - 1) / T::E2);
+            hipLaunchKernelGGL((k_std_aux_tensor<ELEM, P, SKEW>), dim3(nb), dim3(T::TS), l1, s, tp, en);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        const int nb = (int)((nl + T::E3 - 1) / T::E3);
+        hipLaunchKernelGGL((k_std_td_tensor<ELEM, P, SKEW, SECOND>), dim3(nb), dim3(T::TS), l2, s, tp, en, dudt, rk);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
+template <int ELEM, int P>
+hipError_t launch_std_tensor_p(const TensorParams& tp, const Ensemble& en, const double* u, double* dudt,
+                               const RKStage& rk, hipStream_t s, int which) {
+    const bool skew = tp.base.skew, second = tp.base.second_order;
+    if (skew && second) return launch_std_tensor_t<ELEM, P, true, true>(tp, en, u, dudt, rk, s, which);
+    if (skew) return launch_std_tensor_t<ELEM, P, true, false>(tp, en, u, dudt, rk, s, which);
+    if (second) return launch_std_tensor_t<ELEM, P, false, true>(tp, en, u, dudt, rk, s, which);
+    return launch_std_tensor_t<ELEM, P, false, false>(tp, en, u, dudt, rk, s, which);
+}
+
+#define SSE_STD_CASES                                                                               \
+    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5)          \
+    SSE_CASE(ELEM_TRI, 6) SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4)          \
+    SSE_CASE(ELEM_TET, 5)
+
+// largest LDS of the three kernels, or 0 if (elem, P) has no compiled kernels
+size_t std_tensor_config(int elem, int P) {
+#define SSE_CASE(E_, PP)                                                                              \
+    if (elem == E_ && P == PP)                                                                        \
+        return cmax(cmax((int)std_tensor_lds<E_, PP>(0), (int)std_tensor_lds<E_, PP>(1)),             \
+                    (int)std_tensor_lds<E_, PP>(2));
+    SSE_STD_CASES
+#undef SSE_CASE
+    return 0;
+}
+
+hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, const double* u, double* dudt,
+                             const RKStage& rk, hipStream_t s, int which) {
+    const Ensemble en{R, b_member};
+#define SSE_CASE(E_, PP) \
+    if (tp.elem == E_ && tp.P == PP) return launch_std_tensor_p<E_, PP>(tp, en, u, dudt, rk, s, which);
+    SSE_STD_CASES
+#undef SSE_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sse

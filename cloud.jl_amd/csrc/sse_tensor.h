@@ -81,14 +81,16 @@ __device__ __forceinline__ int element_of_block(int b, int nb, int remap) {
 }
 
 // ----------------------------------------------------------------------------- sum factorisation
-// All arrays are component-major LDS arrays: x[e * stride + point].
-template <int ELEM, int P, int TS>
+// All arrays are component-major LDS arrays: x[e * stride + point]. NCV
+// components (conserved variables, or several elements or directions side by
+// side) go through together, so each factor entry loaded serves all of them.
+template <int ELEM, int P, int TS, int NCV = Shape<ELEM, P>::NC>
 struct SumFact;
 
-template <int P, int TS>
-struct SumFact<ELEM_TRI, P, TS> {
+template <int P, int TS, int NCV>
+struct SumFact<ELEM_TRI, P, TS, NCV> {
     using Sh = Shape<ELEM_TRI, P>;
-    static constexpr int N1 = Sh::N1, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ;
+    static constexpr int N1 = Sh::N1, NC = NCV, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ;
 
     // y[a1,a2] = sum_b1 A[a1,b1] sum_{b2 <= P-b1} B[a2,b1,b2] x[q(b1,b2)]
     __device__ static void V(const TensorParams& tp, int tl, const double* x, double* y, double* Z, double*) {
@@ -161,10 +163,10 @@ struct SumFact<ELEM_TRI, P, TS> {
     }
 };
 
-template <int P, int TS>
-struct SumFact<ELEM_TET, P, TS> {
+template <int P, int TS, int NCV>
+struct SumFact<ELEM_TET, P, TS, NCV> {
     using Sh = Shape<ELEM_TET, P>;
-    static constexpr int N1 = Sh::N1, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ, NW = Sh::NW;
+    static constexpr int N1 = Sh::N1, NC = NCV, NP = Sh::NP, NQ = Sh::NQ, NZ = Sh::NZ, NW = Sh::NW;
     static constexpr int NN = N1 * N1;
 
     __device__ static void decode_q(int q, int& b1, int& b2) {
