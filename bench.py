@@ -230,6 +230,7 @@ def bench_advdiff(args):
         semi_discrete_residual(dudt, u, solver)
     solver.lib.sse_synchronize(solver.handle)
     el = time.perf_counter() - t0
+    sweep = advdiff_sweep(args, law, solver, u) if args.ad_members > 0 else None
     line = {
         "metric": "DOF-residuals/s (3D advection-diffusion BR1 p=5 FP64)", "value": dof * args.steps / el,
         "unit": "DOF-residuals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -239,7 +240,98 @@ def bench_advdiff(args):
                                "ModalTensor p=5, StandardForm skew-symmetric + LF, PhysicalOperator",
                    "N_e": N_e, "DOF": dof, "kernels": solver.describe()},
     }
+    if sweep:
+        line["sweep"] = sweep
     print(json.dumps(line), flush=True)
+
+
+def advdiff_sweep(args, law, solver, u):
+    """Batched sweep of diffusion coefficients b ~ U[1e-3, 1e-2] (seeded) on the
+    mesh of `solver`: all members advanced `ad_steps` CK2N54 steps as one ensemble,
+    against a loop of plain per-problem solves (timed on `ad_loop` members), then
+    the gradient of the final energy 1/2 u' M u with respect to each b by the
+    discrete tangent, checked against central differences in b on two members."""
+    import numpy as np
+    import torch
+
+    from sse.analysis import EnergyConservationAnalysis
+    from sse.batch import BatchedSolver
+    from sse.conservation_laws import LinearAdvectionDiffusionEquation
+    from sse.time_integration import rk_steps
+
+    R, n, dt = args.ad_members, args.ad_steps, args.ad_dt
+    bvals = np.random.default_rng(20261202).uniform(1e-3, 1e-2, R)
+    members = [solver.with_law(LinearAdvectionDiffusionEquation(law.a, float(b))) for b in bvals]
+    t0 = time.perf_counter()
+    batch = BatchedSolver(members)
+    log(f"sweep: {R} members, set-up {time.perf_counter() - t0:.1f} s; {batch.describe()[:200]}")
+    energy = EnergyConservationAnalysis(solver)  # the mass matrix does not depend on b
+    dof = u.numel()
+    # warm-up (kernel attributes, graph capture), then the timed sweep
+    states = [u.clone() for _ in members]
+    batch.rk_steps(states, [dt] * R, 2)
+    states = [u.clone() for _ in members]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.rk_steps(states, [dt] * R, n)
+    torch.cuda.synchronize()
+    t_sweep = time.perf_counter() - t0
+    J0 = float(energy.evaluate_conservation(u)[0])
+    JN = [float(energy.evaluate_conservation(x)[0]) for x in states]
+    log(f"sweep: {n} steps x {R} members in {t_sweep:.2f} s; energy {J0:.6e} -> [{min(JN):.6e}, {max(JN):.6e}]")
+    # loop of plain per-problem solves on a few members (one handle each)
+    L = min(args.ad_loop, R)
+    t_loop = 0.0
+    for i in range(L):
+        s_i = solver.with_law(LinearAdvectionDiffusionEquation(law.a, float(bvals[i])))
+        x = u.clone()
+        rk_steps(s_i, x, dt, 1)  # warm-up
+        x = u.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rk_steps(s_i, x, dt, n)
+        torch.cuda.synchronize()
+        t_loop += time.perf_counter() - t0
+        err = float((x - states[i]).abs().max() / states[i].abs().max())
+        if err > 1e-13:
+            raise SystemExit(f"ensemble member {i} differs from its own solve: {err:.3e}")
+        s_i.close()
+    loop_total = t_loop / L * R
+    log(f"loop: {L} members {t_loop:.2f} s -> {R} members {loop_total:.2f} s; speed-up {loop_total / t_sweep:.2f}x")
+    # gradient of the final energy with respect to b: tangent pairs
+    states = [u.clone() for _ in members]
+    tangents = [torch.zeros_like(u) for _ in members]
+    batch.rk_tangent(states, tangents, [dt] * R, 1)  # warm-up: allocates the pair buffers
+    states = [u.clone() for _ in members]
+    tangents = [torch.zeros_like(u) for _ in members]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.rk_tangent(states, tangents, [dt] * R, n)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    grads = [float(energy.evaluate_conservation_residual(x, v)[0]) for x, v in zip(states, tangents)]
+    # central differences in b on two members (J_N is a polynomial in b)
+    fd_errs = []
+    for i in range(min(2, R)):
+        Js = []
+        for sgn in (1.0, -1.0):
+            bb = float(bvals[i]) * (1.0 + sgn * 1e-4)
+            s_i = solver.with_law(LinearAdvectionDiffusionEquation(law.a, bb))
+            x = u.clone()
+            rk_steps(s_i, x, dt, n)
+            Js.append(float(energy.evaluate_conservation(x)[0]))
+            s_i.close()
+        fd = (Js[0] - Js[1]) / (2e-4 * float(bvals[i]))
+        fd_errs.append(abs(fd - grads[i]) / abs(grads[i]))
+    log(f"tangent: {t_grad:.2f} s ({t_grad / t_sweep:.2f}x the sweep); dJ/db in [{min(grads):.4e}, {max(grads):.4e}]; "
+        f"FD rel err {max(fd_errs):.2e}")
+    batch.close()
+    return {"members": R, "steps": n, "dt": dt, "b_range": [1e-3, 1e-2],
+            "sweep_s": t_sweep, "dof_steps_per_s": dof * R * n / t_sweep,
+            "loop_s_extrapolated": loop_total, "loop_members_timed": L, "speedup_vs_loop": loop_total / t_sweep,
+            "energy_0": J0, "energy_N_range": [min(JN), max(JN)],
+            "gradient_s": t_grad, "gradient_over_sweep": t_grad / t_sweep,
+            "dJ_db_range": [min(grads), max(grads)], "fd_rel_err": max(fd_errs)}
 
 
 def main():
@@ -257,6 +349,10 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--adj-mesh", type=int, default=12)
     ap.add_argument("--ad-mesh", type=int, default=24)
+    ap.add_argument("--ad-members", type=int, default=64, help="diffusivities in the sweep (0: none)")
+    ap.add_argument("--ad-steps", type=int, default=10)
+    ap.add_argument("--ad-dt", type=float, default=1e-4)
+    ap.add_argument("--ad-loop", type=int, default=4, help="members timed as plain per-problem solves")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-directions", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=200)

@@ -61,8 +61,8 @@ int struct_occupancy(int elem, int P);
 hipError_t launch_struct(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
                          hipStream_t s, int n_blocks);
 size_t std_tensor_config(int elem, int P);
-hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, const double* u, double* dudt,
-                             const RKStage& rk, hipStream_t s, int which);
+hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, int tangent, const double* u,
+                             double* dudt, const RKStage& rk, hipStream_t s, int which);
 }  // namespace sse
 
 using sse::DevParams;
@@ -76,8 +76,7 @@ struct sse_handle {
     bool use_struct = false;
     int struct_blocks = 0;
     bool use_std_tensor = false;  // StandardForm scalar-law kernels with compile-time shapes
-    int members = 1;              // problems sharing this discretisation and mesh (ensemble)
-    double* b_member = nullptr;   // diffusion coefficient of each member (device), or null
+    int members = 1;              // lanes per element the intermediates hold (ensemble capacity)
     int element = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
@@ -644,7 +643,7 @@ int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int 
     }
     hipError_t e;
     if (h->use_std_tensor) {
-        e = sse::launch_std_tensor(h->TP, h->members, h->b_member, u, dudt, rk, h->stream, which);
+        e = sse::launch_std_tensor(h->TP, 1, nullptr, 0, u, dudt, rk, h->stream, which);
     } else if (h->P.standard) {  // phase 1 without entropy projection, then the StandardForm kernels
         e = (which & 1) ? sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, 1) : hipSuccess;
         if (e == hipSuccess) e = sse::launch_standard(h->P, dudt, rk, h->stream, which);
@@ -652,6 +651,16 @@ int launch(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int 
         e = h->use_tensor ? sse::launch_tensor(h->TP, u, dudt, rk, h->stream, which)
                           : sse::launch_generic_dispatch(h->P, u, dudt, rk, h->stream, which);
     }
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// One launch over R members of an ensemble handle (R <= h->members); u, dudt and
+// the RK buffers hold the R members back to back.
+int launch_ens(sse_handle* h, const double* u, double* dudt, const RKStage& rk, int which, int R, const double* b,
+               int tangent) {
+    if (!h->use_std_tensor || R > h->members) return fail(h, SSE_ERR_UNSUPPORTED, "not an ensemble handle");
+    hipError_t e = sse::launch_std_tensor(h->TP, R, b, tangent, u, dudt, rk, h->stream, which);
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -1163,6 +1172,12 @@ struct sse_batch {
     std::vector<size_t> elems;                      // N_e of each problem
     std::vector<size_t> per_elem;                   // N_p * N_c of each problem
     std::vector<double*> state, tmp, dt_e;          // per group, device
+    // ensemble groups (members share one mesh and differ only in b): lanes per
+    // launch (0 for a group of concatenated meshes) and each member's b
+    std::vector<int> chunk;
+    std::vector<double*> b_dev;
+    // tangent pairs of ensemble groups, allocated on first use: (u, v) per member
+    std::vector<double*> t_state, t_tmp, t_dt, t_b;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1173,6 +1188,9 @@ namespace {
 
 thread_local std::string g_batch_create_error;
 
+// device memory the intermediates of one ensemble launch may take
+constexpr size_t ENSEMBLE_BUDGET = (size_t)16 << 30;
+
 int bfail(sse_batch* b, int code, const std::string& msg) {
     if (b) b->err = msg;
     else g_batch_create_error = msg;
@@ -1180,13 +1198,15 @@ int bfail(sse_batch* b, int code, const std::string& msg) {
 }
 
 bool same_array(const void* a, const void* b, size_t bytes) {
-    if (!a || !b) return a == b;
+    if (a == b) return true;
+    if (!a || !b) return false;
     return std::memcmp(a, b, bytes) == 0;
 }
 
 // true when problems a and b can share one handle (same reference element and
-// operators, law, form and fluxes; geometry may differ)
-bool same_discretisation(const sse_desc* a, const sse_desc* b) {
+// operators, law, form and fluxes; geometry may differ). With any_b the
+// diffusion coefficients may differ too.
+bool same_discretisation(const sse_desc* a, const sse_desc* b, bool any_b = false) {
     const int32_t* ia = &a->abi_version;
     const int32_t* ib = &b->abi_version;
     // integer header fields up to viscous_flux, except N_e
@@ -1195,7 +1215,7 @@ bool same_discretisation(const sse_desc* a, const sse_desc* b) {
         if ((const char*)(ia + t) == (const char*)&a->N_e) continue;
         if (ia[t] != ib[t]) return false;
     }
-    if (a->gamma != b->gamma || a->half_lambda != b->half_lambda || a->b != b->b) return false;
+    if (a->gamma != b->gamma || a->half_lambda != b->half_lambda || (!any_b && a->b != b->b)) return false;
     for (int m = 0; m < 3; ++m)
         if (a->a[m] != b->a[m]) return false;
     if (a->V_p != b->V_p) return false;
@@ -1215,10 +1235,52 @@ bool same_discretisation(const sse_desc* a, const sse_desc* b) {
            ((a->nJq == nullptr) == (b->nJq == nullptr));
 }
 
+// true when a and b have the same mesh and geometric factors
+bool same_mesh(const sse_desc* a, const sse_desc* b) {
+    if (a->N_e != b->N_e) return false;
+    const size_t Nq = a->N_q, Nf = a->N_f, d = a->d, Ne = a->N_e, dd = sizeof(double);
+    return same_array(a->J_q, b->J_q, Nq * Ne * dd) && same_array(a->Lambda_q, b->Lambda_q, Nq * d * d * Ne * dd) &&
+           same_array(a->nJf, b->nJf, d * Nf * Ne * dd) && same_array(a->J_f, b->J_f, Nf * Ne * dd) &&
+           same_array(a->mapP, b->mapP, Nf * Ne * sizeof(int64_t)) &&
+           same_array(a->nJq, b->nJq, d * a->N_fac * Nq * Ne * dd);
+}
+
 template <typename T>
 void append(std::vector<T>& dst, const T* src, size_t n) {
     dst.insert(dst.end(), src, src + n);
 }
+
+// lanes per element one launch of an ensemble of R members handles (even, so
+// that tangent pairs never straddle two launches)
+int ensemble_chunk(const sse_desc* D, int R) {
+    const size_t per_lane =
+        sizeof(double) * (size_t)D->N_e * ((size_t)D->N_q + D->N_f + (size_t)D->d * (D->N_q + D->N_f));
+    size_t cap = std::max<size_t>(2, ENSEMBLE_BUDGET / std::max<size_t>(per_lane, 1));
+    if (const char* env = getenv("SSE_ENSEMBLE_CHUNK")) cap = std::max(2, atoi(env));  // smaller launches, for tests
+    cap = std::min<size_t>(cap, 2 * (size_t)R);
+    cap &= ~(size_t)1;
+    return (int)std::max<size_t>(cap, 2);
+}
+
+// RK stage of members [c0, c0 + n) of ensemble group g (tangent: lanes of the pair buffers)
+int ensemble_stage(sse_batch* b, size_t g, int s, bool tangent) {
+    sse_handle* h = b->groups[g];
+    const size_t n1 = (size_t)h->P.N_p * h->P.N_c * h->P.N_e, ne = h->P.N_e;
+    const int R = (int)b->members[g].size() * (tangent ? 2 : 1);
+    double* st = tangent ? b->t_state[g] : b->state[g];
+    double* tm = tangent ? b->t_tmp[g] : b->tmp[g];
+    double* de = tangent ? b->t_dt[g] : b->dt_e[g];
+    const double* bb = tangent ? b->t_b[g] : b->b_dev[g];
+    for (int c0 = 0; c0 < R; c0 += b->chunk[g]) {
+        const int n = std::min(b->chunk[g], R - c0);
+        RKStage rk{1, CK_A[s], CK_B[s], 0.0, tm + c0 * n1, st + c0 * n1, de + c0 * ne};
+        int r = launch_ens(h, st + c0 * n1, nullptr, rk, 3, n, bb + c0, tangent ? 1 : 0);
+        if (r) return bfail(b, r, h->err);
+    }
+    return 0;
+}
+
+int batch_alloc(double** p, size_t n) { return hipMalloc(p, std::max<size_t>(n, 1) * sizeof(double)) == hipSuccess; }
 
 }  // namespace
 
@@ -1239,9 +1301,68 @@ int sse_batch_create(const sse_desc* const* descs, int32_t n, sse_batch** out) {
         sse_batch_destroy(b);
         return code;
     };
-    // group problems by discretisation, in order of first appearance
-    std::vector<int> rep;  // representative problem of each group
+    b->elems.resize(n);
+    b->per_elem.resize(n);
     for (int i = 0; i < n; ++i) {
+        b->elems[i] = descs[i]->N_e;
+        b->per_elem[i] = (size_t)descs[i]->N_p * descs[i]->N_c;
+    }
+    if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup(SSE_ERR_HIP, "hipStreamCreate failed");
+    // ensembles first: StandardForm scalar problems on one mesh that differ at most in b
+    std::vector<std::vector<int>> ens;
+    for (int i = 0; i < n; ++i) {
+        const sse_desc* D = descs[i];
+        if (D->form != SSE_STANDARD || D->N_c != 1) continue;
+        bool placed = false;
+        for (auto& e : ens)
+            if (same_discretisation(descs[e[0]], D, true) && same_mesh(descs[e[0]], D)) {
+                e.push_back(i);
+                placed = true;
+                break;
+            }
+        if (!placed) ens.push_back({i});
+    }
+    std::vector<char> grouped(n, 0);
+    for (auto& e : ens) {
+        if (e.size() < 2) continue;
+        const sse_desc* R = descs[e[0]];
+        const int R_n = (int)e.size();
+        const int chunk = ensemble_chunk(R, R_n);
+        sse_handle* h = nullptr;
+        int code = create_handle(R, chunk, &h);
+        if (code == SSE_ERR_UNSUPPORTED) continue;  // not a compiled shape: concatenate instead
+        if (code) return cleanup(code, "ensemble: " + g_create_error);
+        b->groups.push_back(h);
+        sse_set_stream(h, b->stream);
+        b->members.push_back(e);
+        b->offsets.emplace_back();
+        for (int t = 0; t < R_n; ++t) b->offsets.back().push_back((long long)t * R->N_e);
+        b->chunk.push_back(chunk);
+        const size_t n1 = (size_t)R->N_p * R->N_c * R->N_e;
+        double *st = nullptr, *tm = nullptr, *de = nullptr, *bd = nullptr;
+        const bool ok = batch_alloc(&st, n1 * R_n) && batch_alloc(&tm, n1 * R_n) &&
+                        batch_alloc(&de, (size_t)R->N_e * R_n) && batch_alloc(&bd, R_n);
+        b->state.push_back(st);
+        b->tmp.push_back(tm);
+        b->dt_e.push_back(de);
+        b->b_dev.push_back(bd);
+        b->t_state.push_back(nullptr);
+        b->t_tmp.push_back(nullptr);
+        b->t_dt.push_back(nullptr);
+        b->t_b.push_back(nullptr);
+        if (!ok) return cleanup(SSE_ERR_OUT_OF_MEMORY, "hipMalloc failed for the ensemble state");
+        std::vector<double> bv;
+        for (int i : e) bv.push_back(descs[i]->b);
+        if (hipMemcpy(bd, bv.data(), bv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(SSE_ERR_HIP, "hipMemcpy failed");
+        for (int i : e) grouped[i] = 1;
+    }
+    // the rest: group problems by discretisation, in order of first appearance,
+    // and concatenate their meshes
+    std::vector<int> rep;  // representative problem of each group
+    const size_t first = b->groups.size();
+    for (int i = 0; i < n; ++i) {
+        if (grouped[i]) continue;
         int g = -1;
         for (size_t t = 0; t < rep.size(); ++t)
             if (same_discretisation(descs[rep[t]], descs[i])) { g = (int)t; break; }
@@ -1251,26 +1372,20 @@ int sse_batch_create(const sse_desc* const* descs, int32_t n, sse_batch** out) {
             b->offsets.emplace_back();
             g = (int)rep.size() - 1;
         }
-        b->members[g].push_back(i);
+        b->members[first + g].push_back(i);
     }
-    b->elems.resize(n);
-    b->per_elem.resize(n);
-    for (int i = 0; i < n; ++i) {
-        b->elems[i] = descs[i]->N_e;
-        b->per_elem[i] = (size_t)descs[i]->N_p * descs[i]->N_c;
-    }
-    if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup(SSE_ERR_HIP, "hipStreamCreate failed");
     for (size_t g = 0; g < rep.size(); ++g) {
+        const size_t G = first + g;
         const sse_desc* R = descs[rep[g]];
         sse_desc C = *R;
         const size_t Nq = R->N_q, Nf = R->N_f, d = R->d, Nfac = R->N_fac;
         std::vector<double> Jq, Lam, nJf, Jf, nJq;
         std::vector<int64_t> mapP;
         long long off = 0;
-        for (int i : b->members[g]) {
+        for (int i : b->members[G]) {
             const sse_desc* D = descs[i];
             const size_t Ne = D->N_e;
-            b->offsets[g].push_back(off);
+            b->offsets[G].push_back(off);
             append(Jq, D->J_q, Nq * Ne);
             append(Lam, D->Lambda_q, Nq * d * d * Ne);
             append(nJf, D->nJf, d * Nf * Ne);
@@ -1289,21 +1404,22 @@ int sse_batch_create(const sse_desc* const* descs, int32_t n, sse_batch** out) {
         C.mapP = mapP.data();
         sse_handle* h = nullptr;
         int code = sse_create(&C, &h);
-        if (code) return cleanup(code, "group " + std::to_string(g) + ": " + g_create_error);
+        if (code) return cleanup(code, "group " + std::to_string(G) + ": " + g_create_error);
         b->groups.push_back(h);
         sse_set_stream(h, b->stream);
+        b->chunk.push_back(0);
         const size_t nstate = (size_t)C.N_p * C.N_c * C.N_e;
         double *st = nullptr, *tm = nullptr, *de = nullptr;
-        if (hipMalloc(&st, nstate * sizeof(double)) != hipSuccess ||
-            hipMalloc(&tm, nstate * sizeof(double)) != hipSuccess ||
-            hipMalloc(&de, (size_t)C.N_e * sizeof(double)) != hipSuccess) {
-            if (st) (void)hipFree(st);
-            if (tm) (void)hipFree(tm);
-            return cleanup(SSE_ERR_OUT_OF_MEMORY, "hipMalloc failed for the group state");
-        }
+        const bool ok = batch_alloc(&st, nstate) && batch_alloc(&tm, nstate) && batch_alloc(&de, (size_t)C.N_e);
         b->state.push_back(st);
         b->tmp.push_back(tm);
         b->dt_e.push_back(de);
+        b->b_dev.push_back(nullptr);
+        b->t_state.push_back(nullptr);
+        b->t_tmp.push_back(nullptr);
+        b->t_dt.push_back(nullptr);
+        b->t_b.push_back(nullptr);
+        if (!ok) return cleanup(SSE_ERR_OUT_OF_MEMORY, "hipMalloc failed for the group state");
     }
     *out = b;
     return 0;
@@ -1342,6 +1458,11 @@ int sse_batch_rk_steps(sse_batch* b, double* const* u, const double* dt, int32_t
     auto one_step = [&]() -> int {
         for (int s = 0; s < 5; ++s)
             for (size_t g = 0; g < b->groups.size(); ++g) {
+                if (b->chunk[g]) {
+                    int r = ensemble_stage(b, g, s, false);
+                    if (r) return r;
+                    continue;
+                }
                 RKStage rk{1, CK_A[s], CK_B[s], 0.0, b->tmp[g], b->state[g], b->dt_e[g]};
                 int r = launch(b->groups[g], b->state[g], nullptr, rk, 3);
                 if (r) return bfail(b, r, b->groups[g]->err);
@@ -1386,14 +1507,89 @@ int sse_batch_rk_steps(sse_batch* b, double* const* u, const double* dt, int32_t
     return 0;
 }
 
+int sse_batch_rk_tangent(sse_batch* b, double* const* u, double* const* v, const double* dt, int32_t nsteps,
+                         int32_t where) {
+    if (!b) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !v || !dt || nsteps < 0) return bfail(b, SSE_ERR_INVALID_ARGUMENT, "u, v or dt is NULL, or nsteps < 0");
+    if (where != SSE_DEVICE && where != SSE_HOST) return bfail(b, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        if (!b->chunk[g] || !b->groups[g]->P.second_order)
+            return bfail(b, SSE_ERR_UNSUPPORTED,
+                         "tangents with respect to b need every problem in an ensemble of advection-diffusion "
+                         "problems on one mesh");
+    }
+    const hipMemcpyKind in = where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind outk = where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return 0;
+        return bfail(b, SSE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    int rc;
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        const sse_handle* h = b->groups[g];
+        const size_t n1 = (size_t)h->P.N_p * h->P.N_c * h->P.N_e, ne = h->P.N_e;
+        const size_t R2 = 2 * b->members[g].size();
+        if (!b->t_state[g]) {
+            if (!batch_alloc(&b->t_state[g], n1 * R2) || !batch_alloc(&b->t_tmp[g], n1 * R2) ||
+                !batch_alloc(&b->t_dt[g], ne * R2) || !batch_alloc(&b->t_b[g], R2))
+                return bfail(b, SSE_ERR_OUT_OF_MEMORY, "hipMalloc failed for the tangent state");
+        }
+        std::vector<double> de, bv;
+        for (size_t t = 0; t < b->members[g].size(); ++t) {
+            const int i = b->members[g][t];
+            if (!u[i] || !v[i]) return bfail(b, SSE_ERR_INVALID_ARGUMENT, "u[i] or v[i] is NULL");
+            if ((rc = hip_ok(hipMemcpyAsync(b->t_state[g] + 2 * t * n1, u[i], n1 * sizeof(double), in, b->stream),
+                             "state gather")) ||
+                (rc = hip_ok(hipMemcpyAsync(b->t_state[g] + (2 * t + 1) * n1, v[i], n1 * sizeof(double), in,
+                                            b->stream), "tangent gather")))
+                return rc;
+            de.insert(de.end(), 2 * ne, dt[i]);
+        }
+        std::vector<double> bh(b->members[g].size());
+        if ((rc = hip_ok(hipMemcpy(bh.data(), b->b_dev[g], bh.size() * sizeof(double), hipMemcpyDeviceToHost),
+                         "b download")))
+            return rc;
+        for (double x : bh) {
+            bv.push_back(x);
+            bv.push_back(x);
+        }
+        if ((rc = hip_ok(hipMemcpyAsync(b->t_dt[g], de.data(), de.size() * sizeof(double), hipMemcpyHostToDevice,
+                                        b->stream), "dt upload")) ||
+            (rc = hip_ok(hipMemcpyAsync(b->t_b[g], bv.data(), bv.size() * sizeof(double), hipMemcpyHostToDevice,
+                                        b->stream), "b upload")) ||
+            (rc = hip_ok(hipStreamSynchronize(b->stream), "sync")))
+            return rc;
+    }
+    for (int step = 0; step < nsteps; ++step)
+        for (int s = 0; s < 5; ++s)
+            for (size_t g = 0; g < b->groups.size(); ++g)
+                if ((rc = ensemble_stage(b, g, s, true))) return rc;
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        const sse_handle* h = b->groups[g];
+        const size_t n1 = (size_t)h->P.N_p * h->P.N_c * h->P.N_e;
+        for (size_t t = 0; t < b->members[g].size(); ++t) {
+            const int i = b->members[g][t];
+            if ((rc = hip_ok(hipMemcpyAsync(u[i], b->t_state[g] + 2 * t * n1, n1 * sizeof(double), outk, b->stream),
+                             "state scatter")) ||
+                (rc = hip_ok(hipMemcpyAsync(v[i], b->t_state[g] + (2 * t + 1) * n1, n1 * sizeof(double), outk,
+                                            b->stream), "tangent scatter")))
+                return rc;
+        }
+    }
+    if ((rc = hip_ok(hipStreamSynchronize(b->stream), "sync"))) return rc;
+    for (sse_handle* h : b->groups)
+        if ((rc = check_status(h))) return bfail(b, rc, h->err);
+    return 0;
+}
+
 int sse_batch_describe(const sse_batch* b, char* buf, int32_t len) {
     if (!b || !buf || len <= 0) return SSE_ERR_INVALID_ARGUMENT;
     std::string out = "batch groups=" + std::to_string(b->groups.size());
     for (size_t g = 0; g < b->groups.size(); ++g) {
         char one[256];
         sse_describe(b->groups[g], one, sizeof(one));
-        out += " | " + std::to_string(b->members[g].size()) + " problems, N_e=" +
-               std::to_string(b->groups[g]->P.N_e) + ": " + one;
+        out += " | " + std::to_string(b->members[g].size()) + (b->chunk[g] ? " problems on one mesh" : " problems") +
+               ", N_e=" + std::to_string(b->groups[g]->P.N_e) + ": " + one;
     }
     snprintf(buf, len, "%s", out.c_str());
     return 0;
@@ -1410,6 +1606,9 @@ void sse_batch_destroy(sse_batch* b) {
     for (double* p : b->state) (void)hipFree(p);
     for (double* p : b->tmp) (void)hipFree(p);
     for (double* p : b->dt_e) (void)hipFree(p);
+    for (auto* v : {&b->b_dev, &b->t_state, &b->t_tmp, &b->t_dt, &b->t_b})
+        for (double* p : *v)
+            if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -23,6 +23,13 @@
 // Lanes: lane g = k * R + r is member r of element k. Intermediate arrays
 // (u_q, u_f, q_q, q_f) are indexed by lane; states and time steps by
 // r * N_e + k (each member a reference-layout (N_p, 1, N_e) array).
+//
+// Tangent pairs: the residual is linear in u and in b (viscous flux -b q,
+// BR1 flux -b {q}.n, q linear in u), so v = du/db obeys the semi-discrete system
+// v' = R(v; b) + dR/db(u) with dR/db(u) the viscous terms of u at unit b. A
+// member pair (u, v) advanced together by the same low-storage RK stages gives
+// the exact derivative of the discrete solution; only the time derivative
+// kernel knows the pairing (it adds -q(u) to the tangent's viscous flux).
 #include <hip/hip_runtime.h>
 
 #include "sse_tensor.h"
@@ -32,6 +39,8 @@ namespace sse {
 struct Ensemble {
     int R;              // members per element (1: a single problem)
     const double* b;    // diffusion coefficient of each member, or null for the law's b
+    int tangent;        // members come in pairs (u, v = du/db): the odd member's residual
+                        // is R(v; b) + dR/db(u), dR/db(u) = -(viscous terms of u at b = 1)
 };
 
 // threads per workgroup and slots per workgroup of the three kernels
@@ -326,12 +335,16 @@ k_std_td_tensor(TensorParams tp, Ensemble en, double* __restrict__ dudt, RKStage
         const long long g = g0 + (s < nvalid ? s : 0);
         const long long k = g / R;
         const double bm = SECOND ? member_b(p, en, g) : 0.0;
+        const bool tan = SECOND && en.tangent && ((g % R) & 1) && s < nvalid;
         const double x = s < nvalid ? p.uq[(size_t)g * NQ + i] : 0.0;
         double f[D];
 #pragma unroll
         for (int n = 0; n < D; ++n) {
             f[n] = p.law.a[n] * x;
-            if constexpr (SECOND) f[n] -= bm * (s < nvalid ? p.qq[((size_t)g * D + n) * NQ + i] : 0.0);
+            if constexpr (SECOND) {
+                f[n] -= bm * (s < nvalid ? p.qq[((size_t)g * D + n) * NQ + i] : 0.0);
+                if (tan) f[n] -= p.qq[((size_t)(g - 1) * D + n) * NQ + i];  // dR/db(u) of the pair's u
+            }
             sf[(s * D + n) * NQ + i] = f[n];
         }
         const double* Lk = p.Lam + (size_t)k * NQ * D * D;
@@ -373,8 +386,19 @@ k_std_td_tensor(TensorParams tp, Ensemble en, double* __restrict__ dudt, RKStage
                     qn += (qi + qo) * nJ[m];
                     qm += qi * nJ[m];
                 }
-                fs -= bm * 0.5 * qn / jf;
-                if (SKEW) surf -= bm * qm;
+                qn *= bm;
+                qm *= bm;
+                if (en.tangent && ((g % R) & 1)) {  // + the pair's u at unit b
+#pragma unroll
+                    for (int m = 0; m < D; ++m) {
+                        const double qi = p.qf[((size_t)(g - 1) * D + m) * NF + j];
+                        const double qo = p.qf[((size_t)(gn - 1) * D + m) * NF + jn];
+                        qn += (qi + qo) * nJ[m];
+                        qm += qi * nJ[m];
+                    }
+                }
+                fs -= 0.5 * qn / jf;
+                if (SKEW) surf -= qm;
             }
             val = 0.5 * p.B[j] * surf - p.B[j] * jf * fs;
         }
@@ -544,9 +568,10 @@ size_t std_tensor_config(int elem, int P) {
     return 0;
 }
 
-hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, const double* u, double* dudt,
-                             const RKStage& rk, hipStream_t s, int which) {
-    const Ensemble en{R, b_member};
+hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, int tangent, const double* u,
+                             double* dudt, const RKStage& rk, hipStream_t s, int which) {
+    if (tangent && ((R & 1) || !tp.base.second_order)) return hipErrorInvalidValue;
+    const Ensemble en{R, b_member, tangent};
 #define SSE_CASE(E_, PP) \
     if (tp.elem == E_ && tp.P == PP) return launch_std_tensor_p<E_, PP>(tp, en, u, dudt, rk, s, which);
     SSE_STD_CASES

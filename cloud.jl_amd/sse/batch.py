@@ -4,7 +4,9 @@ The reference solves one problem per ``Solver``; a study over many problems
 loops over ``solve`` calls. ``BatchedSolver`` instead groups problems that share
 a discretisation (element type, degree, operators, law, form) into one set of
 kernel launches per phase, and replays the whole CarpenterKennedy2N54 step of
-all groups as one HIP graph.
+all groups as one HIP graph. Scalar StandardForm problems on one mesh that
+differ only in the diffusion coefficient (a parameter sweep; build them with
+``Solver.with_law``) form an ensemble that shares one copy of the geometry.
 """
 
 from __future__ import annotations
@@ -50,18 +52,7 @@ class BatchedSolver:
         n = len(self.solvers)
         if len(states) != n or len(dts) != n:
             raise ValueError("one state and one dt per problem")
-        host = isinstance(states[0], np.ndarray)
-        ptrs = (C.c_void_p * n)()
-        for i, (x, s) in enumerate(zip(states, self.solvers)):
-            N_p, N_c, N_e = s.size()
-            if host:
-                if not (x.flags.f_contiguous and x.shape == (N_p, N_c, N_e) and x.dtype == np.float64):
-                    raise ValueError(f"state {i} must be a Fortran float64 array {(N_p, N_c, N_e)}")
-                ptrs[i] = x.ctypes.data
-            else:
-                if not (x.is_cuda and x.is_contiguous() and tuple(x.shape) == (N_e, N_c, N_p)):
-                    raise ValueError(f"state {i} must be a contiguous CUDA tensor {(N_e, N_c, N_p)}")
-                ptrs[i] = x.data_ptr()
+        ptrs, host = self._pointers(states, "state")
         dt = np.ascontiguousarray(dts, dtype=np.float64)
         code = self.lib.sse_batch_rk_steps(self.handle, ptrs, dt.ctypes.data_as(C.POINTER(C.c_double)),
                                            int(nsteps), abi.SSE_HOST if host else abi.SSE_DEVICE,
@@ -69,6 +60,41 @@ class BatchedSolver:
         if code:
             raise BatchError(code, self.lib.sse_batch_last_error(self.handle).decode())
         return states
+
+    def _pointers(self, states, what):
+        n = len(self.solvers)
+        host = isinstance(states[0], np.ndarray)
+        ptrs = (C.c_void_p * n)()
+        for i, (x, s) in enumerate(zip(states, self.solvers)):
+            N_p, N_c, N_e = s.size()
+            if host:
+                if not (x.flags.f_contiguous and x.shape == (N_p, N_c, N_e) and x.dtype == np.float64):
+                    raise ValueError(f"{what} {i} must be a Fortran float64 array {(N_p, N_c, N_e)}")
+                ptrs[i] = x.ctypes.data
+            else:
+                if not (x.is_cuda and x.is_contiguous() and tuple(x.shape) == (N_e, N_c, N_p)):
+                    raise ValueError(f"{what} {i} must be a contiguous CUDA tensor {(N_e, N_c, N_p)}")
+                ptrs[i] = x.data_ptr()
+        return ptrs, host
+
+    def rk_tangent(self, states, tangents, dts, nsteps: int):
+        """Advances each state u_i and its derivative v_i = du_i/db_i with respect
+        to the problem's diffusion coefficient together (``sse_batch_rk_tangent``),
+        in place. Every problem must belong to an ensemble (advection-diffusion
+        problems sharing one mesh, see ``Solver.with_law``)."""
+        n = len(self.solvers)
+        if len(states) != n or len(tangents) != n or len(dts) != n:
+            raise ValueError("one state, one tangent and one dt per problem")
+        pu, host = self._pointers(states, "state")
+        pv, host_v = self._pointers(tangents, "tangent")
+        if host != host_v:
+            raise ValueError("states and tangents must both be host arrays or both CUDA tensors")
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        code = self.lib.sse_batch_rk_tangent(self.handle, pu, pv, dt.ctypes.data_as(C.POINTER(C.c_double)),
+                                             int(nsteps), abi.SSE_HOST if host else abi.SSE_DEVICE)
+        if code:
+            raise BatchError(code, self.lib.sse_batch_last_error(self.handle).decode())
+        return states, tangents
 
     def close(self):
         if self.handle is not None and self.handle.value:

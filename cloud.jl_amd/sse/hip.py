@@ -58,6 +58,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_batch_create.restype = C.c_int
     lib.sse_batch_rk_steps.argtypes = [vp, C.POINTER(vp), dp, C.c_int32, C.c_int32, C.c_int32]
     lib.sse_batch_rk_steps.restype = C.c_int
+    lib.sse_batch_rk_tangent.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), dp, C.c_int32, C.c_int32]
+    lib.sse_batch_rk_tangent.restype = C.c_int
     lib.sse_batch_describe.argtypes = [vp, C.c_char_p, C.c_int32]
     lib.sse_batch_describe.restype = C.c_int
     lib.sse_batch_last_error.argtypes = [vp]

@@ -14,6 +14,7 @@ runs in libsse_hip.so on the GPU, reached through its C-ABI
 
 from __future__ import annotations
 
+import copy
 import ctypes as C
 from dataclasses import dataclass, field
 from typing import Any, Optional, Tuple
@@ -212,6 +213,24 @@ class DescArrays:
     def shape(self) -> Tuple[int, int, int]:
         return (self.desc.N_p, self.desc.N_c, self.desc.N_e)
 
+    def with_law(self, law) -> "DescArrays":
+        """The same operators and geometry (shared, not copied) under another law of
+        the same kind, e.g. another diffusion coefficient."""
+        if type(law) is not type(self.law) or law.N_c != self.law.N_c:
+            raise ValueError("with_law keeps the kind of conservation law")
+        other = copy.copy(self)
+        other.law = law
+        desc = abi.SseDesc.from_buffer_copy(self.desc)
+        if isinstance(law, EulerEquations):
+            desc.gamma = law.gamma
+        if isinstance(law, LinearAdvectionDiffusionEquation):
+            desc.b = law.b
+        if hasattr(law, "a"):
+            for m, am in enumerate(law.a):
+                desc.a[m] = am
+        other.desc = desc
+        return other
+
 
 # ----------------------------------------------------------------------------- Solver
 class Solver:
@@ -232,6 +251,19 @@ class Solver:
                                  self.strategy)
         self._handle = None
         self._lib = None
+
+    def with_law(self, conservation_law) -> "Solver":
+        """A solver for the same discretisation and mesh under another law of the
+        same kind (a parameter study: e.g. another diffusion coefficient). The host
+        arrays are shared; a BatchedSolver over such solvers keeps one device copy
+        of the geometry."""
+        other = object.__new__(Solver)
+        other.__dict__.update(self.__dict__)
+        other.conservation_law = conservation_law
+        other.arrays = self.arrays.with_law(conservation_law)
+        other._handle = None
+        other._lib = None
+        return other
 
     @property
     def connectivity(self) -> np.ndarray:

@@ -232,6 +232,13 @@ int32_t sse_abi_version(void);
  * form and fluxes are concatenated into one group (one kernel launch per phase
  * for all of them); a batch holds one group per distinct discretisation.
  * Geometry and mapP are per problem, exactly as for sse_create.
+ *
+ * Ensembles: StandardForm problems of a scalar law (linear advection or
+ * advection-diffusion, ModalTensor Tri/Tet) with the same discretisation and
+ * the same mesh (equal geometric factors and mapP, compared by content) that
+ * differ at most in the diffusion coefficient b -- a parameter sweep -- form one
+ * group that keeps one copy of the geometry; each launch advances up to a
+ * bounded number of members side by side (16 GiB of intermediates at most).
  */
 typedef struct sse_batch sse_batch;
 
@@ -246,6 +253,22 @@ int sse_batch_create(const sse_desc* const* descs, int32_t n, sse_batch** out);
  * replayed; otherwise the kernels are launched step by step. Blocks until done. */
 int sse_batch_rk_steps(sse_batch* b, double* const* u, const double* dt, int32_t nsteps, int32_t where,
                        int32_t use_graph);
+
+/* Discrete tangent with respect to the diffusion coefficient, for batches
+ * whose groups are all ensembles of advection-diffusion problems: advances each
+ * problem's state u[i] and v[i] = d u[i] / d b_i together by nsteps
+ * CarpenterKennedy2N54 steps of dt[i] (v[i] usually starts at zero). The
+ * residual is linear in u and in b, so the pair obeys u' = R(u; b),
+ * v' = R(v; b) + dR/db(u) with dR/db(u) the viscous terms of u at unit b; the
+ * same RK stages applied to both give the exact derivative of the discrete
+ * solution, and the gradient of any functional J(u_N) with respect to b_i is
+ * <dJ/du, v_N> (for the energy 1/2 u' M u it is the energy rate of u along v,
+ * sse_functional with SSE_ENERGY and dudt = v). For one parameter this forward sweep is
+ * the cheapest exact gradient: two residuals per stage, no checkpoints. No
+ * counterpart in the reference (ForwardDiff through solve would compute the
+ * same derivative). Blocks until done. */
+int sse_batch_rk_tangent(sse_batch* b, double* const* u, double* const* v, const double* dt, int32_t nsteps,
+                         int32_t where);
 
 /* One-line description of the groups (count, elements, kernel family). */
 int sse_batch_describe(const sse_batch* b, char* buf, int32_t len);

@@ -40,9 +40,9 @@ def lib() -> C.CDLL:
         L.sse_oracle_set_threads.argtypes = [C.c_int]
         L.sse_oracle_max_threads.restype = C.c_int
         L.sse_oracle_ck2n54.argtypes = [dp, dp, dp]
-        L.sse_oracle_residual_jvp.argtypes = [vp, vp, vp, C.c_double, vp, vp]
+        L.sse_oracle_residual_jvp.argtypes = [vp, vp, vp, C.c_double, C.c_double, vp, vp]
         L.sse_oracle_residual_jvp.restype = C.c_int
-        L.sse_oracle_rk_jvp.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int]
+        L.sse_oracle_rk_jvp.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_int]
         L.sse_oracle_rk_jvp.restype = C.c_int
         L.sse_oracle_two_point.argtypes = [C.c_int, C.c_int, C.c_double, vp, C.c_int, vp, vp, vp]
         L.sse_oracle_c2e.argtypes = [C.c_int, C.c_double, vp, vp]
@@ -85,22 +85,24 @@ def rk_steps(arrays, u: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
     return u
 
 
-def residual_jvp(arrays, u: np.ndarray, du: np.ndarray, dlambda: float = 0.0):
-    """(R(u), R'(u) du + dR/dλ dλ) by forward-mode dual numbers through the oracle."""
+def residual_jvp(arrays, u: np.ndarray, du: np.ndarray, dlambda: float = 0.0, db: float = 0.0):
+    """(R(u), R'(u) du + dR/dλ dλ + dR/db db) by forward-mode dual numbers through
+    the oracle (λ the Lax-Friedrichs coefficient, b the diffusion coefficient)."""
     u = np.asfortranarray(u, dtype=np.float64)
     du = np.asfortranarray(du, dtype=np.float64)
     r, dr = np.zeros_like(u, order="F"), np.zeros_like(u, order="F")
     _check(lib().sse_oracle_residual_jvp(C.addressof(arrays.desc), u.ctypes.data, du.ctypes.data,
-                                         float(dlambda), r.ctypes.data, dr.ctypes.data))
+                                         float(dlambda), float(db), r.ctypes.data, dr.ctypes.data))
     return r, dr
 
 
-def rk_jvp(arrays, u: np.ndarray, du: np.ndarray, dt: float, nsteps: int, dlambda: float = 0.0):
+def rk_jvp(arrays, u: np.ndarray, du: np.ndarray, dt: float, nsteps: int, dlambda: float = 0.0,
+           db: float = 0.0):
     """(u_N, du_N): nsteps CK2N54 steps with the tangent carried along."""
     u = np.array(u, dtype=np.float64, order="F", copy=True)
     du = np.array(du, dtype=np.float64, order="F", copy=True)
     _check(lib().sse_oracle_rk_jvp(C.addressof(arrays.desc), u.ctypes.data, du.ctypes.data, float(dlambda),
-                                   float(dt), int(nsteps)))
+                                   float(db), float(dt), int(nsteps)))
     return u, du
 
 

@@ -259,6 +259,7 @@ struct Oracle {
     vector<int64_t> mapP0;    // 0-based
     bool second_order = false;
     T hl = 0.0;               // LaxFriedrichsNumericalFlux.halfλ (a T so that it can carry a tangent)
+    T bd = 0.0;               // diffusion coefficient b (a T for the same reason)
     vector<T> q_q;            // N_q x N_c x d x N_e (auxiliary variable, SecondOrder)
     vector<T> q_f;            // N_f x N_e x N_c x d
 
@@ -268,6 +269,7 @@ struct Oracle {
         if (N_fac <= 0 || N_f % N_fac != 0) return "N_f must be a multiple of N_fac";
         npf = N_f / N_fac;
         hl = D.half_lambda;
+        bd = D.b;
         law.kind = D.law; law.d = d; law.N_c = N_c;
         law.gamma = D.gamma; law.gm1 = D.gamma - 1.0; law.inv_gm1 = 1.0 / (D.gamma - 1.0);
         for (int m = 0; m < 3; ++m) law.a[m] = D.a[m];
@@ -668,7 +670,7 @@ struct Oracle {
                     for (int m = 0; m < d; ++m) {
                         T qin = q_f[j + (size_t)N_f * (k + (size_t)N_e * (e + (size_t)N_c * m))];
                         T qout = q_f[jn + (size_t)N_f * (kn + (size_t)N_e * (e + (size_t)N_c * m))];
-                        s += D.b * (-0.5 * (qin + qout)) * n_f[m + d * (j + (size_t)N_f * k)];
+                        s += bd * (-0.5 * (qin + qout)) * n_f[m + d * (j + (size_t)N_f * k)];
                     }
                     f_f[j + (size_t)N_f * e] += s;
                 }
@@ -685,7 +687,7 @@ struct Oracle {
                     for (int e = 0; e < N_c; ++e) fq[i + (size_t)N_q * e] = F[e + N_c * n];
                 } else {
                     T f = D.a[n] * uqk[i];
-                    if (second_order) f -= D.b * q_q[i + (size_t)N_q * N_c * (n + (size_t)d * k)];
+                    if (second_order) f -= bd * q_q[i + (size_t)N_q * N_c * (n + (size_t)d * k)];
                     fq[i] = f;
                 }
             }
@@ -816,15 +818,17 @@ int sse_oracle_rk_steps(const sse_desc* desc, double* u, double dt, int nsteps) 
     return 0;
 }
 
-// Directional derivative of the residual along (du, dlambda), lambda the
-// LaxFriedrichsNumericalFlux coefficient (halfλ = λ / 2): dudt = R(u), ddudt = R'(u) du +
-// dR/dλ dlambda. Forward-mode dual numbers through the same restatement.
-int sse_oracle_residual_jvp(const sse_desc* desc, const double* u, const double* du, double dlambda, double* dudt,
-                            double* ddudt) {
+// Directional derivative of the residual along (du, dlambda, db), lambda the
+// LaxFriedrichsNumericalFlux coefficient (halfλ = λ / 2) and b the diffusion
+// coefficient: dudt = R(u), ddudt = R'(u) du + dR/dλ dlambda + dR/db db.
+// Forward-mode dual numbers through the same restatement.
+int sse_oracle_residual_jvp(const sse_desc* desc, const double* u, const double* du, double dlambda, double db,
+                            double* dudt, double* ddudt) {
     Oracle<Dual> o;
     g_err = o.init(desc);
     if (!g_err.empty()) return SSE_ERR_UNSUPPORTED;
     o.hl = Dual(desc->half_lambda, 0.5 * dlambda);
+    o.bd = Dual(desc->b, db);
     const size_t n = (size_t)desc->N_p * desc->N_c * desc->N_e;
     vector<Dual> ud(n), rd(n);
     for (size_t i = 0; i < n; ++i) ud[i] = Dual(u[i], du[i]);
@@ -833,12 +837,14 @@ int sse_oracle_residual_jvp(const sse_desc* desc, const double* u, const double*
     return 0;
 }
 
-// nsteps CK2N54 steps of (u, du) together: du becomes d u_N along (du_0, dlambda).
-int sse_oracle_rk_jvp(const sse_desc* desc, double* u, double* du, double dlambda, double dt, int nsteps) {
+// nsteps CK2N54 steps of (u, du) together: du becomes d u_N along (du_0, dlambda, db).
+int sse_oracle_rk_jvp(const sse_desc* desc, double* u, double* du, double dlambda, double db, double dt,
+                      int nsteps) {
     Oracle<Dual> o;
     g_err = o.init(desc);
     if (!g_err.empty()) return SSE_ERR_UNSUPPORTED;
     o.hl = Dual(desc->half_lambda, 0.5 * dlambda);
+    o.bd = Dual(desc->b, db);
     const size_t n = (size_t)desc->N_p * desc->N_c * desc->N_e;
     vector<Dual> ud(n), k(n), tmp(n, Dual(0.0));
     for (size_t i = 0; i < n; ++i) ud[i] = Dual(u[i], du[i]);

@@ -261,6 +261,61 @@ def test_standard_form_br1_rk_steps_match_oracle():
     assert _rel(got, ref) <= RK_TOL, _rel(got, ref)
 
 
+def _advdiff_ensemble(p, bs, M=2):
+    from sse.conservation_laws import LinearAdvectionDiffusionEquation
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=M, p=p)
+    solvers = [ode.p.with_law(LinearAdvectionDiffusionEquation(law.a, b)) for b in bs]
+    u0s = [np.asfortranarray(ode.u0 * (1.0 + 0.25 * i)) for i in range(len(bs))]
+    return solvers, u0s
+
+
+@pytest.mark.parametrize("p,chunk", [(3, None), (5, None), (4, "2")])
+def test_diffusivity_ensemble_matches_oracle(p, chunk, monkeypatch):
+    # a sweep of diffusion coefficients on one mesh runs as one ensemble group
+    # (one geometry copy, members side by side); chunk 2 splits it over launches
+    from sse.batch import BatchedSolver
+    from sse.time_integration import to_device, to_host
+    if chunk:
+        monkeypatch.setenv("SSE_ENSEMBLE_CHUNK", chunk)
+    bs = (2e-3, 5e-3, 1e-2, 4e-3, 7e-3)
+    solvers, u0s = _advdiff_ensemble(p, bs)
+    batch = BatchedSolver(solvers)
+    assert batch.describe().count("one mesh") == 1, batch.describe()
+    dt, n = 1e-3, 10
+    states = [to_device(u) for u in u0s]
+    batch.rk_steps(states, [dt] * len(bs), n)
+    for s, u0, x in zip(solvers, u0s, states):
+        ref = O.rk_steps(s.arrays, u0, dt, n)
+        assert _rel(to_host(x), ref) <= RK_TOL, (s.conservation_law.b, _rel(to_host(x), ref))
+
+
+@pytest.mark.parametrize("chunk", [None, "2"])
+def test_diffusivity_tangent_matches_oracle_forward_mode(chunk, monkeypatch):
+    # (u, du/db) pairs advanced together against forward-mode duals through the
+    # oracle's RK; the energy gradient dJ/db = u' M du/db against the oracle's too
+    import torch
+    from oracle import functionals as F
+    from sse.analysis import EnergyConservationAnalysis
+    from sse.batch import BatchedSolver
+    from sse.time_integration import to_device, to_host
+    if chunk:
+        monkeypatch.setenv("SSE_ENSEMBLE_CHUNK", chunk)
+    bs = (3e-3, 1e-2, 6e-3)
+    solvers, u0s = _advdiff_ensemble(4, bs)
+    batch = BatchedSolver(solvers)
+    dt, n = 1e-3, 10
+    states = [to_device(u) for u in u0s]
+    tangents = [torch.zeros_like(x) for x in states]
+    batch.rk_tangent(states, tangents, [dt] * len(bs), n)
+    for s, u0, x, v in zip(solvers, u0s, states, tangents):
+        uN, duN = O.rk_jvp(s.arrays, u0, np.zeros_like(u0), dt, n, db=1.0)
+        assert _rel(to_host(x), uN) <= RK_TOL
+        assert _rel(to_host(v), duN) <= RK_TOL, _rel(to_host(v), duN)
+        g = F.energy_rate(s.spatial_discretization, uN, duN)[0]
+        got = EnergyConservationAnalysis(s).evaluate_conservation_residual(x, v)[0]
+        assert abs(got - g) <= 1e-10 * abs(g), (got, g)
+
+
 def test_gpu_reproduces_reference_golden_advection_diffusion_1d():
     # test/runtests.jl:13-36, whole trajectory on the GPU
     from sse.time_integration import CarpenterKennedy2N54, solve

@@ -343,6 +343,43 @@ def test_standard_form_br1_converges_to_exact_gaussian():
     assert errs[1] < 0.7 * errs[0] and errs[1] < 0.02, errs
 
 
+def _advdiff_with_b(b, M=2, p=3):
+    from sse.conservation_laws import LinearAdvectionDiffusionEquation
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=M, p=p, b=5e-3, sigma=0.25)
+    return ode, ode.p.with_law(LinearAdvectionDiffusionEquation(law.a, b))
+
+
+def test_br1_residual_derivative_in_b_is_the_viscous_part():
+    # the BR1 residual is affine in b (viscous flux -b q, interface flux -b {q}.n,
+    # q independent of b): dR/db = R(u; 1) - R(u; 0) whatever b; a tangent that
+    # missed the interface term or the volume term fails this
+    ode, s7 = _advdiff_with_b(7e-3)
+    _, s0 = _advdiff_with_b(0.0)
+    _, s1 = _advdiff_with_b(1.0)
+    u = ode.u0
+    r, dr = O.residual_jvp(s7.arrays, u, np.zeros_like(u), 0.0, 1.0)
+    np.testing.assert_allclose(r, O.residual(s7.arrays, u), rtol=0, atol=1e-14 * np.abs(r).max())
+    want = O.residual(s1.arrays, u) - O.residual(s0.arrays, u)
+    assert np.abs(dr - want).max() <= 1e-12 * np.abs(want).max()
+    # and R(u; b) = R(u; 0) + b dR/db
+    np.testing.assert_allclose(r, O.residual(s0.arrays, u) + 7e-3 * dr, rtol=0, atol=1e-12 * np.abs(r).max())
+
+
+def test_br1_rk_tangent_in_b_matches_central_differences():
+    # d u_N / d b through 10 CK2N54 steps (forward-mode duals) against a central
+    # difference in b; u_N is a polynomial in b, the difference error is O(h^2)
+    b, h, dt, n = 5e-3, 1e-6, 2e-3, 10
+    ode, sb = _advdiff_with_b(b)
+    _, sp = _advdiff_with_b(b + h)
+    _, sm = _advdiff_with_b(b - h)
+    u0 = ode.u0
+    uN, duN = O.rk_jvp(sb.arrays, u0, np.zeros_like(u0), dt, n, db=1.0)
+    fd = (O.rk_steps(sp.arrays, u0, dt, n) - O.rk_steps(sm.arrays, u0, dt, n)) / (2 * h)
+    np.testing.assert_allclose(uN, O.rk_steps(sb.arrays, u0, dt, n), rtol=0, atol=1e-14 * np.abs(uN).max())
+    assert np.abs(duN).max() > 0
+    assert np.abs(duN - fd).max() <= 1e-6 * np.abs(duN).max(), np.abs(duN - fd).max() / np.abs(duN).max()
+
+
 # ----------------------------------------------------------------------------- functionals
 def test_functionals_closed_forms():
     from oracle import functionals as F
