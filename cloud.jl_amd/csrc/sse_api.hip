@@ -61,6 +61,7 @@ int struct_occupancy(int elem, int P);
 hipError_t launch_struct(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
                          hipStream_t s, int n_blocks);
 size_t std_tensor_config(int elem, int P);
+hipError_t collapse_metric(double* Lam, const double* c, int NQ, int D, int N_e, hipStream_t s);
 hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, int tangent, const double* u,
                              double* dudt, const RKStage& rk, hipStream_t s, int which);
 }  // namespace sse
@@ -516,15 +517,91 @@ int build_std_tensor(sse_handle* h, const sse_desc* D) {
     std::vector<double> r_val, rt_val;
     ell(rrows, T.wr, r_idx, r_val);
     ell(rtrows, T.wrt, rt_idx, rt_val);
+    // the kernels read ELL rows four entries at a time
+    auto pad4 = [](int& width, int rows, std::vector<int>& idx, std::vector<double>& val) {
+        const int w4 = (std::max(width, 1) + 3) / 4 * 4;
+        idx.resize((size_t)w4 * rows, -1);
+        val.resize((size_t)w4 * rows, 0.0);
+        width = w4;
+    };
+    pad4(T.wr, nf, r_idx, r_val);
+    pad4(T.wrt, nq, rt_idx, rt_val);
     std::vector<double> VA(D->V_A, D->V_A + (size_t)n1 * n1);
     std::vector<double> VB(D->V_B, D->V_B + (size_t)n1 * n1 * n1);
     std::vector<double> VC;
     if (tet) VC.assign(D->V_C, D->V_C + (size_t)n1 * n1 * n1 * n1);
+    // D_m = sum_l diag(c_lm) D1_l along the lines of the tensor grid: D1_l row a
+    // from the line through node a * stride_l (row sums of a derivative vanish),
+    // c by least squares, then the whole of every D_m is checked against it
+    const int d = D->d;
+    int stride[3];
+    for (int l = 0; l < d; ++l) {
+        stride[l] = 1;
+        for (int t = l + 1; t < d; ++t) stride[l] *= n1;
+    }
+    auto coord = [&](int i, int l) { return (i / stride[l]) % n1; };
+    auto Dm = [&](int i, int j, int m) { return D->D[i + (size_t)nq * (j + (size_t)nq * m)]; };
+    std::vector<double> D1((size_t)d * n1 * n1, 0.0), c((size_t)nq * d * d, 0.0);
+    for (int l = 0; l < d; ++l)
+        for (int a = 0; a < n1; ++a) {
+            const int i0 = a * stride[l];
+            int mbest = -1;
+            double best = 0.0;
+            for (int m = 0; m < d; ++m) {
+                double w = 0.0;
+                for (int b2 = 0; b2 < n1; ++b2)
+                    if (b2 != a) w += std::fabs(Dm(i0, i0 + (b2 - a) * stride[l], m));
+                if (w > best) { best = w; mbest = m; }
+            }
+            if (mbest < 0) return 0;
+            double rs = 0.0;
+            for (int b2 = 0; b2 < n1; ++b2)
+                if (b2 != a) {
+                    const double v = Dm(i0, i0 + (b2 - a) * stride[l], mbest);
+                    D1[((size_t)l * n1 + a) * n1 + b2] = v;
+                    rs += v;
+                }
+            D1[((size_t)l * n1 + a) * n1 + a] = -rs;
+        }
+    double dmax = 0.0;
+    for (size_t t = 0; t < (size_t)nq * nq * d; ++t) dmax = std::max(dmax, std::fabs(D->D[t]));
+    for (int i = 0; i < nq; ++i)
+        for (int l = 0; l < d; ++l) {
+            const int a = coord(i, l);
+            double den = 0.0;
+            for (int b2 = 0; b2 < n1; ++b2)
+                if (b2 != a) den += D1[((size_t)l * n1 + a) * n1 + b2] * D1[((size_t)l * n1 + a) * n1 + b2];
+            for (int m = 0; m < d; ++m) {
+                double num = 0.0;
+                for (int b2 = 0; b2 < n1; ++b2)
+                    if (b2 != a) num += Dm(i, i + (b2 - a) * stride[l], m) * D1[((size_t)l * n1 + a) * n1 + b2];
+                c[((size_t)i * d + l) * d + m] = den > 0.0 ? num / den : 0.0;
+            }
+        }
+    for (int m = 0; m < d; ++m)
+        for (int i = 0; i < nq; ++i)
+            for (int j = 0; j < nq; ++j) {
+                double rec = 0.0;
+                for (int l = 0; l < d; ++l) {
+                    bool on_line = true;
+                    for (int t = 0; t < d; ++t)
+                        if (t != l && coord(i, t) != coord(j, t)) on_line = false;
+                    if (on_line) rec += c[((size_t)i * d + l) * d + m] * D1[((size_t)l * n1 + coord(i, l)) * n1 + coord(j, l)];
+                }
+                if (std::fabs(rec - Dm(i, j, m)) > 1e-12 * std::max(dmax, 1.0)) return 0;
+            }
     int rc;
+    const double* c_dev = nullptr;
     if ((rc = upload(h, r_idx, &T.r_idx)) || (rc = upload(h, r_val, &T.r_val)) || (rc = upload(h, rt_idx, &T.rt_idx)) ||
-        (rc = upload(h, rt_val, &T.rt_val)) || (rc = upload(h, VA, &T.VA)) || (rc = upload(h, VB, &T.VB)))
+        (rc = upload(h, rt_val, &T.rt_val)) || (rc = upload(h, VA, &T.VA)) || (rc = upload(h, VB, &T.VB)) ||
+        (rc = upload(h, D1, &T.D1)) || (rc = upload(h, c, &c_dev)))
         return rc;
     if (tet && (rc = upload(h, VC, &T.VC))) return rc;
+    // the kernels read Lt_ln = sum_m c_lm Lambda_mn in place of Lambda (this handle's copy)
+    hipError_t e = sse::collapse_metric(const_cast<double*>(P.Lam), c_dev, nq, d, P.N_e, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("metric set-up: ") + hipGetErrorString(e));
+    h->TP.base = h->P;
     T.xcd_remap = 1;
     h->use_std_tensor = true;
     return 0;

@@ -76,6 +76,8 @@ struct TensorParams {
     const double* VA;    // warped-product factors (column-major, as in the reference)
     const double* VB;
     const double* VC;
+    const double* D1;    // StandardForm scalar kernels: 1D derivative matrix of each collapsed
+                         // direction, d x N1 x N1 row-major (D_m = sum_l diag(c_lm) D1_l along lines)
     int xcd_remap;       // 1: spread contiguous element ranges over the 8 XCDs
     double* uf_ext;      // when set, nodal_values also writes each facet state into the
                          // neighbour's exterior slot [k][e][j] (mapP is an involution)

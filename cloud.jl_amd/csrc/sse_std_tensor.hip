@@ -14,6 +14,14 @@
 //    from registers;
 //  * V and V' are sum-factorised (warped-product factors), the mass solve is
 //    the weight-adjusted one with M^{-1} = I (src/Solvers/mass_matrix.jl:59-75);
+//  * the reference derivative operators are applied along lines: with
+//    D_m = sum_l diag(c_lm) D1_l (D1_l the 1D derivative in collapsed direction l
+//    on every line of the tensor grid, src/SpatialDiscretizations/
+//    SpatialDiscretizations.jl:414-418) and the per-element metric folded into
+//    Lt_ln = sum_m c_lm Lambda_mn once at set-up,
+//      sum_m D_m' (W Lambda_mn x) = sum_l D1_l' (W Lt_ln x),
+//      sum_m Lambda_mn (D_m x)    = sum_l Lt_ln (D1_l x),
+//    so a node's derivative reads N1 neighbours per direction and no table;
 //  * the BR1 auxiliary variable of all d directions goes through one
 //    three-direction mass solve, and R q_q reuses the stored facet values;
 //  * with the skew-symmetric mapping, R x at the facets is never recomputed:
@@ -43,7 +51,9 @@ struct Ensemble {
                         // is R(v; b) + dR/db(u), dR/db(u) = -(viscous terms of u at b = 1)
 };
 
-// threads per workgroup and slots per workgroup of the three kernels
+// threads per workgroup and slots per workgroup of the three kernels (few slots
+// per workgroup measured fastest: more workgroups in flight hide the latency of
+// the dependent stages better than wider reuse of the small tables)
 template <int ELEM, int P>
 struct StdTune;
 #define SSE_STD_TUNE(E_, PP, ts, e1, e2, e3) \
@@ -51,20 +61,31 @@ struct StdTune;
     struct StdTune<E_, PP> {                 \
         static constexpr int TS = ts, E1 = e1, E2 = e2, E3 = e3; \
     };
-SSE_STD_TUNE(ELEM_TRI, 2, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TRI, 3, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TRI, 4, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TRI, 5, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TRI, 6, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TET, 2, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TET, 3, 64, 8, 4, 4)
-SSE_STD_TUNE(ELEM_TET, 4, 128, 8, 2, 4)
-SSE_STD_TUNE(ELEM_TET, 5, 256, 8, 2, 4)
+SSE_STD_TUNE(ELEM_TRI, 2, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TRI, 3, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TRI, 4, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TRI, 5, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TRI, 6, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TET, 2, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TET, 3, 64, 8, 2, 2)
+SSE_STD_TUNE(ELEM_TET, 4, 128, 8, 1, 2)
+SSE_STD_TUNE(ELEM_TET, 5, 256, 8, 1, 2)
 #undef SSE_STD_TUNE
 
 namespace {
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// Node i of the tensor grid has coordinate (i / stride_l) % N1 in direction l
+// (the last direction fastest).
+template <int ELEM, int P>
+struct Lines {
+    static constexpr int D = Shape<ELEM, P>::D, N1 = P + 1;
+    __host__ __device__ static constexpr int stride(int l) {
+        return D == 3 ? (l == 0 ? N1 * N1 : (l == 1 ? N1 : 1)) : (l == 0 ? N1 : 1);
+    }
+    __device__ static int coord(int i, int l) { return (i / stride(l)) % N1; }
+};
 
 __device__ __forceinline__ size_t state_row(long long g, int R, int N_e) {
     const long long k = g / R, r = g - k * R;
@@ -82,6 +103,30 @@ __device__ __forceinline__ long long neighbour(const DevParams& p, int R, long l
     const int kn = gl / NF;
     jn = gl - kn * NF;
     return (long long)kn * R + r;
+}
+
+// acc[c] += sum over row `row` of an ELL table (width a multiple of 4, -1 pads
+// with value 0) of val * x[c * XS + idx]: four entries per step, so that a
+// step's loads are in flight together
+template <int NCV, int XS>
+__device__ __forceinline__ void ell_apply(const int* __restrict__ idx, const double* __restrict__ val, int width,
+                                          int nrows, int row, const double* x, double* acc) {
+    for (int w = 0; w < width; w += 4) {
+        int ii[4];
+        double vv[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            ii[t] = idx[(w + t) * nrows + row];
+            vv[t] = val[(w + t) * nrows + row];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int i = ii[t] < 0 ? 0 : ii[t];
+#pragma unroll
+            for (int c = 0; c < NCV; ++c) acc[c] += vv[t] * x[c * XS + i];
+        }
+        if (ii[3] < 0) break;
+    }
 }
 
 }  // namespace
@@ -123,13 +168,7 @@ k_std_nodal_tensor(TensorParams tp, Ensemble en, const double* __restrict__ u) {
         double acc[E];
 #pragma unroll
         for (int s = 0; s < E; ++s) acc[s] = 0.0;
-        for (int w = 0; w < tp.wr; ++w) {
-            const int i = tp.r_idx[w * NF + j];
-            if (i < 0) break;
-            const double v = tp.r_val[w * NF + j];
-#pragma unroll
-            for (int s = 0; s < E; ++s) acc[s] += v * sq[s * NQ + i];
-        }
+        ell_apply<E, NQ>(tp.r_idx, tp.r_val, tp.wr, NF, j, sq, acc);
 #pragma unroll
         for (int s = 0; s < E; ++s)
             if (s < nvalid) p.uf[(size_t)(g0 + s) * NF + j] = acc[s];
@@ -140,24 +179,28 @@ k_std_nodal_tensor(TensorParams tp, Ensemble en, const double* __restrict__ u) {
 // q_m = M^{-1} V' [ -VT_m(u_q) + R' (B J_f u*_m) ],  u*_m = 1/2 (u^- + u^+) n_m,
 // VT_n(x) = h sum_m D_m' (W Lambda_mn x) - h W sum_m Lambda_mn (D_m x) + R' (1/2 B nJf_n R x)
 // (h = 1/2, skew-symmetric mapping) or sum_m D_m' (W Lambda_mn x) (standard mapping).
-// With the skew form the facet terms combine to R' (1/2 B nJf_n u^+).
-template <int ELEM, int P>
+// With the skew form the facet terms combine to R' (1/2 B nJf_n u^+). All d
+// directions go through each pass together.
+template <int ELEM, int P, bool SKEW>
 struct StdAuxLayout {
     using Sh = Shape<ELEM, P>;
     static constexpr int E = StdTune<ELEM, P>::E2, D = Sh::D;
-    static constexpr int A = cmax(D * Sh::NQ, D * (Sh::NZ + Sh::NP));  // D u, then SumFact Z and y
-    static constexpr int B = cmax(D * Sh::NQ, D * Sh::NW);             // g, then SumFact W
-    static constexpr int uq = 0, up = uq + E * Sh::NQ, um = up + E * Sh::NF, a = um + E * Sh::NF;
+    static constexpr int A = cmax(D * Sh::NQ, D * (Sh::NZ + Sh::NP));  // D1_l u, then SumFact Z and y
+    static constexpr int B = cmax(D * D * Sh::NQ, D * Sh::NW);         // g_ln, then SumFact W
+    // u^- only for the standard mapping
+    static constexpr int uq = 0, up = uq + E * Sh::NQ, um = up + E * Sh::NF, a = um + (SKEW ? 0 : E * Sh::NF);
     static constexpr int b = a + E * A, r = b + E * B, tf = r + E * D * Sh::NQ;
-    static constexpr int size = tf + E * Sh::NF;
+    static constexpr int d1 = tf + E * D * Sh::NF;
+    static constexpr int size = d1 + D * Sh::N1 * Sh::N1;
 };
 
 template <int ELEM, int P, bool SKEW>
 __global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(TensorParams tp, Ensemble en) {
     using Sh = Shape<ELEM, P>;
     using T = StdTune<ELEM, P>;
-    using L = StdAuxLayout<ELEM, P>;
-    constexpr int E = T::E2, TS = T::TS, D = Sh::D, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF;
+    using L = StdAuxLayout<ELEM, P, SKEW>;
+    using LN = Lines<ELEM, P>;
+    constexpr int E = T::E2, TS = T::TS, D = Sh::D, NQ = Sh::NQ, NF = Sh::NF, N1 = Sh::N1;
     using SF = SumFact<ELEM, P, TS, D * E>;
     extern __shared__ double sm[];
     const DevParams& p = tp.base;
@@ -167,9 +210,10 @@ __global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(Tenso
     const long long g0 = (long long)element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * E;
     const int nvalid = (int)(nl - g0 < E ? nl - g0 : E);
     double *suq = sm + L::uq, *sup = sm + L::up, *sum = sm + L::um, *sDu = sm + L::a, *sg = sm + L::b,
-           *sr = sm + L::r, *stf = sm + L::tf;
+           *sr = sm + L::r, *stf = sm + L::tf, *sD1 = sm + L::d1;
     constexpr double h = SKEW ? 0.5 : 1.0;
 
+    for (int t = tl; t < D * N1 * N1; t += TS) sD1[t] = tp.D1[t];
     for (int t = tl; t < E * NQ; t += TS) suq[t] = t < nvalid * NQ ? p.uq[(size_t)g0 * NQ + t] : 0.0;
     for (int t = tl; t < E * NF; t += TS) {
         const int s = t / NF, j = t - s * NF;
@@ -179,93 +223,88 @@ __global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(Tenso
             int jn;
             const long long gn = neighbour(p, R, g, NF, j, jn);
             up = p.uf[(size_t)gn * NF + jn];
-            um = p.uf[(size_t)g * NF + j];
+            if (!SKEW) um = p.uf[(size_t)g * NF + j];
         }
         sup[t] = up;
-        sum[t] = um;
+        if (!SKEW) sum[t] = um;
     }
     __syncthreads();
-    if constexpr (SKEW) {  // D_m u_q for all m
+    if constexpr (SKEW) {  // D1_l u_q along the lines of each direction l
         for (int i = tl; i < NQ; i += TS) {
-            double acc[E][D];
 #pragma unroll
-            for (int s = 0; s < E; ++s)
+            for (int l = 0; l < D; ++l) {
+                const int sl = LN::stride(l), a = LN::coord(i, l), base = i - a * sl;
+                const double* drow = sD1 + (l * N1 + a) * N1;
+                double acc[E];
 #pragma unroll
-                for (int m = 0; m < D; ++m) acc[s][m] = 0.0;
-            for (int t = p.D_ptr[i]; t < p.D_ptr[i + 1]; ++t) {
-                const int j = p.D_idx[t];
-                double dv[D];
+                for (int s = 0; s < E; ++s) acc[s] = 0.0;
 #pragma unroll
-                for (int m = 0; m < D; ++m) dv[m] = p.D_val[t * D + m];
+                for (int bb = 0; bb < N1; ++bb) {
+                    const double dv = drow[bb];
 #pragma unroll
-                for (int s = 0; s < E; ++s) {
-                    const double x = suq[s * NQ + j];
-#pragma unroll
-                    for (int m = 0; m < D; ++m) acc[s][m] += dv[m] * x;
+                    for (int s = 0; s < E; ++s) acc[s] += dv * suq[s * NQ + base + bb * sl];
                 }
+#pragma unroll
+                for (int s = 0; s < E; ++s) sDu[(s * D + l) * NQ + i] = acc[s];
             }
-#pragma unroll
-            for (int s = 0; s < E; ++s)
-#pragma unroll
-                for (int m = 0; m < D; ++m) sDu[(s * D + m) * NQ + i] = acc[s][m];
         }
         __syncthreads();
     }
-    for (int n = 0; n < D; ++n) {
-        // g_m = h W Lambda_mn u_q; r_n starts as h W sum_m Lambda_mn (D_m u_q)
-        for (int t = tl; t < E * NQ; t += TS) {
-            const int s = t / NQ, i = t - s * NQ;
-            const long long k = (g0 + (s < nvalid ? s : 0)) / R;
-            const double* Lk = p.Lam + (size_t)k * NQ * D * D;
-            const double hw = h * p.W[i];
-            const double x = suq[t];
+    // g_ln = h W Lt_ln u_q; r_n starts as h W sum_l Lt_ln (D1_l u_q)
+    for (int t = tl; t < E * NQ; t += TS) {
+        const int s = t / NQ, i = t - s * NQ;
+        const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+        const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+        const double hw = h * p.W[i];
+        const double x = suq[t];
+        double lam[D][D];
+#pragma unroll
+        for (int n = 0; n < D; ++n)
+#pragma unroll
+            for (int l = 0; l < D; ++l) lam[l][n] = Lk[i + NQ * (l + D * n)];
+#pragma unroll
+        for (int n = 0; n < D; ++n) {
             double dterm = 0.0;
 #pragma unroll
-            for (int m = 0; m < D; ++m) {
-                const double lam = Lk[i + NQ * (m + D * n)];
-                sg[(s * D + m) * NQ + i] = hw * lam * x;
-                if constexpr (SKEW) dterm += lam * sDu[(s * D + m) * NQ + i];
+            for (int l = 0; l < D; ++l) {
+                sg[((s * D + l) * D + n) * NQ + i] = hw * lam[l][n] * x;
+                if constexpr (SKEW) dterm += lam[l][n] * sDu[(s * D + l) * NQ + i];
             }
             sr[(s * D + n) * NQ + i] = hw * dterm;
         }
-        // facet: 1/2 B nJf_n u^+ (skew) or 1/2 B nJf_n (u^- + u^+)
-        for (int t = tl; t < E * NF; t += TS) {
-            const int s = t / NF, j = t - s * NF;
-            const long long k = (g0 + (s < nvalid ? s : 0)) / R;
-            const double nj = p.nJf[(size_t)k * D * NF + n + D * j];
-            stf[t] = 0.5 * p.B[j] * nj * (SKEW ? sup[t] : sup[t] + sum[t]);
-        }
-        __syncthreads();
-        // r_n -= sum_m D_m' g_m;  r_n += R' tf
-        for (int i = tl; i < NQ; i += TS) {
-            double acc[E];
-#pragma unroll
-            for (int s = 0; s < E; ++s) acc[s] = 0.0;
-            for (int t = p.Dt_ptr[i]; t < p.Dt_ptr[i + 1]; ++t) {
-                const int j = p.Dt_idx[t];
-                double dv[D];
-#pragma unroll
-                for (int m = 0; m < D; ++m) dv[m] = p.Dt_val[t * D + m];
-#pragma unroll
-                for (int s = 0; s < E; ++s) {
-                    double a = 0.0;
-#pragma unroll
-                    for (int m = 0; m < D; ++m) a += dv[m] * sg[(s * D + m) * NQ + j];
-                    acc[s] -= a;
-                }
-            }
-            for (int w = 0; w < tp.wrt; ++w) {
-                const int j = tp.rt_idx[w * NQ + i];
-                if (j < 0) break;
-                const double v = tp.rt_val[w * NQ + i];
-#pragma unroll
-                for (int s = 0; s < E; ++s) acc[s] += v * stf[s * NF + j];
-            }
-#pragma unroll
-            for (int s = 0; s < E; ++s) sr[(s * D + n) * NQ + i] += acc[s];
-        }
-        __syncthreads();
     }
+    // facet: 1/2 B nJf_n u^+ (skew) or 1/2 B nJf_n (u^- + u^+)
+    for (int t = tl; t < E * NF; t += TS) {
+        const int s = t / NF, j = t - s * NF;
+        const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+        const double* nJ = p.nJf + (size_t)k * D * NF + D * j;
+        const double x = 0.5 * p.B[j] * (SKEW ? sup[t] : sup[t] + sum[t]);  // sum unread when SKEW
+#pragma unroll
+        for (int n = 0; n < D; ++n) stf[(s * D + n) * NF + j] = nJ[n] * x;
+    }
+    __syncthreads();
+    // r_n -= sum_l D1_l' g_ln;  r_n += R' tf_n
+    for (int i = tl; i < NQ; i += TS) {
+        double acc[E * D];
+#pragma unroll
+        for (int c = 0; c < E * D; ++c) acc[c] = 0.0;
+#pragma unroll 1
+        for (int l = 0; l < D; ++l) {
+            const int sl = LN::stride(l), a = LN::coord(i, l), base = i - a * sl;
+#pragma unroll
+            for (int bb = 0; bb < N1; ++bb) {
+                const double dv = sD1[(l * N1 + bb) * N1 + a];
+#pragma unroll
+                for (int s = 0; s < E; ++s)
+#pragma unroll
+                    for (int n = 0; n < D; ++n) acc[s * D + n] -= dv * sg[((s * D + l) * D + n) * NQ + base + bb * sl];
+            }
+        }
+        ell_apply<E * D, NF>(tp.rt_idx, tp.rt_val, tp.wrt, NQ, i, stf, acc);
+#pragma unroll
+        for (int c = 0; c < E * D; ++c) sr[c * NQ + i] += acc[c];
+    }
+    __syncthreads();
     // three-direction mass solve: q = V' (W/J o V (V' r)); then q_q = V q
     double* sZ = sDu;
     double* sy = sDu + E * D * Sh::NZ;
@@ -285,13 +324,7 @@ __global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(Tenso
         double acc[E * D];
 #pragma unroll
         for (int c = 0; c < E * D; ++c) acc[c] = 0.0;
-        for (int w = 0; w < tp.wr; ++w) {
-            const int i = tp.r_idx[w * NF + j];
-            if (i < 0) break;
-            const double v = tp.r_val[w * NF + j];
-#pragma unroll
-            for (int c = 0; c < E * D; ++c) acc[c] += v * sr[c * NQ + i];
-        }
+        ell_apply<E * D, NQ>(tp.r_idx, tp.r_val, tp.wr, NF, j, sr, acc);
 #pragma unroll
         for (int c = 0; c < E * D; ++c)
             if (c < nvalid * D) p.qf[((size_t)g0 * D + c) * NF + j] = acc[c];
@@ -307,8 +340,8 @@ struct StdTdLayout {
     static constexpr int E = StdTune<ELEM, P>::E3, D = Sh::D;
     static constexpr int F = 2 * D * Sh::NQ;  // f_n and G_m, then SumFact Z, W, y, tmp
     static_assert(F >= Sh::NZ + cmax(Sh::NW, 1) + Sh::NP + Sh::NQ, "mass-solve scratch");
-    static constexpr int f = 0, r = f + E * F, t = r + E * Sh::NQ;
-    static constexpr int size = t + E * Sh::NF;
+    static constexpr int f = 0, r = f + E * F, t = r + E * Sh::NQ, d1 = t + E * Sh::NF;
+    static constexpr int size = d1 + D * Sh::N1 * Sh::N1;
 };
 
 template <int ELEM, int P, bool SKEW, bool SECOND>
@@ -326,10 +359,13 @@ k_std_td_tensor(TensorParams tp, Ensemble en, double* __restrict__ dudt, RKStage
     const long long nl = (long long)p.N_e * R;
     const long long g0 = (long long)element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap) * E;
     const int nvalid = (int)(nl - g0 < E ? nl - g0 : E);
-    double *sf = sm + L::f, *sG = sf + E * D * NQ, *sr = sm + L::r, *st = sm + L::t;
+    double *sf = sm + L::f, *sG = sf + E * D * NQ, *sr = sm + L::r, *st = sm + L::t, *sD1 = sm + L::d1;
     constexpr double h = SKEW ? 0.5 : 1.0;
+    using LN = Lines<ELEM, P>;
+    constexpr int N1 = Sh::N1;
+    for (int t = tl; t < D * N1 * N1; t += TS) sD1[t] = tp.D1[t];
 
-    // volume fluxes f_n and G_m = h W sum_n Lambda_mn f_n
+    // volume fluxes f_n and G_l = h W sum_n Lt_ln f_n
     for (int t = tl; t < E * NQ; t += TS) {
         const int s = t / NQ, i = t - s * NQ;
         const long long g = g0 + (s < nvalid ? s : 0);
@@ -405,65 +441,48 @@ k_std_td_tensor(TensorParams tp, Ensemble en, double* __restrict__ dudt, RKStage
         st[t] = val;
     }
     __syncthreads();
-    // r = sum_m D_m' G_m - h W sum_{m,n} Lambda_mn (D_m f_n) + R' t
+    // r = sum_l D1_l' G_l - h W sum_{l,n} Lt_ln (D1_l f_n) + R' t
     for (int i = tl; i < NQ; i += TS) {
         double acc[E];
 #pragma unroll
         for (int s = 0; s < E; ++s) acc[s] = 0.0;
-        for (int t = p.Dt_ptr[i]; t < p.Dt_ptr[i + 1]; ++t) {
-            const int j = p.Dt_idx[t];
-            double dv[D];
+#pragma unroll 1
+        for (int l = 0; l < D; ++l) {
+            const int sl = LN::stride(l), a = LN::coord(i, l), base = i - a * sl;
 #pragma unroll
-            for (int m = 0; m < D; ++m) dv[m] = p.Dt_val[t * D + m];
+            for (int bb = 0; bb < N1; ++bb) {
+                const double dv = sD1[(l * N1 + bb) * N1 + a];
 #pragma unroll
-            for (int s = 0; s < E; ++s)
+                for (int s = 0; s < E; ++s) acc[s] += dv * sG[(s * D + l) * NQ + base + bb * sl];
+            }
+            if constexpr (SKEW) {
+                double y[E][D];
 #pragma unroll
-                for (int m = 0; m < D; ++m) acc[s] += dv[m] * sG[(s * D + m) * NQ + j];
-        }
-        if constexpr (SKEW) {
-            for (int s0 = 0; s0 < E; s0 += 2) {  // two slots at a time: 2 D^2 accumulators
-                double dd[2][D][D];
+                for (int s = 0; s < E; ++s)
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
+                    for (int n = 0; n < D; ++n) y[s][n] = 0.0;
+                const double* drow = sD1 + (l * N1 + a) * N1;
 #pragma unroll
-                    for (int m = 0; m < D; ++m)
+                for (int bb = 0; bb < N1; ++bb) {
+                    const double dv = drow[bb];
 #pragma unroll
-                        for (int n = 0; n < D; ++n) dd[s][m][n] = 0.0;
-                for (int t = p.D_ptr[i]; t < p.D_ptr[i + 1]; ++t) {
-                    const int j = p.D_idx[t];
-                    double dv[D];
+                    for (int s = 0; s < E; ++s)
 #pragma unroll
-                    for (int m = 0; m < D; ++m) dv[m] = p.D_val[t * D + m];
-#pragma unroll
-                    for (int s = 0; s < 2; ++s)
-#pragma unroll
-                        for (int n = 0; n < D; ++n) {
-                            const double x = (s0 + s < E) ? sf[((s0 + s) * D + n) * NQ + j] : 0.0;
-#pragma unroll
-                            for (int m = 0; m < D; ++m) dd[s][m][n] += dv[m] * x;
-                        }
+                        for (int n = 0; n < D; ++n) y[s][n] += dv * sf[(s * D + n) * NQ + base + bb * sl];
                 }
+                const double hw = h * p.W[i];
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    if (s0 + s >= E) break;
-                    const long long k = (g0 + (s0 + s < nvalid ? s0 + s : 0)) / R;
+                for (int s = 0; s < E; ++s) {
+                    const long long k = (g0 + (s < nvalid ? s : 0)) / R;
                     const double* Lk = p.Lam + (size_t)k * NQ * D * D;
-                    double a = 0.0;
+                    double c = 0.0;
 #pragma unroll
-                    for (int m = 0; m < D; ++m)
-#pragma unroll
-                        for (int n = 0; n < D; ++n) a += Lk[i + NQ * (m + D * n)] * dd[s][m][n];
-                    acc[s0 + s] -= h * p.W[i] * a;
+                    for (int n = 0; n < D; ++n) c += Lk[i + NQ * (l + D * n)] * y[s][n];
+                    acc[s] -= hw * c;
                 }
             }
         }
-        for (int w = 0; w < tp.wrt; ++w) {
-            const int j = tp.rt_idx[w * NQ + i];
-            if (j < 0) break;
-            const double v = tp.rt_val[w * NQ + i];
-#pragma unroll
-            for (int s = 0; s < E; ++s) acc[s] += v * st[s * NF + j];
-        }
+        ell_apply<E, NF>(tp.rt_idx, tp.rt_val, tp.wrt, NQ, i, st, acc);
 #pragma unroll
         for (int s = 0; s < E; ++s) sr[s * NQ + i] = acc[s];
     }
@@ -495,11 +514,41 @@ k_std_td_tensor(TensorParams tp, Ensemble en, double* __restrict__ dudt, RKStage
     }
 }
 
+// ----------------------------------------------------------------------------- set-up
+// In place: Lambda_mn -> Lt_ln = sum_m c_lm Lambda_mn at every node of every
+// element (c: N_q x d x d, c[(i d + l) d + m]).
+__global__ void k_collapse_metric(double* __restrict__ Lam, const double* __restrict__ c, int NQ, int D,
+                                  long long n_nodes) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_nodes) return;
+    const long long k = t / NQ;
+    const int i = (int)(t - k * NQ);
+    double* Lk = Lam + (size_t)k * NQ * D * D;
+    double in[9], out[9];
+    for (int m = 0; m < D; ++m)
+        for (int n = 0; n < D; ++n) in[m + 3 * n] = Lk[i + NQ * (m + D * n)];
+    for (int l = 0; l < D; ++l)
+        for (int n = 0; n < D; ++n) {
+            double s = 0.0;
+            for (int m = 0; m < D; ++m) s += c[(i * D + l) * D + m] * in[m + 3 * n];
+            out[l + 3 * n] = s;
+        }
+    for (int l = 0; l < D; ++l)
+        for (int n = 0; n < D; ++n) Lk[i + NQ * (l + D * n)] = out[l + 3 * n];
+}
+
+hipError_t collapse_metric(double* Lam, const double* c, int NQ, int D, int N_e, hipStream_t s) {
+    const long long n = (long long)NQ * N_e;
+    hipLaunchKernelGGL(k_collapse_metric, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Lam, c, NQ, D, n);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------- launch
 template <int ELEM, int P>
 size_t std_tensor_lds(int which) {
     if (which == 0) return sizeof(double) * StdNodalLayout<ELEM, P>::size;
-    if (which == 1) return sizeof(double) * StdAuxLayout<ELEM, P>::size;
+    if (which == 1) return sizeof(double) * cmax(StdAuxLayout<ELEM, P, true>::size, StdAuxLayout<ELEM, P, false>::size);
+    if (which == 3) return sizeof(double) * StdAuxLayout<ELEM, P, true>::size;
     return sizeof(double) * StdTdLayout<ELEM, P>::size;
 }
 
@@ -508,7 +557,8 @@ hipError_t launch_std_tensor_t(const TensorParams& tp, const Ensemble& en, const
                                const RKStage& rk, hipStream_t s, int which) {
     using T = StdTune<ELEM, P>;
     static bool attr_set = false;
-    const size_t l0 = std_tensor_lds<ELEM, P>(0), l1 = std_tensor_lds<ELEM, P>(1), l2 = std_tensor_lds<ELEM, P>(2);
+    const size_t l0 = std_tensor_lds<ELEM, P>(0), l1 = std_tensor_lds<ELEM, P>(SKEW ? 3 : 1),
+                 l2 = std_tensor_lds<ELEM, P>(2);
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void*)k_std_nodal_tensor<ELEM, P>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l0);

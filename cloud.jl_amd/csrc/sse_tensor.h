@@ -259,25 +259,38 @@ struct SumFact<ELEM_TET, P, TS, NCV> {
             for (int e = 0; e < NC; ++e) Z[e * NZ + o] = s[e];
         }
         __syncthreads();
-        for (int q = tl; q < Sh::NPAIR12; q += TS) {
-            int b1, b2;
-            decode_q(q, b1, b2);
-            const int off = mode_offset3<P>(b1, b2);
-            for (int b3 = 0; b3 <= P - b1 - b2; ++b3) {
-                double s[NC];
+        for (int m = tl; m < NP; m += TS) {  // one thread per mode (b1, b2, b3)
+            int b1, b2, b3;
+            decode_mode(m, b1, b2, b3);
+            const int q = q_index<P>(b1, b2);
+            double s[NC];
 #pragma unroll
-                for (int e = 0; e < NC; ++e) s[e] = 0.0;
+            for (int e = 0; e < NC; ++e) s[e] = 0.0;
 #pragma unroll
-                for (int a3 = 0; a3 < N1; ++a3) {
-                    const double cv = tp.VC[a3 + N1 * (b1 + (P + 1) * (b2 + (P + 1) * b3))];
+            for (int a3 = 0; a3 < N1; ++a3) {
+                const double cv = tp.VC[a3 + N1 * (b1 + (P + 1) * (b2 + (P + 1) * b3))];
 #pragma unroll
-                    for (int e = 0; e < NC; ++e) s[e] += cv * Z[e * NZ + q * N1 + a3];
-                }
-#pragma unroll
-                for (int e = 0; e < NC; ++e) y[e * NP + off + b3] = s[e];
+                for (int e = 0; e < NC; ++e) s[e] += cv * Z[e * NZ + q * N1 + a3];
             }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) y[e * NP + m] = s[e];
         }
         __syncthreads();
+    }
+
+    // mode number m -> (b1, b2, b3), modes nested b1, b2, b3 (b3 fastest)
+    __device__ static void decode_mode(int m, int& b1, int& b2, int& b3) {
+        b1 = 0;
+        while (b1 < P && m >= (P - b1 + 1) * (P - b1 + 2) / 2) {
+            m -= (P - b1 + 1) * (P - b1 + 2) / 2;
+            ++b1;
+        }
+        b2 = 0;
+        while (b2 < P - b1 && m >= P - b1 - b2 + 1) {
+            m -= P - b1 - b2 + 1;
+            ++b2;
+        }
+        b3 = m;
     }
 };
 
