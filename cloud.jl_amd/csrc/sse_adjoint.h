@@ -58,10 +58,12 @@ __device__ __forceinline__ void inv_logmean_grad(double x, double y, double& dx,
 // (no logarithm evaluated); the branch test is the forward one.
 __device__ __forceinline__ double logmean_vg(double x, double y, double lx, double ly, double ix, double iy,
                                              double& dx, double& dy) {
-    const double f2 = (x * (x - 2.0 * y) + y * y) / (x * (x + 2.0 * y) + y * y);
-    if (f2 < 1.0e-4) {
-        const double r = 1.0 / (x + y);
-        const double q = (x - y) * r;
+    // the forward's branch test f2 = ((x - y) / (x + y))^2 < 1e-4, without the division
+    const double dxy = x - y, sxy = x + y;
+    if (dxy * dxy < 1.0e-4 * (sxy * sxy)) {
+        const double r = 1.0 / sxy;
+        const double q = dxy * r;
+        const double f2 = q * q;
         const double den = 210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0));
         const double dden = 70.0 + f2 * (84.0 + f2 * 90.0);
         const double g = 105.0 / den;
@@ -81,10 +83,11 @@ __device__ __forceinline__ double logmean_vg(double x, double y, double lx, doub
 // inv_logmean(x, y) and its partial derivatives, from x, y and their logarithms
 __device__ __forceinline__ double inv_logmean_vg(double x, double y, double lx, double ly, double ix, double iy,
                                                  double& dx, double& dy) {
-    const double f2 = (x * (x - 2.0 * y) + y * y) / (x * (x + 2.0 * y) + y * y);
-    if (f2 < 1.0e-4) {
-        const double r = 1.0 / (x + y);
-        const double q = (x - y) * r;
+    const double dxy = x - y, sxy = x + y;
+    if (dxy * dxy < 1.0e-4 * (sxy * sxy)) {
+        const double r = 1.0 / sxy;
+        const double q = dxy * r;
+        const double f2 = q * q;
         const double den = 210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0));
         const double dden = 70.0 + f2 * (84.0 + f2 * 90.0);
         const double c = r * (1.0 / 105.0);
@@ -104,9 +107,9 @@ __device__ __forceinline__ double inv_logmean_vg(double x, double y, double lx, 
 template <int D>
 struct EulerAdj {
     static constexpr int NC = D + 2;
-    // hoisted per-node quantities: rho, v (D), p, log rho, beta = rho / p, log beta, 1 / rho
-    static constexpr int NH = D + 6;
-    enum { H_RHO = 0, H_V = 1, H_P = D + 1, H_LR = D + 2, H_B = D + 3, H_LB = D + 4, H_IR = D + 5 };
+    // hoisted per-node quantities: rho, v (D), p, log rho, beta = rho / p, log beta, 1 / rho, 1 / p
+    static constexpr int NH = D + 7;
+    enum { H_RHO = 0, H_V = 1, H_P = D + 1, H_LR = D + 2, H_B = D + 3, H_LB = D + 4, H_IR = D + 5, H_IP = D + 6 };
 
     __device__ static void hoist_full(const LawParams& P, const double* u, double* h) {
         const double ir = 1.0 / u[0];
@@ -117,12 +120,14 @@ struct EulerAdj {
             s += h[H_V + m] * u[m + 1];
         }
         const double p = P.gm1 * (u[D + 1] - 0.5 * s);
+        const double ip = 1.0 / p;
         h[H_RHO] = u[0];
         h[H_P] = p;
         h[H_LR] = log(u[0]);
-        h[H_B] = u[0] / p;
+        h[H_B] = u[0] * ip;
         h[H_LB] = log(h[H_B]);
         h[H_IR] = ir;
+        h[H_IP] = ip;
     }
 
     // ec_dot_adj_side from hoisted states: adds the pull-back of Gb through
@@ -158,7 +163,6 @@ struct EulerAdj {
         const double* hs = left ? hL : hR;
         const double* ho = left ? hR : hL;
         const double po = left ? pR : pL;
-        const double ps = left ? pL : pR;
         const double avo = left ? avR : avL;
         double vb[D];
 #pragma unroll
@@ -166,7 +170,7 @@ struct EulerAdj {
             vb[m] = 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * f + avb * a[m]) + 0.5 * Ccb * ho[H_V + m];
         double pb = 0.5 * Gb[D + 1] * avo + 0.5 * pavgb;
         const double bb = P.inv_gm1 * Ccb * (left ? dbx : dby);
-        const double ips = 1.0 / ps;
+        const double ips = left ? hL[H_IP] : hR[H_IP];
         double rb = rhoavgb * (left ? drx : dry) + bb * ips;
         pb += -bb * hs[H_B] * ips;
         // (rho, v, p) -> (rho, m, E), p = gm1 (E - 1/2 rho |v|^2), v = m / rho

@@ -61,6 +61,12 @@ int struct_occupancy(int elem, int P);
 hipError_t launch_struct(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
                          hipStream_t s, int n_blocks);
 size_t std_tensor_config(int elem, int P);
+size_t vjp_tensor_config(int elem, int P);
+hipError_t launch_vjp_tensor(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
+                             double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
+                             hipStream_t s);
+hipError_t launch_vjp_sum(const double* lam_partial, int n, double* lam_out, double lam_scale, int accumulate,
+                          hipStream_t s);
 hipError_t collapse_metric(double* Lam, const double* c, int NQ, int D, int N_e, hipStream_t s);
 hipError_t launch_std_tensor(const TensorParams& tp, int R, const double* b_member, int tangent, const double* u,
                              double* dudt, const RKStage& rk, hipStream_t s, int which);
@@ -87,6 +93,8 @@ struct sse_handle {
     double* fpartial = nullptr;  // functionals: per-element partial sums and the result
     double* fout = nullptr;
     double *vjp_rbar = nullptr, *vjp_fbar = nullptr, *vjp_lam = nullptr, *vjp_lamout = nullptr;  // VJP scratch
+    double* vjp_fext = nullptr;  // fbar in the neighbour's exterior slot (tensor VJP kernels)
+    bool vjp_tensor = false;
     std::vector<void*> allocations;
     std::string err;
 };
@@ -1001,6 +1009,11 @@ int vjp_check(sse_handle* h) {
                          (rc = alloc(h, (size_t)P.N_f * P.N_c * Ne, &h->vjp_fbar)) || (rc = alloc(h, Ne, &h->vjp_lam)) ||
                          (rc = alloc(h, 1, &h->vjp_lamout))))
         return rc;
+    // compile-time-shaped VJP on the line-structured tables of the forward kernel
+    const char* env = getenv("SSE_VJP");
+    const size_t lds = h->use_struct ? sse::vjp_tensor_config(h->TP.elem, h->TP.P) : 0;
+    h->vjp_tensor = lds > 0 && lds <= 160 * 1024 && !(env && !strcmp(env, "generic"));
+    if (h->vjp_tensor && !h->vjp_fext && (rc = alloc(h, (size_t)P.N_f * P.N_c * Ne, &h->vjp_fext))) return rc;
     return 0;
 }
 
@@ -1009,9 +1022,14 @@ int vjp_device(sse_handle* h, const double* u, const double* ybar, double* ubar,
     int rc = launch(h, u, nullptr, RKStage{}, 1);  // u tilde: the handle's own phase 1
     if (rc) return rc;
     hipError_t e = hipSuccess;
-    if (e == hipSuccess)
+    if (h->vjp_tensor) {
+        e = sse::launch_vjp_tensor(h->TP, h->ST, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_fext, h->vjp_lam,
+                                   h->stream);
+        if (e == hipSuccess) e = sse::launch_vjp_sum(h->vjp_lam, h->P.N_e, h->vjp_lamout, lam_scale, accumulate, h->stream);
+    } else {
         e = sse::launch_vjp(h->P, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_lam, h->vjp_lamout, lam_scale,
                             accumulate, h->stream);
+    }
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return 0;
 }

@@ -497,6 +497,12 @@ hipError_t launch_vjp_t(const DevParams& p, const double* u, const double* ybar,
     return hipGetLastError();
 }
 
+hipError_t launch_vjp_sum(const double* lam_partial, int n, double* lam_out, double lam_scale, int accumulate,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_vjp_sum, dim3(1), dim3(VT), 0, s, lam_partial, n, lam_out, lam_scale, accumulate);
+    return hipGetLastError();
+}
+
 // Phases V1/V2 after the forward nodal_values (u tilde in p.uq / p.uf); the λ
 // cotangent (times lam_scale) is written to, or added into, lam_out.
 hipError_t launch_vjp(const DevParams& p, const double* u, const double* ybar, double* ubar, double* rbar,

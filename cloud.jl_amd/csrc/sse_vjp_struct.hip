@@ -1,0 +1,492 @@
+// sse_vjp_struct.hip -- vector-Jacobian product of the flux-differencing Euler
+// residual for ModalTensor Tri/Tet with compile-time element shapes: the
+// reverse of the line-structured time derivative (sse_struct.hip) and of the
+// entropy projection, same algebra as the general kernels of sse_vjp.hip.
+//
+//   dudt = M^{-1} V' r_q        ->  rbar = V M^{-1} ybar         (phase V1, sum-factorised)
+//   r_q -= R' f_f               ->  fbar = -R rbar
+//   f_f = B J_f f*(u_f, u_f+)   ->  interface flux adjoint at each facet node, for its own
+//                                   flux and for the neighbour's flux at the shared node
+//   Hadamard rows along lines   ->  each volume node gathers (dT_ij/du_i)' (rbar_j - rbar_i)
+//                                   over its line partners j (S_m from the line tables)
+//   facet correction            ->  volume side by C rows, facet side by column chunks
+//   u_q = e2c(w_q'), u_f = e2c(R w_q'), w_q' = V M^{-1} V' (WJ c2e(V u))  reversed.
+// The entropy variables of the projected states are taken as c2e(u_q), c2e(u_f)
+// (the inverse of e2c, exact up to rounding) instead of recomputing the
+// projection. Every sum has a fixed order; no atomics.
+#include <hip/hip_runtime.h>
+
+#include "sse_adjoint.h"
+#include "sse_tensor.h"
+
+namespace sse {
+
+namespace {
+
+// team size and split of a node's partner list (as the forward kernel)
+template <int ELEM, int P>
+struct VjpTune;
+template <int P> struct VjpTune<ELEM_TRI, P> { static constexpr int TS = 64, SPLIT = 1; };
+template <> struct VjpTune<ELEM_TET, 2> { static constexpr int TS = 64, SPLIT = 1; };
+template <> struct VjpTune<ELEM_TET, 3> { static constexpr int TS = 128, SPLIT = 2; };
+template <> struct VjpTune<ELEM_TET, 4> { static constexpr int TS = 256, SPLIT = 2; };
+template <> struct VjpTune<ELEM_TET, 5> { static constexpr int TS = 256, SPLIT = 1; };
+
+template <int ELEM, int P>
+struct VjpLayout {
+    using Sh = Shape<ELEM, P>;
+    using T = VjpTune<ELEM, P>;
+    static constexpr int NC = Sh::NC, D = Sh::D, NH = EulerAdj<Sh::D>::NH;
+    static constexpr int ITEMS = ELEM == ELEM_TET ? 3 * Sh::N1 * Sh::N1 + Sh::N1 * Sh::N1 * Sh::N1 : 3 * Sh::N1;
+    static constexpr int MAXITEM = (ITEMS + T::TS - 1) / T::TS * T::TS;
+    static constexpr int off_h = 0;                          // NH x NQ hoisted volume states
+    static constexpr int off_L = off_h + NH * Sh::NQ;        // D*D x NQ
+    static constexpr int off_hf = off_L + D * D * Sh::NQ;    // NH x NF hoisted facet states
+    static constexpr int off_nJ = off_hf + NH * Sh::NF;      // D x NF
+    static constexpr int off_rb = off_nJ + D * Sh::NF;       // NC x NQ rbar
+    static constexpr int off_fb = off_rb + NC * Sh::NQ;      // NC x NF fbar
+    static constexpr int off_st = off_fb + NC * Sh::NF;      // NC x MAXITEM chunk partials
+    static constexpr int off_ub = off_st + NC * MAXITEM;     // NC x NQ split partial rows, then w_q' cotangent
+    static constexpr int off_wf = off_ub + NC * Sh::NQ;      // NC x NF w_f cotangent
+    static constexpr int off_red = off_wf + NC * Sh::NF;     // TS reduction buffer
+    static constexpr int size = off_red + T::TS;
+    // reverse projection temporaries over the (dead) hoisted states
+    static constexpr int off_z = 0, off_W = off_z + NC * Sh::NZ, off_y = off_W + NC * (Sh::NW > 0 ? Sh::NW : 1);
+    static constexpr int off_x = off_y + NC * Sh::NP, off_u = off_x + NC * Sh::NQ, off_u0 = off_u + NC * Sh::NP;
+    static_assert(off_u0 + NC * Sh::NQ <= off_nJ, "reverse-projection temporaries alias");
+};
+
+// conservative state from the hoisted (rho, v, p, ...) of EulerAdj::hoist_full
+template <int D>
+__device__ __forceinline__ void prim_to_conservative(const LawParams& P, const double* h, double* u) {
+    using A = EulerAdj<D>;
+    const double rho = h[A::H_RHO];
+    double s = 0.0;
+    u[0] = rho;
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        u[m + 1] = rho * h[A::H_V + m];
+        s += h[A::H_V + m] * u[m + 1];
+    }
+    u[D + 1] = h[A::H_P] * P.inv_gm1 + 0.5 * s;
+}
+
+}  // namespace
+
+// V1: rbar = V M^{-1} ybar (M^{-1} = V' (W / J) V, weight-adjusted), fbar = -R rbar,
+// fbar also into the neighbour's exterior slot [k][e][j] (mapP is an involution)
+template <int ELEM, int P>
+__global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
+k_vjp_v1_tensor(TensorParams tp, const double* __restrict__ ybar, double* __restrict__ rbar,
+                double* __restrict__ fbar, double* __restrict__ fbar_ext) {
+    using Sh = Shape<ELEM, P>;
+    constexpr int TS = VjpTune<ELEM, P>::TS, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF;
+    using SF = SumFact<ELEM, P, TS>;
+    extern __shared__ double sm[];
+    const DevParams& p = tp.base;
+    const int k = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap), tl = threadIdx.x;
+    double* sy = sm;                 // NC x NP
+    double* sx = sy + NC * NP;       // NC x NQ
+    double* sZ = sx + NC * NQ;       // NC x NZ
+    double* sW = sZ + NC * Sh::NZ;   // NC x NW
+    const double* yk = ybar + (size_t)k * NP * NC;
+    for (int t = tl; t < NP * NC; t += TS) sy[t] = yk[t];
+    __syncthreads();
+    SF::V(tp, tl, sy, sx, sZ, sW);
+    const double* Jk = p.Jq + (size_t)k * NQ;
+    for (int i = tl; i < NQ; i += TS) {
+        const double jw = p.W[i] / Jk[i];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
+    }
+    __syncthreads();
+    SF::Vt(tp, tl, sx, sy, sZ, sW);
+    SF::V(tp, tl, sy, sx, sZ, sW);
+    double* rk = rbar + (size_t)k * NQ * NC;
+    for (int t = tl; t < NQ * NC; t += TS) rk[t] = sx[t];
+    for (int j = tl; j < NF; j += TS) {
+        double acc[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) acc[e] = 0.0;
+        for (int w = 0; w < tp.wr; ++w) {
+            const int i = tp.r_idx[w * NF + j];
+            if (i < 0) break;
+            const double v = tp.r_val[w * NF + j];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) acc[e] += v * sx[e * NQ + i];
+        }
+        const int g = p.mapP[j + (size_t)NF * k];
+        const int kn = g / NF, jn = g - kn * NF;
+        double* fk = fbar + (size_t)k * NF * NC;
+        double* fx = fbar_ext + (size_t)kn * NF * NC;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            fk[e * NF + j] = -acc[e];
+            fx[e * NF + jn] = -acc[e];
+        }
+    }
+}
+
+// V2: everything else, one element per workgroup
+template <int ELEM, int P>
+__global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
+k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, const double* __restrict__ rbar,
+                const double* __restrict__ fbar, const double* __restrict__ fbar_ext, double* __restrict__ ubar,
+                double* __restrict__ lam_partial) {
+    using Sh = Shape<ELEM, P>;
+    using T = VjpTune<ELEM, P>;
+    using L = VjpLayout<ELEM, P>;
+    using A = EulerAdj<Sh::D>;
+    using Ph = Physics<Sh::D, LAW_EULER>;
+    using SF = SumFact<ELEM, P, T::TS>;
+    constexpr int D = Sh::D, NC = Sh::NC, NH = L::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
+    constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
+    constexpr int NPART = D * (N1 - 1), QH = (NPART + SPLIT - 1) / SPLIT;
+    constexpr int MAXFI = 8;
+    static_assert(NQ <= NT && NF <= NT, "one node per thread");
+    extern __shared__ double sm[];
+    const DevParams& p = tp.base;
+    const int k = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap);
+    const int tt = threadIdx.x, half = tt / NT, tl = tt - half * NT;
+    double* sh = sm + L::off_h;
+    double* sL = sm + L::off_L;
+    double* shf = sm + L::off_hf;
+    double* snJ = sm + L::off_nJ;
+    double* srb = sm + L::off_rb;
+    double* sfb = sm + L::off_fb;
+    double* sst = sm + L::off_st;
+    double* sub = sm + L::off_ub;
+    double* swf = sm + L::off_wf;
+    double* red = sm + L::off_red;
+
+    // ---- phase A: loads, hoisting, interface flux adjoints
+    double hi[NH], Li[D * D], ubq[NC];
+    double lam = 0.0;
+#pragma unroll
+    for (int e = 0; e < NC; ++e) ubq[e] = 0.0;
+    if (half == 0 && tl < NQ) {
+        const double* uqk = p.uq + (size_t)k * NQ * NC;
+        const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+        const double* rk = rbar + (size_t)k * NQ * NC;
+        double uq[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            uq[e] = uqk[e * NQ + tl];
+            srb[e * NQ + tl] = rk[e * NQ + tl];
+        }
+        A::hoist_full(p.law, uq, hi);
+#pragma unroll
+        for (int v = 0; v < NH; ++v) sh[v * NQ + tl] = hi[v];
+#pragma unroll
+        for (int mn = 0; mn < D * D; ++mn) {
+            Li[mn] = Lk[mn * NQ + tl];
+            sL[mn * NQ + tl] = Li[mn];
+        }
+    }
+    if (half == 0 && tl < NF) {
+        const double* ufk = p.uf + (size_t)k * NF * NC;
+        const double* uxk = st.uf_ext + (size_t)k * NF * NC;
+        const double* fk = fbar + (size_t)k * NF * NC;
+        const double* fxk = fbar_ext + (size_t)k * NF * NC;
+        double uf[NC], ux[NC], ubf[NC], hin[NH], hout[NH], fo[NC], nrm[D], gb[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ubf[e] = 0.0;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            uf[e] = ufk[e * NF + tl];
+            ux[e] = uxk[e * NF + tl];
+            const double f = fk[e * NF + tl];
+            sfb[e * NF + tl] = f;
+            fo[e] = fxk[e * NF + tl];
+        }
+        A::hoist_full(p.law, uf, hin);
+        A::hoist_full(p.law, ux, hout);
+#pragma unroll
+        for (int v = 0; v < NH; ++v) shf[v * NF + tl] = hin[v];
+        const double jf = p.Jf[tl + (size_t)NF * k];
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            const double nj = p.nJf[m + D * (tl + (size_t)NF * k)];
+            snJ[m * NF + tl] = nj;
+            nrm[m] = nj / jf;
+        }
+        const bool lf = (p.law.inviscid_flux == 1);
+        // this node's flux f*(u_in, u_out, n), cotangent B J_f fbar (keep u_in)
+        const double bj = p.B[tl] * jf;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) gb[e] = bj * sfb[e * NF + tl];
+        A::ec_dot_adj_h(p.law, hin, hout, nrm, gb, true, ubf);
+        if (lf) lam += 0.5 * A::lf_diss_adj(p.law, uf, ux, nrm, gb, true, ubf);  // halfλ = λ / 2
+        // the neighbour's flux at the shared node, f*(u_out, u_in, n'), cotangent
+        // B' J_f' fbar' (keep u_in, its exterior state)
+        const int g = p.mapP[tl + (size_t)NF * k];
+        const int kn = g / NF, jn = g - kn * NF;
+        const double jfn = p.Jf[jn + (size_t)NF * kn];
+#pragma unroll
+        for (int m = 0; m < D; ++m) nrm[m] = p.nJf[m + D * (jn + (size_t)NF * kn)] / jfn;
+        const double bjn = p.B[jn] * jfn;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) gb[e] = bjn * fo[e];
+        A::ec_dot_adj_h(p.law, hout, hin, nrm, gb, false, ubf);
+        if (lf) A::lf_diss_adj(p.law, ux, uf, nrm, gb, false, ubf);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) swf[e * NF + tl] = ubf[e];  // interface part of the u_f cotangent
+    }
+    __syncthreads();
+
+    // ---- phase B: Hadamard row (line partners) and facet-correction row of node tl
+    if (SPLIT > 1 && half > 0 && tl < NQ) {
+#pragma unroll
+        for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
+#pragma unroll
+        for (int mn = 0; mn < D * D; ++mn) Li[mn] = sL[mn * NQ + tl];
+    }
+    if (tl < NQ) {
+        double hj[NH], av[D], gb[NC], rbi[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) rbi[e] = srb[e * NQ + tl];
+        const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
+        const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
+        int f = q0 / (N1 - 1), dlt = q0 - f * (N1 - 1) + 1;
+#pragma unroll 1
+        for (int q = q0; q < q1; ++q) {
+            const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
+            const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
+            const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+            const int j = tl + (b - af) * stride;
+#pragma unroll
+            for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+#pragma unroll
+            for (int n = 0; n < D; ++n) av[n] = 0.0;
+            for (int m = f; m < D; ++m) {
+                const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+#pragma unroll
+                for (int n = 0; n < D; ++n) av[n] += s * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) gb[e] = srb[e * NQ + j] - rbi[e];
+            A::ec_dot_adj_h(p.law, hi, hj, av, gb, true, ubq);
+            if (++dlt == N1) {
+                dlt = 1;
+                ++f;
+            }
+        }
+        const int wh = (st.wcr + SPLIT - 1) / SPLIT;
+#pragma unroll 1
+        for (int w = half * wh; w < (half + 1) * wh && w < st.wcr; ++w) {
+            const int j = st.cr_j[w * NQ + tl];
+            if (j < 0) continue;
+            const double crv = st.cr_v[w * NQ + tl];
+            const int fc = j / NPF;
+#pragma unroll
+            for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+#pragma unroll
+            for (int n = 0; n < D; ++n) {
+                double s = snJ[n * NF + j];
+#pragma unroll
+                for (int l = 0; l < D; ++l) s += Li[l + D * n] * p.fn[l + D * fc];
+                av[n] = 0.5 * crv * s;
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) gb[e] = -(rbi[e] + sfb[e * NF + j]);
+            A::ec_dot_adj_h(p.law, hi, hj, av, gb, true, ubq);
+        }
+    }
+    // ---- phase C: facet-correction column chunks (facet side)
+#pragma unroll 1
+    for (int it = 0; it < MAXITEM / TS; ++it) {
+        const int item = it * TS + tt;
+        const int j = item < st.n_items ? st.ci_j[item] : -1;
+        if (j < 0) continue;
+        double hj[NH], hv[NH], av[D], gb[NC], part[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) part[e] = 0.0;
+#pragma unroll
+        for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+        const int fc = j / NPF;
+#pragma unroll 1
+        for (int w = 0; w < st.ch; ++w) {
+            const int i = st.ci_i[w * MAXITEM + item];
+            if (i < 0) continue;
+            const double cv = st.ci_v[w * MAXITEM + item];
+#pragma unroll
+            for (int v = 0; v < NH; ++v) hv[v] = sh[v * NQ + i];
+#pragma unroll
+            for (int n = 0; n < D; ++n) {
+                double s = snJ[n * NF + j];
+#pragma unroll
+                for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * p.fn[l + D * fc];
+                av[n] = 0.5 * cv * s;
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) gb[e] = -(srb[e * NQ + i] + sfb[e * NF + j]);
+            A::ec_dot_adj_h(p.law, hv, hj, av, gb, false, part);
+        }
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sst[e * MAXITEM + item] = part[e];
+    }
+    if (SPLIT > 1 && half == 1 && tl < NQ) {
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sub[e * NQ + tl] = ubq[e];
+    }
+    __syncthreads();
+    // ---- phase D: cotangents of the entropy variables, w = c2e(u): volume and facet
+    if (half == 0 && tl < NQ) {
+        if (SPLIT > 1) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ubq[e] += sub[e * NQ + tl];
+        }
+    }
+    if (half == 0 && tl < NF) {
+        double ubf[NC], uf[NC], hf[NH];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl];
+#pragma unroll
+        for (int w = 0; w < MAXFI; ++w) {
+            const int item = w < st.wfi ? st.fi[w * NF + tl] : -1;
+            if (item < 0) continue;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ubf[e] += sst[e * MAXITEM + item];
+        }
+#pragma unroll
+        for (int v = 0; v < NH; ++v) hf[v] = shf[v * NF + tl];
+        prim_to_conservative<D>(p.law, hf, uf);
+        double w[NC], wb[NC];
+        Ph::c2e(p.law, uf, w);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) wb[e] = 0.0;
+        A::e2c_adj(p.law, w, ubf, wb);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) swf[e * NF + tl] = wb[e];
+    }
+    __syncthreads();  // the w_f cotangents are complete
+    if (half == 0 && tl < NQ) {
+        double w[NC], wb[NC], uq[NC];
+#pragma unroll
+        for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
+        prim_to_conservative<D>(p.law, hi, uq);
+        Ph::c2e(p.law, uq, w);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) wb[e] = 0.0;
+        A::e2c_adj(p.law, w, ubq, wb);
+        // w_q' cotangent += R' (w_f cotangent)
+        for (int ww = 0; ww < tp.wrt; ++ww) {
+            const int jj = tp.rt_idx[ww * NQ + tl];
+            if (jj < 0) break;
+            const double rv = tp.rt_val[ww * NQ + tl];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) wb[e] += rv * swf[e * NF + jj];
+        }
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sub[e * NQ + tl] = wb[e];
+    }
+    // ---- phase E: w_q' = V M^{-1} V' (WJ c2e(V u)) reversed
+    double* sZ = sm + L::off_z;
+    double* sW = sm + L::off_W;
+    double* sy = sm + L::off_y;
+    double* sx = sm + L::off_x;
+    double* su = sm + L::off_u;
+    double* su0 = sm + L::off_u0;
+    const double* uk = u + (size_t)k * NP * NC;
+    for (int t = tt; t < NP * NC; t += TS) su[t] = uk[t];
+    __syncthreads();
+    SF::V(tp, tt, su, su0, sZ, sW);    // V u
+    SF::Vt(tp, tt, sub, sy, sZ, sW);   // V' (w_q' cotangent)
+    SF::V(tp, tt, sy, sx, sZ, sW);     // M^{-1} = V' (W/J V .)
+    const double* Jk = p.Jq + (size_t)k * NQ;
+    for (int i = tt; i < NQ; i += TS) {
+        const double jw = p.W[i] / Jk[i];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
+    }
+    __syncthreads();
+    SF::Vt(tp, tt, sx, sy, sZ, sW);
+    SF::V(tp, tt, sy, sx, sZ, sW);     // V (M^{-1} V' w_q'bar), then times WJ and through c2e
+    for (int i = tt; i < NQ; i += TS) {
+        const double wj = p.W[i] * Jk[i];
+        double u0[NC], wb[NC], ub[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            u0[e] = su0[e * NQ + i];
+            wb[e] = wj * sx[e * NQ + i];
+            ub[e] = 0.0;
+        }
+        A::c2e_adj(p.law, u0, wb, ub);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) sub[e * NQ + i] = ub[e];
+    }
+    __syncthreads();
+    SF::Vt(tp, tt, sub, sy, sZ, sW);
+    double* outk = ubar + (size_t)k * NP * NC;
+    for (int t = tt; t < NP * NC; t += TS) outk[t] = sy[t];
+    // λ cotangent of this element, summed in a fixed order
+    red[tt] = lam;
+    __syncthreads();
+    for (int s = TS / 2; s > 0; s >>= 1) {
+        if (tt < s) red[tt] += red[tt + s];
+        __syncthreads();
+    }
+    if (tt == 0) lam_partial[k] = red[0];
+}
+
+// ----------------------------------------------------------------------------- launch
+template <int ELEM, int P>
+size_t vjp_tensor_lds(int which) {
+    using Sh = Shape<ELEM, P>;
+    if (which == 1) return sizeof(double) * Sh::NC * (Sh::NP + Sh::NQ + Sh::NZ + (Sh::NW > 0 ? Sh::NW : 1));
+    return sizeof(double) * VjpLayout<ELEM, P>::size;
+}
+
+template <int ELEM, int P>
+hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
+                               double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
+                               hipStream_t s) {
+    using T = VjpTune<ELEM, P>;
+    static bool attr = false;
+    const size_t l1 = vjp_tensor_lds<ELEM, P>(1), l2 = vjp_tensor_lds<ELEM, P>(2);
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_vjp_v1_tensor<ELEM, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_vjp_v2_tensor<ELEM, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)l2);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    const int nb = tp.base.N_e;
+    hipLaunchKernelGGL((k_vjp_v1_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l1, s, tp, ybar, rbar, fbar, fbar_ext);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l2, s, tp, st, u, rbar, fbar, fbar_ext, ubar,
+                       lam_partial);
+    return hipGetLastError();
+}
+
+#define SSE_VJP_CASES                                                                                      \
+    SSE_CASE(ELEM_TRI, 2) SSE_CASE(ELEM_TRI, 3) SSE_CASE(ELEM_TRI, 4) SSE_CASE(ELEM_TRI, 5) SSE_CASE(ELEM_TRI, 6) \
+    SSE_CASE(ELEM_TET, 2) SSE_CASE(ELEM_TET, 3) SSE_CASE(ELEM_TET, 4) SSE_CASE(ELEM_TET, 5)
+
+// LDS of the larger phase, or 0 when (elem, P) has no compiled kernels
+size_t vjp_tensor_config(int elem, int P) {
+#define SSE_CASE(E_, PP)                                                                         \
+    if (elem == E_ && P == PP) {                                                                 \
+        const size_t a = vjp_tensor_lds<E_, PP>(1), b = vjp_tensor_lds<E_, PP>(2);               \
+        return a > b ? a : b;                                                                    \
+    }
+    SSE_VJP_CASES
+#undef SSE_CASE
+    return 0;
+}
+
+hipError_t launch_vjp_tensor(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
+                             double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
+                             hipStream_t s) {
+#define SSE_CASE(E_, PP)                                                                                   \
+    if (tp.elem == E_ && tp.P == PP)                                                                       \
+        return launch_vjp_tensor_t<E_, PP>(tp, st, u, ybar, ubar, rbar, fbar, fbar_ext, lam_partial, s);
+    SSE_VJP_CASES
+#undef SSE_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sse

@@ -419,6 +419,8 @@ def _ec_interface(mk):
 
 VJP_CASES = {
     "wave3d_tet_p3_lf": lambda: density_wave_3d_config(M=2, p=3, ic_strength=0.2)[4],
+    "wave3d_tet_p4_lf": lambda: density_wave_3d_config(M=2, p=4, ic_strength=0.2)[4],
+    "vortex2d_tri_p2_lf": lambda: vortex_2d_config(M=3, p=2)[4],
     "wave3d_tet_p3_ec": lambda: _ec_interface(lambda f: density_wave_3d_config(M=2, p=3, ic_strength=0.2, form=f))[4],
     "vortex2d_tri_p4_lf": lambda: vortex_2d_config(M=3, p=4)[4],
     "euler1d_gauss_nodal_ec": lambda: euler_1d_gauss()[4],
@@ -446,6 +448,22 @@ def test_residual_vjp_dot_product_against_oracle_jvp(name):
     from sse.time_integration import to_device, to_host
     ub_dev, lam_dev = residual_vjp(ode.p, to_device(u), to_device(ybar))
     assert _rel(to_host(ub_dev), ubar) <= 1e-14 and abs(lam_dev - lam_bar) <= 1e-12 * (1 + abs(lam_bar))
+
+
+@pytest.mark.parametrize("name", ["wave3d_tet_p3_lf", "wave3d_tet_p4_lf", "vortex2d_tri_p4_lf"])
+def test_tensor_vjp_equals_general_vjp(name, monkeypatch):
+    # the compile-time-shaped VJP kernels against the general ones, entry by entry
+    from sse.adjoint import residual_vjp
+    ode = VJP_CASES[name]()
+    u = _perturbed(ode, amp=1e-3)
+    ybar = np.asfortranarray(np.random.default_rng(5).standard_normal(u.shape))
+    fast = ode.p.with_law(ode.p.conservation_law)
+    ub_fast, lam_fast = residual_vjp(fast, u, ybar)
+    monkeypatch.setenv("SSE_VJP", "generic")
+    general = ode.p.with_law(ode.p.conservation_law)
+    ub_gen, lam_gen = residual_vjp(general, u, ybar)
+    assert _rel(ub_fast, ub_gen) <= 1e-12, _rel(ub_fast, ub_gen)
+    assert abs(lam_fast - lam_gen) <= 1e-12 * abs(lam_gen), (lam_fast, lam_gen)
 
 
 def test_rk_adjoint_matches_oracle_forward_mode():
