@@ -186,7 +186,9 @@ def bench_adjoint(args):
         errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
     fd_lam = (J_of(u0, h) - J_of(u0, -h)) / (2 * h)
     lam_err = abs(fd_lam - glam) / max(abs(glam), 1e-300)
-    log(f"FD check: max rel err over {len(errs)} directions {max(errs):.2e}; dJ/dλ {glam:.6e} vs FD {fd_lam:.6e}")
+    fd_max = max(errs) if errs else None
+    fd_txt = f"{fd_max:.2e}" if errs else "n/a"
+    log(f"FD check: max rel err over {len(errs)} directions {fd_txt}; dJ/dλ {glam:.6e} vs FD {fd_lam:.6e}")
     line = {
         "metric": "adjoint/forward cost (3D Euler p=3 curved tets, 200 RK steps, entropy objective)",
         "value": (t_adj - t_fwd) / t_fwd, "unit": "x", "n_gpus": 1, "steps": nsteps, "warmup": 2,
@@ -198,7 +200,7 @@ def bench_adjoint(args):
                    "t_forward_s": t_fwd, "t_adjoint_total_s": t_adj,
                    "adjoint_including_forward_sweep_over_forward": t_adj / t_fwd,
                    "J": J, "dJ_dlambda": glam,
-                   "fd_max_rel_err": max(errs), "fd_directions": len(errs), "fd_lambda_rel_err": lam_err},
+                   "fd_max_rel_err": fd_max, "fd_directions": len(errs), "fd_lambda_rel_err": lam_err},
     }
     print(json.dumps(line), flush=True)
 
