@@ -460,6 +460,26 @@ struct EulerAdj {
         ub[0] += rb;
     }
 
+    // u = e2c(w): ub -> wb (added), from the state's rho, v, p: du/dw is the symmetric
+    // matrix A0 of the entropy variables (Hughes, Franca & Mallet, Comput. Methods Appl.
+    // Mech. Engrg. 54, 1986), so wb += A0 ub with no logarithm or power.
+    __device__ static void e2c_adj_prim(const LawParams& P, double rho, const double* v, double p,
+                                        const double* ub, double* wb) {
+        double vv = 0.0, vu = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            vv += v[m] * v[m];
+            vu += v[m] * ub[m + 1];
+        }
+        const double E = p * P.inv_gm1 + 0.5 * rho * vv;
+        const double rH = E + p;  // rho H
+        const double c = rho * ub[0] + rho * vu + rH * ub[D + 1];
+        wb[0] += rho * ub[0] + rho * vu + E * ub[D + 1];
+#pragma unroll
+        for (int m = 0; m < D; ++m) wb[m + 1] += v[m] * c + p * ub[m + 1];
+        wb[D + 1] += E * ub[0] + rH * vu + (rH * rH / rho - P.gamma * p * p / rho * P.inv_gm1) * ub[D + 1];
+    }
+
     // u = e2c(w): ub -> wb (added)
     __device__ static void e2c_adj(const LawParams& P, const double* win, const double* ub, double* wb) {
         double w[NC];

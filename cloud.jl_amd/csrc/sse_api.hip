@@ -64,7 +64,7 @@ size_t std_tensor_config(int elem, int P);
 size_t vjp_tensor_config(int elem, int P);
 hipError_t launch_vjp_tensor(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
                              double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
-                             hipStream_t s);
+                             double scale, int accumulate, hipStream_t s);
 hipError_t launch_vjp_sum(const double* lam_partial, int n, double* lam_out, double lam_scale, int accumulate,
                           hipStream_t s);
 hipError_t collapse_metric(double* Lam, const double* c, int NQ, int D, int N_e, hipStream_t s);
@@ -1017,16 +1017,63 @@ int vjp_check(sse_handle* h) {
     return 0;
 }
 
+// Phase-1 outputs of the tensor kernels (projected volume and facet states and the
+// exterior facet states): the handle's own buffers, or per-stage copies the RK
+// adjoint keeps from its recomputation.
+struct Phase1Bufs {
+    double* uq;
+    double* uf;
+    double* ufx;
+};
+
+void with_phase1(const sse_handle* h, const Phase1Bufs& b, sse::TensorParams& tp, sse::StructTables& st) {
+    tp = h->TP;
+    st = h->ST;
+    tp.base.uq = b.uq;
+    tp.base.uf = b.uf;
+    tp.uf_ext = b.ufx;
+    st.uf_ext = b.ufx;
+}
+
+// One RK stage of the line-structured kernels with phase 1 written to b
+int launch_struct_into(sse_handle* h, double* u, const RKStage& rk, const Phase1Bufs& b) {
+    sse::TensorParams tp;
+    sse::StructTables st;
+    with_phase1(h, b, tp, st);
+    hipError_t e = sse::launch_tensor(tp, u, nullptr, rk, h->stream, 1);
+    if (e == hipSuccess) e = sse::launch_struct(tp, st, nullptr, rk, h->stream, h->struct_blocks);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// Tensor VJP: ubar (+)= scale (dR/du)' ybar, the element λ partials (+)= scale (dR/dλ)' ybar.
+// With `stored`, phase 1 of u already sits in those buffers; otherwise it is run here.
+int vjp_tensor_device(sse_handle* h, const double* u, const double* ybar, double* ubar, double scale, int accumulate,
+                      const Phase1Bufs* stored) {
+    Phase1Bufs own{h->TP.base.uq, h->TP.base.uf, h->TP.uf_ext};
+    if (!stored) {
+        int rc = launch(h, u, nullptr, RKStage{}, 1);
+        if (rc) return rc;
+    }
+    sse::TensorParams tp;
+    sse::StructTables st;
+    with_phase1(h, stored ? *stored : own, tp, st);
+    hipError_t e = sse::launch_vjp_tensor(tp, st, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_fext, h->vjp_lam,
+                                          scale, accumulate, h->stream);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
 // ubar = (dR/du)' ybar on device buffers; lam_out (device) gets or accumulates scale * (dR/dλ)' ybar
 int vjp_device(sse_handle* h, const double* u, const double* ybar, double* ubar, double lam_scale, int accumulate) {
-    int rc = launch(h, u, nullptr, RKStage{}, 1);  // u tilde: the handle's own phase 1
-    if (rc) return rc;
     hipError_t e = hipSuccess;
     if (h->vjp_tensor) {
-        e = sse::launch_vjp_tensor(h->TP, h->ST, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_fext, h->vjp_lam,
-                                   h->stream);
-        if (e == hipSuccess) e = sse::launch_vjp_sum(h->vjp_lam, h->P.N_e, h->vjp_lamout, lam_scale, accumulate, h->stream);
+        int rc = vjp_tensor_device(h, u, ybar, ubar, 1.0, 0, nullptr);
+        if (rc) return rc;
+        e = sse::launch_vjp_sum(h->vjp_lam, h->P.N_e, h->vjp_lamout, lam_scale, accumulate, h->stream);
     } else {
+        int rc = launch(h, u, nullptr, RKStage{}, 1);  // u tilde: the handle's own phase 1
+        if (rc) return rc;
         e = sse::launch_vjp(h->P, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_lam, h->vjp_lamout, lam_scale,
                             accumulate, h->stream);
     }
@@ -1091,13 +1138,27 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         for (void* q : mine) (void)hipFree(q);
     };
     double *u = nullptr, *ktmp = nullptr, *ub = nullptr, *kb = nullptr, *vb = nullptr, *ckpt = nullptr,
-           *stages = nullptr;
+           *stages = nullptr, *p1 = nullptr;
     if ((rc = dalloc(n, &u)) || (rc = dalloc(n, &ktmp)) || (rc = dalloc(n, &ub)) || (rc = dalloc(n, &kb)) ||
         (rc = dalloc(n, &vb)) || (rc = dalloc(n * (size_t)(nseg > 0 ? nseg : 1), &ckpt)) ||
         (rc = dalloc(n * 5 * (size_t)(seglen > 0 ? seglen : 1), &stages))) {
         release();
         return rc;
     }
+    // tensor VJP: keep each recomputed stage's phase-1 outputs for its VJP (when they fit
+    // in 32 GiB), so that the reverse sweep does not run phase 1 again
+    const bool fast = h->vjp_tensor;
+    const size_t p1q = (size_t)h->P.N_q * h->P.N_c * h->P.N_e, p1f = (size_t)h->P.N_f * h->P.N_c * h->P.N_e;
+    const size_t p1_stage = p1q + 2 * p1f, n_stages = 5 * (size_t)(seglen > 0 ? seglen : 1);
+    const bool keep_p1 = fast && p1_stage * n_stages * sizeof(double) <= ((size_t)32 << 30);
+    if (keep_p1 && (rc = dalloc(p1_stage * n_stages, &p1))) {
+        release();
+        return rc;
+    }
+    auto p1_of = [&](int idx) {
+        double* b = p1 + p1_stage * (size_t)idx;
+        return Phase1Bufs{b, b + p1q, b + p1q + p1f};
+    };
     auto hip_step = [&](hipError_t e, const char* what) {
         return e == hipSuccess ? 0 : fail(h, SSE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     };
@@ -1128,6 +1189,7 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
     ADJ_TRY(hip_step(hipStreamSynchronize(h->stream), "sync"));
     ADJ_TRY(hip_step(sse::launch_entropy_grad(h->P, u, ub, h->stream), "entropy gradient"));
     ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lamout, 0, sizeof(double), h->stream), "zero"));
+    if (fast) ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lam, 0, h->P.N_e * sizeof(double), h->stream), "zero"));
     // reverse sweep, one segment at a time
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int first = seg * checkpoint_every;
@@ -1136,7 +1198,13 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
                                         h->stream), "restore"));
         for (int step = first; step < last; ++step)
             for (int s = 0; s < 5; ++s) {
-                double* keep = stages + n * (size_t)(5 * (step - first) + s);
+                const int idx = 5 * (step - first) + s;
+                double* keep = stages + n * (size_t)idx;
+                if (keep_p1) {  // the stage kernel also stores the stage state
+                    RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr, keep};
+                    ADJ_TRY(launch_struct_into(h, u, rk, p1_of(idx)));
+                    continue;
+                }
                 ADJ_TRY(hip_step(hipMemcpyAsync(keep, u, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
                                  "stage state"));
                 RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
@@ -1144,6 +1212,19 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
             }
         // adjoint of the 2N stages: kbar += B_s ubar; ubar += dt R'(u_s)' kbar; kbar *= A_s
         for (int step = last - 1; step >= first; --step) {
+            if (fast) {
+                // kbar = B_4 ubar; per stage ubar += dt R'(u_s)' kbar (written by the VJP kernel),
+                // then kbar = A_s kbar + B_{s-1} ubar
+                ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[4], ub, 0.0, kb, h->stream), "axpby"));
+                for (int s = 4; s >= 0; --s) {
+                    const int idx = 5 * (step - first) + s;
+                    const double* us = stages + n * (size_t)idx;
+                    const Phase1Bufs b = keep_p1 ? p1_of(idx) : Phase1Bufs{nullptr, nullptr, nullptr};
+                    ADJ_TRY(vjp_tensor_device(h, us, kb, ub, dt, 1, keep_p1 ? &b : nullptr));
+                    if (s > 0) ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[s - 1], ub, CK_A[s], kb, h->stream), "axpby"));
+                }
+                continue;
+            }
             ADJ_TRY(hip_step(hipMemsetAsync(kb, 0, n * sizeof(double), h->stream), "zero"));
             for (int s = 4; s >= 0; --s) {
                 const double* us = stages + n * (size_t)(5 * (step - first) + s);
@@ -1154,6 +1235,7 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
             }
         }
     }
+    if (fast) ADJ_TRY(hip_step(sse::launch_vjp_sum(h->vjp_lam, h->P.N_e, h->vjp_lamout, 1.0, 0, h->stream), "λ sum"));
     ADJ_TRY(hip_step(hipMemcpyAsync(grad_u0, ub, n * sizeof(double),
                                     where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream),
                      "copy gradient"));

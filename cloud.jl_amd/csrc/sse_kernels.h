@@ -90,6 +90,7 @@ struct RKStage {
     double* tmp;    // low-storage register, (N_p, N_c, N_e)
     double* u;      // state updated in place
     const double* dt_e;  // per-element time step (batched independent problems), or null for dt
+    double* keep;        // line-structured kernel only: receives the state before the update, or null
     __device__ __forceinline__ double dt_of(size_t k) const { return dt_e ? dt_e[k] : dt; }
 };
 

@@ -317,6 +317,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             for (int t = tt; t < NP * NC; t += TS) {
                 const double kt = (rk.A == 0.0 ? 0.0 : rk.A * rk.tmp[base + t]) + rk.dt_of(k) * sy[t];
                 rk.tmp[base + t] = kt;
+                if (rk.keep) rk.keep[base + t] = rk.u[base + t];
                 rk.u[base + t] += rk.B * kt;
             }
         }

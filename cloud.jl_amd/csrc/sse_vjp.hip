@@ -519,7 +519,7 @@ hipError_t launch_vjp(const DevParams& p, const double* u, const double* ybar, d
 // y = a x + b y over n entries
 __global__ void k_axpby(size_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = a * x[i] + b * y[i];
+    if (i < n) y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];  // b = 0 does not read y
 }
 
 hipError_t launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t s) {

@@ -11,9 +11,9 @@
 //                                   over its line partners j (S_m from the line tables)
 //   facet correction            ->  volume side by C rows, facet side by column chunks
 //   u_q = e2c(w_q'), u_f = e2c(R w_q'), w_q' = V M^{-1} V' (WJ c2e(V u))  reversed.
-// The entropy variables of the projected states are taken as c2e(u_q), c2e(u_f)
-// (the inverse of e2c, exact up to rounding) instead of recomputing the
-// projection. Every sum has a fixed order; no atomics.
+// The pull-back through u = e2c(w) at the projected states uses du/dw in closed
+// form from the states' (rho, v, p), so the projection is not recomputed. Every
+// sum has a fixed order; no atomics.
 #include <hip/hip_runtime.h>
 
 #include "sse_adjoint.h"
@@ -56,22 +56,14 @@ struct VjpLayout {
     static_assert(off_u0 + NC * Sh::NQ <= off_nJ, "reverse-projection temporaries alias");
 };
 
-// conservative state from the hoisted (rho, v, p, ...) of EulerAdj::hoist_full
-template <int D>
-__device__ __forceinline__ void prim_to_conservative(const LawParams& P, const double* h, double* u) {
-    using A = EulerAdj<D>;
-    const double rho = h[A::H_RHO];
-    double s = 0.0;
-    u[0] = rho;
-#pragma unroll
-    for (int m = 0; m < D; ++m) {
-        u[m + 1] = rho * h[A::H_V + m];
-        s += h[A::H_V + m] * u[m + 1];
-    }
-    u[D + 1] = h[A::H_P] * P.inv_gm1 + 0.5 * s;
-}
-
 }  // namespace
+
+// Output mode of phase V2: ubar (and the element's λ cotangent) = scale * VJP,
+// or += scale * VJP when accumulating (the RK adjoint's ubar += dt R'(u_s)' kbar).
+struct VjpOut {
+    double scale;
+    int accumulate;
+};
 
 // V1: rbar = V M^{-1} ybar (M^{-1} = V' (W / J) V, weight-adjusted), fbar = -R rbar,
 // fbar also into the neighbour's exterior slot [k][e][j] (mapP is an involution)
@@ -132,12 +124,11 @@ template <int ELEM, int P>
 __global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
 k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, const double* __restrict__ rbar,
                 const double* __restrict__ fbar, const double* __restrict__ fbar_ext, double* __restrict__ ubar,
-                double* __restrict__ lam_partial) {
+                double* __restrict__ lam_partial, VjpOut out) {
     using Sh = Shape<ELEM, P>;
     using T = VjpTune<ELEM, P>;
     using L = VjpLayout<ELEM, P>;
     using A = EulerAdj<Sh::D>;
-    using Ph = Physics<Sh::D, LAW_EULER>;
     using SF = SumFact<ELEM, P, T::TS>;
     constexpr int D = Sh::D, NC = Sh::NC, NH = L::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
     constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
@@ -338,7 +329,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
         }
     }
     if (half == 0 && tl < NF) {
-        double ubf[NC], uf[NC], hf[NH];
+        double ubf[NC], hf[NH];
 #pragma unroll
         for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl];
 #pragma unroll
@@ -350,25 +341,21 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
         }
 #pragma unroll
         for (int v = 0; v < NH; ++v) hf[v] = shf[v * NF + tl];
-        prim_to_conservative<D>(p.law, hf, uf);
-        double w[NC], wb[NC];
-        Ph::c2e(p.law, uf, w);
+        double wb[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) wb[e] = 0.0;
-        A::e2c_adj(p.law, w, ubf, wb);
+        A::e2c_adj_prim(p.law, hf[A::H_RHO], hf + A::H_V, hf[A::H_P], ubf, wb);
 #pragma unroll
         for (int e = 0; e < NC; ++e) swf[e * NF + tl] = wb[e];
     }
     __syncthreads();  // the w_f cotangents are complete
     if (half == 0 && tl < NQ) {
-        double w[NC], wb[NC], uq[NC];
+        double wb[NC];
 #pragma unroll
         for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
-        prim_to_conservative<D>(p.law, hi, uq);
-        Ph::c2e(p.law, uq, w);
 #pragma unroll
         for (int e = 0; e < NC; ++e) wb[e] = 0.0;
-        A::e2c_adj(p.law, w, ubq, wb);
+        A::e2c_adj_prim(p.law, hi[A::H_RHO], hi + A::H_V, hi[A::H_P], ubq, wb);
         // w_q' cotangent += R' (w_f cotangent)
         for (int ww = 0; ww < tp.wrt; ++ww) {
             const int jj = tp.rt_idx[ww * NQ + tl];
@@ -418,7 +405,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
     __syncthreads();
     SF::Vt(tp, tt, sub, sy, sZ, sW);
     double* outk = ubar + (size_t)k * NP * NC;
-    for (int t = tt; t < NP * NC; t += TS) outk[t] = sy[t];
+    for (int t = tt; t < NP * NC; t += TS) outk[t] = (out.accumulate ? outk[t] : 0.0) + out.scale * sy[t];
     // λ cotangent of this element, summed in a fixed order
     red[tt] = lam;
     __syncthreads();
@@ -426,7 +413,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
         if (tt < s) red[tt] += red[tt + s];
         __syncthreads();
     }
-    if (tt == 0) lam_partial[k] = red[0];
+    if (tt == 0) lam_partial[k] = (out.accumulate ? lam_partial[k] : 0.0) + out.scale * red[0];
 }
 
 // ----------------------------------------------------------------------------- launch
@@ -440,7 +427,7 @@ size_t vjp_tensor_lds(int which) {
 template <int ELEM, int P>
 hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
                                double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
-                               hipStream_t s) {
+                               double scale, int accumulate, hipStream_t s) {
     using T = VjpTune<ELEM, P>;
     static bool attr = false;
     const size_t l1 = vjp_tensor_lds<ELEM, P>(1), l2 = vjp_tensor_lds<ELEM, P>(2);
@@ -458,7 +445,7 @@ hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, c
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l2, s, tp, st, u, rbar, fbar, fbar_ext, ubar,
-                       lam_partial);
+                       lam_partial, VjpOut{scale, accumulate});
     return hipGetLastError();
 }
 
@@ -478,12 +465,15 @@ size_t vjp_tensor_config(int elem, int P) {
     return 0;
 }
 
+// ubar = scale * (dR/du)' ybar and lam_partial[k] = scale * element k's (dR/dλ)' ybar, or
+// both added (accumulate != 0); phase 1 must have run on u with tp/st's buffers.
 hipError_t launch_vjp_tensor(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
                              double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
-                             hipStream_t s) {
+                             double scale, int accumulate, hipStream_t s) {
 #define SSE_CASE(E_, PP)                                                                                   \
     if (tp.elem == E_ && tp.P == PP)                                                                       \
-        return launch_vjp_tensor_t<E_, PP>(tp, st, u, ybar, ubar, rbar, fbar, fbar_ext, lam_partial, s);
+        return launch_vjp_tensor_t<E_, PP>(tp, st, u, ybar, ubar, rbar, fbar, fbar_ext, lam_partial, scale,  \
+                                           accumulate, s);
     SSE_VJP_CASES
 #undef SSE_CASE
     return hipErrorInvalidValue;
