@@ -64,7 +64,8 @@ size_t std_tensor_config(int elem, int P);
 size_t vjp_tensor_config(int elem, int P);
 hipError_t launch_vjp_tensor(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
                              double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
-                             double scale, int accumulate, hipStream_t s);
+                             double scale, int accumulate, const double* in_x, double in_a, double in_b,
+                             double* in_store, hipStream_t s);
 hipError_t launch_vjp_sum(const double* lam_partial, int n, double* lam_out, double lam_scale, int accumulate,
                           hipStream_t s);
 hipError_t collapse_metric(double* Lam, const double* c, int NQ, int D, int N_e, hipStream_t s);
@@ -1046,10 +1047,12 @@ int launch_struct_into(sse_handle* h, double* u, const RKStage& rk, const Phase1
     return 0;
 }
 
-// Tensor VJP: ubar (+)= scale (dR/du)' ybar, the element λ partials (+)= scale (dR/dλ)' ybar.
-// With `stored`, phase 1 of u already sits in those buffers; otherwise it is run here.
-int vjp_tensor_device(sse_handle* h, const double* u, const double* ybar, double* ubar, double scale, int accumulate,
-                      const Phase1Bufs* stored) {
+// Tensor VJP: ubar (+)= scale (dR/du)' y, the element λ partials (+)= scale (dR/dλ)' y, with
+// y = ybar, or y = a ybar + b x stored back into ybar when x is set (the RK adjoint's
+// kbar update). With `stored`, phase 1 of u already sits in those buffers; otherwise it
+// is run here.
+int vjp_tensor_device(sse_handle* h, const double* u, double* ybar, double* ubar, double scale, int accumulate,
+                      const Phase1Bufs* stored, const double* x = nullptr, double a = 1.0, double b = 0.0) {
     Phase1Bufs own{h->TP.base.uq, h->TP.base.uf, h->TP.uf_ext};
     if (!stored) {
         int rc = launch(h, u, nullptr, RKStage{}, 1);
@@ -1059,7 +1062,7 @@ int vjp_tensor_device(sse_handle* h, const double* u, const double* ybar, double
     sse::StructTables st;
     with_phase1(h, stored ? *stored : own, tp, st);
     hipError_t e = sse::launch_vjp_tensor(tp, st, u, ybar, ubar, h->vjp_rbar, h->vjp_fbar, h->vjp_fext, h->vjp_lam,
-                                          scale, accumulate, h->stream);
+                                          scale, accumulate, x, a, b, x ? ybar : nullptr, h->stream);
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -1068,7 +1071,7 @@ int vjp_tensor_device(sse_handle* h, const double* u, const double* ybar, double
 int vjp_device(sse_handle* h, const double* u, const double* ybar, double* ubar, double lam_scale, int accumulate) {
     hipError_t e = hipSuccess;
     if (h->vjp_tensor) {
-        int rc = vjp_tensor_device(h, u, ybar, ubar, 1.0, 0, nullptr);
+        int rc = vjp_tensor_device(h, u, const_cast<double*>(ybar), ubar, 1.0, 0, nullptr);  // ybar only read
         if (rc) return rc;
         e = sse::launch_vjp_sum(h->vjp_lam, h->P.N_e, h->vjp_lamout, lam_scale, accumulate, h->stream);
     } else {
@@ -1213,15 +1216,14 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         // adjoint of the 2N stages: kbar += B_s ubar; ubar += dt R'(u_s)' kbar; kbar *= A_s
         for (int step = last - 1; step >= first; --step) {
             if (fast) {
-                // kbar = B_4 ubar; per stage ubar += dt R'(u_s)' kbar (written by the VJP kernel),
-                // then kbar = A_s kbar + B_{s-1} ubar
-                ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[4], ub, 0.0, kb, h->stream), "axpby"));
+                // per stage: kbar = A_{s+1} kbar + B_s ubar (A_5 = 0; done by the VJP's first
+                // kernel), then ubar += dt R'(u_s)' kbar (written by its last kernel)
                 for (int s = 4; s >= 0; --s) {
                     const int idx = 5 * (step - first) + s;
                     const double* us = stages + n * (size_t)idx;
                     const Phase1Bufs b = keep_p1 ? p1_of(idx) : Phase1Bufs{nullptr, nullptr, nullptr};
-                    ADJ_TRY(vjp_tensor_device(h, us, kb, ub, dt, 1, keep_p1 ? &b : nullptr));
-                    if (s > 0) ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[s - 1], ub, CK_A[s], kb, h->stream), "axpby"));
+                    ADJ_TRY(vjp_tensor_device(h, us, kb, ub, dt, 1, keep_p1 ? &b : nullptr, ub,
+                                              s == 4 ? 0.0 : CK_A[s + 1], CK_B[s]));
                 }
                 continue;
             }

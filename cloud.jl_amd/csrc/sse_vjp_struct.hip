@@ -58,6 +58,14 @@ struct VjpLayout {
 
 }  // namespace
 
+// Input of phase V1: ybar := a ybar + b x (x optional, a = 0 does not read ybar),
+// stored back to `store` when set (the RK adjoint's kbar = A_s kbar + B_{s-1} ubar).
+struct VjpIn {
+    const double* x;
+    double a, b;
+    double* store;
+};
+
 // Output mode of phase V2: ubar (and the element's λ cotangent) = scale * VJP,
 // or += scale * VJP when accumulating (the RK adjoint's ubar += dt R'(u_s)' kbar).
 struct VjpOut {
@@ -69,8 +77,8 @@ struct VjpOut {
 // fbar also into the neighbour's exterior slot [k][e][j] (mapP is an involution)
 template <int ELEM, int P>
 __global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
-k_vjp_v1_tensor(TensorParams tp, const double* __restrict__ ybar, double* __restrict__ rbar,
-                double* __restrict__ fbar, double* __restrict__ fbar_ext) {
+k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, double* __restrict__ fbar,
+                double* __restrict__ fbar_ext, VjpIn in) {
     using Sh = Shape<ELEM, P>;
     constexpr int TS = VjpTune<ELEM, P>::TS, NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF;
     using SF = SumFact<ELEM, P, TS>;
@@ -81,8 +89,13 @@ k_vjp_v1_tensor(TensorParams tp, const double* __restrict__ ybar, double* __rest
     double* sx = sy + NC * NP;       // NC x NQ
     double* sZ = sx + NC * NQ;       // NC x NZ
     double* sW = sZ + NC * Sh::NZ;   // NC x NW
-    const double* yk = ybar + (size_t)k * NP * NC;
-    for (int t = tl; t < NP * NC; t += TS) sy[t] = yk[t];
+    const size_t kb = (size_t)k * NP * NC;
+    for (int t = tl; t < NP * NC; t += TS) {
+        double y = in.a == 0.0 ? 0.0 : in.a * ybar[kb + t];
+        if (in.x) y += in.b * in.x[kb + t];
+        if (in.store) in.store[kb + t] = y;
+        sy[t] = y;
+    }
     __syncthreads();
     SF::V(tp, tl, sy, sx, sZ, sW);
     const double* Jk = p.Jq + (size_t)k * NQ;
@@ -151,7 +164,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
     double* red = sm + L::off_red;
 
     // ---- phase A: loads, hoisting, interface flux adjoints
-    double hi[NH], Li[D * D], ubq[NC];
+    double hi[NH], ubq[NC];
     double lam = 0.0;
 #pragma unroll
     for (int e = 0; e < NC; ++e) ubq[e] = 0.0;
@@ -169,10 +182,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
 #pragma unroll
         for (int v = 0; v < NH; ++v) sh[v * NQ + tl] = hi[v];
 #pragma unroll
-        for (int mn = 0; mn < D * D; ++mn) {
-            Li[mn] = Lk[mn * NQ + tl];
-            sL[mn * NQ + tl] = Li[mn];
-        }
+        for (int mn = 0; mn < D * D; ++mn) sL[mn * NQ + tl] = Lk[mn * NQ + tl];
     }
     if (half == 0 && tl < NF) {
         const double* ufk = p.uf + (size_t)k * NF * NC;
@@ -229,13 +239,9 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
     if (SPLIT > 1 && half > 0 && tl < NQ) {
 #pragma unroll
         for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
-#pragma unroll
-        for (int mn = 0; mn < D * D; ++mn) Li[mn] = sL[mn * NQ + tl];
     }
-    if (tl < NQ) {
-        double hj[NH], av[D], gb[NC], rbi[NC];
-#pragma unroll
-        for (int e = 0; e < NC; ++e) rbi[e] = srb[e * NQ + tl];
+    if (tl < NQ) {  // the node's metric and rbar are read from LDS (fewer live registers)
+        double hj[NH], av[D], gb[NC];
         const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
         const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
         int f = q0 / (N1 - 1), dlt = q0 - f * (N1 - 1) + 1;
@@ -252,10 +258,10 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
             for (int m = f; m < D; ++m) {
                 const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
 #pragma unroll
-                for (int n = 0; n < D; ++n) av[n] += s * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
+                for (int n = 0; n < D; ++n) av[n] += s * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
             }
 #pragma unroll
-            for (int e = 0; e < NC; ++e) gb[e] = srb[e * NQ + j] - rbi[e];
+            for (int e = 0; e < NC; ++e) gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
             A::ec_dot_adj_h(p.law, hi, hj, av, gb, true, ubq);
             if (++dlt == N1) {
                 dlt = 1;
@@ -275,11 +281,11 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
             for (int n = 0; n < D; ++n) {
                 double s = snJ[n * NF + j];
 #pragma unroll
-                for (int l = 0; l < D; ++l) s += Li[l + D * n] * p.fn[l + D * fc];
+                for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + tl] * p.fn[l + D * fc];
                 av[n] = 0.5 * crv * s;
             }
 #pragma unroll
-            for (int e = 0; e < NC; ++e) gb[e] = -(rbi[e] + sfb[e * NF + j]);
+            for (int e = 0; e < NC; ++e) gb[e] = -(srb[e * NQ + tl] + sfb[e * NF + j]);
             A::ec_dot_adj_h(p.law, hi, hj, av, gb, true, ubq);
         }
     }
@@ -427,7 +433,7 @@ size_t vjp_tensor_lds(int which) {
 template <int ELEM, int P>
 hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
                                double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
-                               double scale, int accumulate, hipStream_t s) {
+                               double scale, int accumulate, const VjpIn& in, hipStream_t s) {
     using T = VjpTune<ELEM, P>;
     static bool attr = false;
     const size_t l1 = vjp_tensor_lds<ELEM, P>(1), l2 = vjp_tensor_lds<ELEM, P>(2);
@@ -441,7 +447,7 @@ hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, c
         attr = true;
     }
     const int nb = tp.base.N_e;
-    hipLaunchKernelGGL((k_vjp_v1_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l1, s, tp, ybar, rbar, fbar, fbar_ext);
+    hipLaunchKernelGGL((k_vjp_v1_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l1, s, tp, ybar, rbar, fbar, fbar_ext, in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l2, s, tp, st, u, rbar, fbar, fbar_ext, ubar,
@@ -465,15 +471,18 @@ size_t vjp_tensor_config(int elem, int P) {
     return 0;
 }
 
-// ubar = scale * (dR/du)' ybar and lam_partial[k] = scale * element k's (dR/dλ)' ybar, or
-// both added (accumulate != 0); phase 1 must have run on u with tp/st's buffers.
+// ubar = scale * (dR/du)' y and lam_partial[k] = scale * element k's (dR/dλ)' y, or both
+// added (accumulate != 0), with y = in_a ybar + in_b in_x (in_x optional; y also stored to
+// in_store when set); phase 1 must have run on u with tp/st's buffers.
 hipError_t launch_vjp_tensor(const TensorParams& tp, const StructTables& st, const double* u, const double* ybar,
                              double* ubar, double* rbar, double* fbar, double* fbar_ext, double* lam_partial,
-                             double scale, int accumulate, hipStream_t s) {
+                             double scale, int accumulate, const double* in_x, double in_a, double in_b,
+                             double* in_store, hipStream_t s) {
+    const VjpIn in{in_x, in_a, in_b, in_store};
 #define SSE_CASE(E_, PP)                                                                                   \
     if (tp.elem == E_ && tp.P == PP)                                                                       \
         return launch_vjp_tensor_t<E_, PP>(tp, st, u, ybar, ubar, rbar, fbar, fbar_ext, lam_partial, scale,  \
-                                           accumulate, s);
+                                           accumulate, in, s);
     SSE_VJP_CASES
 #undef SSE_CASE
     return hipErrorInvalidValue;
