@@ -350,6 +350,7 @@ def main():
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--adj-mesh", type=int, default=12)
+    ap.add_argument("--autotune", action="store_true", help="pick the residual's kernel configuration by timing")
     ap.add_argument("--ad-mesh", type=int, default=24)
     ap.add_argument("--ad-members", type=int, default=64, help="diffusivities in the sweep (0: none)")
     ap.add_argument("--ad-steps", type=int, default=10)
@@ -390,6 +391,14 @@ def main():
 
     u = to_device(ode.u0)
     dudt = torch.empty_like(u)
+    tuned = None
+    if args.autotune:  # time every compiled configuration on this state, keep the fastest
+        from sse import tune
+        timings = tune.autotune(solver, u, nrep=10)
+        tune.select(timings[0])
+        solver = solver.with_law(solver.conservation_law)
+        tuned = [{"kernels": t.describe, "ms": round(t.total_ms, 4)} for t in timings]
+        log("autotune: " + "; ".join(f"{t.total_ms:.3f} ms {t.describe}" for t in timings))
     log("kernels: " + solver.describe())
     for _ in range(args.warmup):
         semi_discrete_residual(dudt, u, solver)
@@ -440,13 +449,15 @@ def main():
                                "ModalTensor p=%d, %d^3 hexes x 6 tets, one residual per step" % (args.p, args.mesh),
                    "N_e": N_e, "p": args.p, "N_p": N_p, "N_q": sd.reference_approximation.N_q,
                    "N_f": sd.reference_approximation.N_f, "N_c": N_c, "DOF": dof,
-                   "kernel_ms": kms},
+                   "kernel_ms": kms, "kernels": solver.describe()},
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": dom, "flops_per_element": wk.flops,
                      "algorithmic_bytes_per_element": wk.bytes,
                      "hbm_frac_algorithmic": achieved_gbs / HBM_PEAK_GBS},
     }
+    if tuned:
+        line["config"]["autotune"] = tuned
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 8
         src = torch.rand(n, dtype=torch.float64, device=u.device)

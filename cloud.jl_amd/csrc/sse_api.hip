@@ -944,8 +944,8 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     if (h->use_struct) {
         const sse::TensorParams& T = h->TP;
         const sse::StructTables& S = h->ST;
-        snprintf(buf, len, "struct elem=%d P=%d blocks=%d wcr=%d items=%d ch=%d wfi=%d wrt=%d",
-                 T.elem, T.P, h->struct_blocks, S.wcr, S.n_items, S.ch, S.wfi, T.wrt);
+        snprintf(buf, len, "struct elem=%d P=%d variant=%d blocks=%d wcr=%d items=%d ch=%d wfi=%d wrt=%d",
+                 T.elem, T.P, T.variant, h->struct_blocks, S.wcr, S.n_items, S.ch, S.wfi, T.wrt);
     } else if (h->use_tensor) {
         const sse::TensorParams& T = h->TP;
         snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",

@@ -28,11 +28,13 @@ namespace sse {
 // node's partner list; their partial sums meet in LDS)
 template <int ELEM, int P>
 struct StructTune;
-template <int P> struct StructTune<ELEM_TRI, P> { static constexpr int TS = 64, EPB = 2, SPLIT = 1; };
-template <> struct StructTune<ELEM_TET, 2> { static constexpr int TS = 64, EPB = 1, SPLIT = 1; };
-template <> struct StructTune<ELEM_TET, 3> { static constexpr int TS = 128, EPB = 1, SPLIT = 2; };
-template <> struct StructTune<ELEM_TET, 4> { static constexpr int TS = 256, EPB = 1, SPLIT = 2; };
-template <> struct StructTune<ELEM_TET, 5> { static constexpr int TS = 256, EPB = 1, SPLIT = 1; };
+// WAVES: waves per SIMD the register allocation must allow (the LDS of Tet p = 4
+// allows 4; one more register would drop it to 3)
+template <int P> struct StructTune<ELEM_TRI, P> { static constexpr int TS = 64, EPB = 2, SPLIT = 1, WAVES = 1; };
+template <> struct StructTune<ELEM_TET, 2> { static constexpr int TS = 64, EPB = 1, SPLIT = 1, WAVES = 1; };
+template <> struct StructTune<ELEM_TET, 3> { static constexpr int TS = 128, EPB = 1, SPLIT = 2, WAVES = 4; };
+template <> struct StructTune<ELEM_TET, 4> { static constexpr int TS = 256, EPB = 1, SPLIT = 2, WAVES = 4; };
+template <> struct StructTune<ELEM_TET, 5> { static constexpr int TS = 256, EPB = 1, SPLIT = 1, WAVES = 1; };
 
 template <int ELEM, int P>
 struct StructLayout {
@@ -64,7 +66,7 @@ struct StructLayout {
 
 
 template <int ELEM, int P>
-__global__ void __launch_bounds__((StructTune<ELEM, P>::TS * StructTune<ELEM, P>::EPB))
+__global__ void __launch_bounds__((StructTune<ELEM, P>::TS * StructTune<ELEM, P>::EPB), (StructTune<ELEM, P>::WAVES))
 k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ dudt, RKStage rk, int n_teams) {
     using Sh = Shape<ELEM, P>;
     using T = StructTune<ELEM, P>;

@@ -57,6 +57,9 @@ SSE_TUNE(ELEM_TET, 4, 2, 128, 1, 2, 128, 2)
 SSE_TUNE(ELEM_TET, 4, 3, 128, 2, 2, 128, 4)
 SSE_TUNE(ELEM_TET, 4, 4, 128, 1, 1, 64, 2)
 SSE_TUNE(ELEM_TET, 5, 0, 256, 1, 2, 256, 1)
+SSE_TUNE(ELEM_TET, 5, 1, 256, 1, 2, 128, 2)
+SSE_TUNE(ELEM_TET, 3, 1, 64, 2, 3, 64, 4)
+SSE_TUNE(ELEM_TRI, 4, 1, 64, 2, 3, 64, 4)
 #undef SSE_TUNE
 
 // Line-structured operator tables (host-built from the descriptor's S, C and R).

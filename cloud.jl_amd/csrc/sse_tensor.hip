@@ -405,7 +405,7 @@ hipError_t launch_tensor_t(const TensorParams& tp, const double* u, double* dudt
     SSE_CASE(ELEM_TRI, 2, 0) SSE_CASE(ELEM_TRI, 3, 0) SSE_CASE(ELEM_TRI, 4, 0) SSE_CASE(ELEM_TRI, 5, 0) \
     SSE_CASE(ELEM_TRI, 6, 0) SSE_CASE(ELEM_TET, 2, 0) SSE_CASE(ELEM_TET, 3, 0) SSE_CASE(ELEM_TET, 4, 0) \
     SSE_CASE(ELEM_TET, 4, 1) SSE_CASE(ELEM_TET, 4, 2) SSE_CASE(ELEM_TET, 4, 3) SSE_CASE(ELEM_TET, 4, 4) \
-    SSE_CASE(ELEM_TET, 5, 0)
+    SSE_CASE(ELEM_TET, 5, 0) SSE_CASE(ELEM_TET, 5, 1) SSE_CASE(ELEM_TET, 3, 1) SSE_CASE(ELEM_TRI, 4, 1)
 
 // returns false if (elem, P, variant) has no compiled kernels
 bool tensor_config(int elem, int P, int var, TensorConfig* c) {

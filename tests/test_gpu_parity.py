@@ -450,6 +450,29 @@ def test_residual_vjp_dot_product_against_oracle_jvp(name):
     assert _rel(to_host(ub_dev), ubar) <= 1e-14 and abs(lam_dev - lam_bar) <= 1e-12 * (1 + abs(lam_bar))
 
 
+@pytest.mark.parametrize("make", [lambda: density_wave_3d_config(M=2, p=4)[4],
+                                  lambda: vortex_2d_config(M=4, p=4)[4],
+                                  lambda: advection_diffusion_3d_config(M=2, p=3)[4]],
+                         ids=["tet_p4_euler", "tri_p4_euler", "tet_p3_br1"])
+def test_autotune_times_every_configuration_and_selects_one(make, monkeypatch):
+    # every compiled configuration gives the same residual (checked inside autotune)
+    # and the selection is what a new handle then runs
+    from sse import tune
+    from sse.time_integration import to_device
+    monkeypatch.delenv("SSE_KERNELS", raising=False)
+    monkeypatch.delenv("SSE_TENSOR_VARIANT", raising=False)
+    ode = make()
+    u = to_device(_perturbed(ode, amp=1e-3))
+    timings = tune.autotune(ode.p, u, nrep=3)
+    assert len(timings) >= 2 and all(t.total_ms > 0 for t in timings)
+    assert len({t.describe for t in timings}) == len(timings)
+    tune.select(timings[0])
+    fresh = ode.p.with_law(ode.p.conservation_law)
+    assert fresh.describe() == timings[0].describe
+    monkeypatch.delenv("SSE_KERNELS", raising=False)
+    monkeypatch.delenv("SSE_TENSOR_VARIANT", raising=False)
+
+
 @pytest.mark.parametrize("name", ["wave3d_tet_p3_lf", "wave3d_tet_p4_lf", "vortex2d_tri_p4_lf"])
 def test_tensor_vjp_equals_general_vjp(name, monkeypatch):
     # the compile-time-shaped VJP kernels against the general ones, entry by entry
