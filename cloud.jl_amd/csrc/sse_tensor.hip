@@ -24,16 +24,20 @@
 namespace sse {
 
 // ----------------------------------------------------------------------------- nodal_values
+// The sum-factorisation temporary Z shares the volume-node array: in V it is dead
+// before the last stage writes the output, in V' the input is dead after the first
+// stage. The modal input u and the modal w share one array (u is dead after u_q = V u).
 template <int ELEM, int P>
 struct K1Layout {
     using Sh = Shape<ELEM, P>;
     static constexpr int NC = Sh::NC;
-    static constexpr int off_u = 0;                         // NC x NP
-    static constexpr int off_a = off_u + NC * Sh::NP;       // NC x NQ
-    static constexpr int off_w = off_a + NC * Sh::NQ;       // NC x NP
-    static constexpr int off_z = off_w + NC * Sh::NP;       // NC x NZ
-    static constexpr int off_W = off_z + NC * Sh::NZ;       // NC x NW
-    static constexpr int size = off_W + NC * Sh::NW;
+    static_assert(Sh::NZ <= Sh::NQ, "Z inside the volume-node array");
+    static constexpr int off_a = 0;                                  // NC x NQ (and Z)
+    static constexpr int off_z = off_a;                              // NC x NZ
+    static constexpr int off_W = off_a + NC * Sh::NQ;                // NC x NW
+    static constexpr int off_u = off_W + NC * (Sh::NW > 0 ? Sh::NW : 1);  // NC x NP: u, then w
+    static constexpr int off_w = off_u;
+    static constexpr int size = off_u + NC * Sh::NP;
 };
 
 template <int ELEM, int P, int VAR>
