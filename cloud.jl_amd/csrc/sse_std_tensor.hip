@@ -137,7 +137,8 @@ template <int ELEM, int P>
 struct StdNodalLayout {
     using Sh = Shape<ELEM, P>;
     static constexpr int E = StdTune<ELEM, P>::E1;
-    static constexpr int u = 0, q = u + E * Sh::NP, z = q + E * Sh::NQ, w = z + E * Sh::NZ;
+    // the sum-factorisation temporary Z lives in u_q's array (dead before V's last stage)
+    static constexpr int u = 0, q = u + E * Sh::NP, z = q, w = q + E * Sh::NQ;
     static constexpr int size = w + E * cmax(Sh::NW, 1);
 };
 

@@ -86,9 +86,9 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
     const DevParams& p = tp.base;
     const int k = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap), tl = threadIdx.x;
     double* sy = sm;                 // NC x NP
-    double* sx = sy + NC * NP;       // NC x NQ
-    double* sZ = sx + NC * NQ;       // NC x NZ
-    double* sW = sZ + NC * Sh::NZ;   // NC x NW
+    double* sx = sy + NC * NP;       // NC x NQ; also the sum-factorisation temporary Z
+    double* sZ = sx;                 // (dead before V's last stage, and after V''s first)
+    double* sW = sx + NC * NQ;       // NC x NW
     const size_t kb = (size_t)k * NP * NC;
     for (int t = tl; t < NP * NC; t += TS) {
         double y = in.a == 0.0 ? 0.0 : in.a * ybar[kb + t];
@@ -426,7 +426,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
 template <int ELEM, int P>
 size_t vjp_tensor_lds(int which) {
     using Sh = Shape<ELEM, P>;
-    if (which == 1) return sizeof(double) * Sh::NC * (Sh::NP + Sh::NQ + Sh::NZ + (Sh::NW > 0 ? Sh::NW : 1));
+    if (which == 1) return sizeof(double) * Sh::NC * (Sh::NP + Sh::NQ + (Sh::NW > 0 ? Sh::NW : 1));
     return sizeof(double) * VjpLayout<ELEM, P>::size;
 }
 
