@@ -48,21 +48,42 @@ class ODESolution:
     u: np.ndarray
 
 
-def solve(ode: ODEProblem, alg=None, dt: float = None, nsteps: int = None) -> ODESolution:
+def solve(ode: ODEProblem, alg=None, dt: float = None, nsteps: int = None, callback=None) -> ODESolution:
     """``solve(ode, CarpenterKennedy2N54(), dt=dt, adaptive=false)`` from tspan[0]
     to tspan[1] (or for ``nsteps`` steps). As OrdinaryDiffEq does with the end of
-    tspan as a stop time, a last shorter step lands exactly on tspan[1]."""
+    tspan as a stop time, a last shorter step lands exactly on tspan[1].
+
+    `callback` (e.g. ``sse.io.SaveCallback``, the reference's ``save_callback``,
+    ``src/File/save.jl:74-86``) is called as callback(solver, u_device, t, step) at
+    the start, after every ``callback.interval`` steps, and at the end."""
     if dt is None:
         raise ValueError("fixed-step integration needs dt")
     t0, t1 = ode.tspan
     u = to_device(ode.u0)
     if nsteps is not None:
-        rk_steps(ode.p, u, dt, nsteps, t0)
-        return ODESolution(t0 + nsteps * dt, to_host(u))
-    n = int(np.floor((t1 - t0) / dt * (1.0 + 1e-12)))
-    rk_steps(ode.p, u, dt, n, t0)
-    t = t0 + n * dt
-    rest = t1 - t
-    if rest > 1e-12 * max(abs(t1), dt):
-        rk_steps(ode.p, u, rest, 1, t)
-    return ODESolution(t1, to_host(u))
+        n, rest = int(nsteps), 0.0
+    else:
+        n = int(np.floor((t1 - t0) / dt * (1.0 + 1e-12)))
+        rest = t1 - (t0 + n * dt)
+        if rest <= 1e-12 * max(abs(t1), dt):
+            rest = 0.0
+    if callback is None:
+        rk_steps(ode.p, u, dt, n, t0)
+        if rest:
+            rk_steps(ode.p, u, rest, 1, t0 + n * dt)
+    else:
+        every = max(1, int(callback.interval))
+        callback(ode.p, u, t0, 0)
+        step = 0
+        while step < n:
+            k = min(every - step % every, n - step)
+            rk_steps(ode.p, u, dt, k, t0 + step * dt)
+            step += k
+            if step % every == 0:
+                callback(ode.p, u, t0 + step * dt, step)
+        if rest:
+            rk_steps(ode.p, u, rest, 1, t0 + n * dt)
+            step += 1
+        callback(ode.p, u, t1 if nsteps is None else t0 + n * dt, step)  # end of tspan
+    t_end = t0 + n * dt + rest if nsteps is None else t0 + n * dt
+    return ODESolution(t1 if nsteps is None else t_end, to_host(u))

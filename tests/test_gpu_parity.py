@@ -473,6 +473,39 @@ def test_autotune_times_every_configuration_and_selects_one(make, monkeypatch):
     monkeypatch.delenv("SSE_TENSOR_VARIANT", raising=False)
 
 
+def test_solve_with_save_callback_writes_the_reference_layout(tmp_path):
+    # src/File/save.jl:74-86: every `interval` steps and both ends of tspan, with du = R(u)
+    from sse import io
+    from sse.time_integration import CarpenterKennedy2N54, solve
+    law, sd, ic, form, ode = vortex_2d_config(M=3, p=3)
+    ode.tspan = (0.0, 0.0105)
+    path = io.save_project(law, sd, ic, form, ode.tspan, str(tmp_path / "res"))
+    sol = solve(ode, CarpenterKennedy2N54(), dt=1e-3, callback=io.SaveCallback(path, interval=4))
+    assert list(io.load_time_steps(path)) == [0, 4, 8, 11]
+    u, du, t = io.load_solution(path, 11, load_du=True)
+    assert t == 0.0105 and np.array_equal(u, sol.u)
+    assert _rel(du, O.residual(ode.p.arrays, u)) <= RES_TOL
+    u0, t0 = io.load_solution(path, 0)
+    assert t0 == 0.0 and np.array_equal(u0, ode.u0)
+    ref = O.rk_steps(ode.p.arrays, ode.u0, 1e-3, 8)
+    assert _rel(io.load_solution(path, 8)[0], ref) <= RK_TOL
+
+
+def test_reference_scaling_benchmark_problem_and_timings(tmp_path):
+    # src/Analysis/benchmark.jl:116-183: the residual of its problem (ChanWarping-curved
+    # Tri p=4, isentropic vortex) against the oracle, then the timing report
+    from sse.benchmark import scaling_test_euler_2d, scaling_test_euler_2d_problem
+    ode = scaling_test_euler_2d_problem(4, 4)
+    got = _gpu_residual(ode)
+    ref = O.residual(ode.p.arrays, ode.u0)
+    assert _rel(got, ref) <= RES_TOL, _rel(got, ref)
+    for _ in range(2):
+        r = scaling_test_euler_2d(4, 4, path=str(tmp_path), samples=5, verbose=False)
+        assert 0 < r["min_ms"] <= r["median_ms"] and r["N_e"] == 32
+    assert np.load(tmp_path / "minimum.npy").shape == (2,)
+    assert np.load(tmp_path / "median.npy").shape == (2,)
+
+
 @pytest.mark.parametrize("name", ["wave3d_tet_p3_lf", "wave3d_tet_p4_lf", "vortex2d_tri_p4_lf"])
 def test_tensor_vjp_equals_general_vjp(name, monkeypatch):
     # the compile-time-shaped VJP kernels against the general ones, entry by entry

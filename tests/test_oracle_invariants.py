@@ -408,3 +408,18 @@ def test_energy_functional_on_affine_mesh_is_half_l2_norm():
     uq = np.einsum("qp,pck->qck", ref.Vdense(), ode.u0)
     WJ = ref.W[:, None] * sd.geometric_factors.J_q
     np.testing.assert_allclose(F.energy(sd, ode.u0), 0.5 * np.einsum("qk,qck->c", WJ, uq ** 2), rtol=1e-12)
+
+
+def test_reference_scaling_benchmark_problem():
+    # src/Analysis/benchmark.jl:120-159: sizes, end time L / Ma, and a mass-conserving
+    # residual on the ChanWarping-curved mesh
+    from sse.benchmark import scaling_test_euler_2d_problem
+    ode = scaling_test_euler_2d_problem(3, 3)
+    assert ode.p.size() == (10, 4, 18) and ode.tspan == (0.0, 2.5)
+    sd = ode.p.spatial_discretization
+    ref = sd.reference_approximation
+    dudt = O.residual(ode.p.arrays, ode.u0)
+    WJ = ref.W[:, None] * sd.geometric_factors.J_q
+    mass = np.einsum("qk,qp,pck->c", WJ, ref.Vdense(), dudt)
+    scale = np.einsum("qk,qp,pck->c", WJ, np.abs(ref.Vdense()), np.abs(dudt))
+    assert np.all(np.abs(mass) <= 1e-13 * scale), (mass, scale)
