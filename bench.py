@@ -217,7 +217,7 @@ def bench_advdiff(args):
     from sse.workloads import advection_diffusion_3d_config
 
     t0 = time.perf_counter()
-    law, sd, ex, form, ode = advection_diffusion_3d_config(M=args.ad_mesh, p=5, b=5e-3, sigma=0.1)
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=args.ad_mesh, p=5, b=5e-3, sigma=0.1, backend="hip")
     solver = ode.p
     N_p, N_c, N_e = solver.size()
     dof = N_p * N_c * N_e
@@ -244,6 +244,70 @@ def bench_advdiff(args):
     }
     if sweep:
         line["sweep"] = sweep
+    print(json.dumps(line), flush=True)
+
+
+def bench_setup(args):
+    """Set-up of the 3D advection-diffusion problem (24^3 hexes x 6 curved tets, p=5):
+    host (NumPy) against GPU (sse_geometric_factors, sse_apply_reference_mapping,
+    sse_project_l2), stage by stage, with the largest difference of the arrays they
+    produce. Prints one JSON line; value = GPU set-up seconds (the mesh connectivity,
+    shared by both, is timed apart)."""
+    import torch
+
+    from sse.conservation_laws import BR1, InitialDataGaussian, LaxFriedrichsNumericalFlux
+    from sse.conservation_laws import LinearAdvectionDiffusionEquation
+    from sse.geometry import ChanWilcoxMetrics, make_spatial_discretization
+    from sse.mesh import UniformWarping, uniform_periodic_mesh, warp_mesh
+    from sse.reference_approximation import ModalTensor, ReferenceApproximationTet
+    from sse.solvers import PhysicalOperator, SkewSymmetricMapping, Solver, StandardForm, initialize
+
+    M, p = args.ad_mesh, 5
+    law = LinearAdvectionDiffusionEquation((1.0, 0.5, 0.25), 5e-3)
+    ic = InitialDataGaussian(1.0, 0.1, (0.5, 0.5, 0.5))
+    form = StandardForm(mapping_form=SkewSymmetricMapping(), inviscid_numerical_flux=LaxFriedrichsNumericalFlux(),
+                        viscous_numerical_flux=BR1())
+    t0 = time.perf_counter()
+    ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=p)
+    t_ref = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mesh = warp_mesh(uniform_periodic_mesh(ref, ((0.0, 1.0),) * 3, (M, M, M)), ref,
+                     UniformWarping(0.05, (1.0, 1.0, 1.0)))
+    t_mesh = time.perf_counter() - t0
+    torch.cuda.init()
+    out, arrays = {}, {}
+    for backend in ("hip", "host"):
+        st = {}
+        t0 = time.perf_counter()
+        sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics(), backend=backend)
+        st["geometric_factors_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        u0 = initialize(ic, sd)
+        st["initial_projection_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        solver = Solver(law, sd, form, PhysicalOperator())  # reference mapping + descriptor + handle
+        st["solver_s"] = time.perf_counter() - t0
+        st["total_s"] = sum(st.values())
+        log(f"set-up {backend}: " + ", ".join(f"{k} {v:.2f}" for k, v in st.items()))
+        arrays[backend] = (sd.geometric_factors, u0)
+        solver.close()
+        out[backend] = st
+        del sd, solver
+    (gh, uh), (gd, ud) = arrays["host"], arrays["hip"]
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    diffs = {n: rel(getattr(gd, n), getattr(gh, n)) for n in ("J_q", "Lambda_q", "J_f", "nJf", "nJq")}
+    diffs["u0"] = rel(ud, uh)
+    log("max relative differences: " + ", ".join(f"{k} {v:.1e}" for k, v in diffs.items()))
+    line = {
+        "metric": "set-up seconds (3D advection-diffusion, 82,944 curved tets, p=5)", "value": out["hip"]["total_s"],
+        "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * out["hip"]["total_s"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"set-up of the 3D advection-diffusion problem, {M}^3 hexes x 6 curved tets, "
+                               "ModalTensor p=5, curl metrics, PhysicalOperator",
+                   "N_e": mesh.N_e, "reference_approximation_s": t_ref, "mesh_s": t_mesh,
+                   "gpu": out["hip"], "host": out["host"], "host_over_gpu": out["host"]["total_s"] / out["hip"]["total_s"],
+                   "max_rel_diff": diffs},
+    }
     print(json.dumps(line), flush=True)
 
 
@@ -345,7 +409,7 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint", "advdiff"],
+    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint", "advdiff", "setup"],
                     help="residual: the curved-tet Euler p=4 residual (default); batched: the "
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
@@ -375,6 +439,8 @@ def main():
         return bench_adjoint(args)
     if args.config == "advdiff":
         return bench_advdiff(args)
+    if args.config == "setup":
+        return bench_setup(args)
 
     import torch
 

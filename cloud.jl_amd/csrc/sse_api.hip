@@ -767,6 +767,11 @@ size_t state_size(const sse_handle* h) { return (size_t)h->P.N_p * h->P.N_c * h-
 
 }  // namespace
 
+namespace sse {
+// error of a call that has no handle (set-up entries, failed creation)
+int set_global_error(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+}  // namespace sse
+
 extern "C" {
 
 int32_t sse_abi_version(void) { return SSE_ABI_VERSION; }

@@ -53,5 +53,17 @@ EXPORTED_SYMBOLS = (
     "sse_synchronize", "sse_last_error", "sse_destroy", "sse_abi_version", "sse_describe",
     "sse_batch_create", "sse_batch_rk_steps", "sse_batch_describe", "sse_batch_last_error",
     "sse_batch_destroy", "sse_functional", "sse_residual_vjp", "sse_rk_adjoint", "sse_batch_rk_tangent",
+    "sse_geometric_factors", "sse_apply_reference_mapping", "sse_project_l2",
 )
+
+SSE_METRICS_EXACT, SSE_METRICS_CURL = 1, 2
+
+
+class SseGeometryOps(C.Structure):
+    _fields_ = [
+        ("d", C.c_int32), ("metrics", C.c_int32), ("N_p", C.c_int32), ("N_q", C.c_int32),
+        ("N_f", C.c_int32), ("N_fac", C.c_int32),
+        ("D", _dp), ("Vq", _dp), ("Vf", _dp), ("nrstJ", _dp), ("P_J", _dp),
+        ("N_p1", C.c_int32), ("I1", _dp), ("D1", _dp), ("Vq1", _dp), ("Vf1", _dp),
+    ]
 SSE_MASS, SSE_ENERGY, SSE_ENTROPY = 0, 1, 2

@@ -17,6 +17,7 @@ J_f (N_f, N_e); nJf (d, N_f, N_e); nJq (d, N_fac, N_q, N_e).
 
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 from typing import Optional
 
@@ -126,6 +127,21 @@ def _gf3d(x, y, z, Dr, Ds, Dt):
     return rxJ, sxJ, txJ, ryJ, syJ, tyJ, rzJ, szJ, tzJ, J
 
 
+def _tet_curl_operators(rd):
+    """The degree-(N+1) nodal operators of the tetrahedral curl metrics
+    (``mesh.jl:418-447``): degree-N -> degree-(N+1) interpolation, the three
+    derivative matrices at the degree-(N+1) nodes, and Vq, Vf composed with the
+    degree-(N+1) -> degree-N interpolation."""
+    N = rd.N
+    r1, s1, t1 = S.nodes_tet(N + 1)
+    V1p, Vr1, Vs1, Vt1 = S.basis_tet(N + 1, r1, s1, t1)
+    V1pinv = np.linalg.inv(V1p)
+    N_to_N1 = S.vandermonde_tet(N, r1, s1, t1) @ np.linalg.inv(rd.VDM)
+    N1_to_N = S.vandermonde_tet(N + 1, *rd.rst) @ V1pinv
+    D1 = (Vr1 @ V1pinv, Vs1 @ V1pinv, Vt1 @ V1pinv)
+    return N_to_N1, D1, rd.Vq @ N1_to_N, rd.Vf @ N1_to_N
+
+
 def geometric_factors_curl(mesh: MeshData, ref: ReferenceApproximation) -> GeometricFactors:
     """Conservative-curl (Chan-Wilcox) metrics: Tri ``mesh.jl:284-339``,
     Tet ``mesh.jl:410-506``."""
@@ -143,18 +159,10 @@ def geometric_factors_curl(mesh: MeshData, ref: ReferenceApproximation) -> Geome
             Lf[:, l, m, :] = rd.Vf @ v
         J_q = rd.Vq @ J
     elif elem == TET:
-        N = rd.N
-        r1, s1, t1 = S.nodes_tet(N + 1)
-        V1p, Vr1, Vs1, Vt1 = S.basis_tet(N + 1, r1, s1, t1)
-        V1pinv = np.linalg.inv(V1p)
-        N_to_N1 = S.vandermonde_tet(N, r1, s1, t1) @ np.linalg.inv(rd.VDM)
-        N1_to_N = S.vandermonde_tet(N + 1, *rd.rst) @ V1pinv
-        Vq = rd.Vq @ N1_to_N
-        Vf = rd.Vf @ N1_to_N
+        N_to_N1, D1, Vq, Vf = _tet_curl_operators(rd)
         J = _gf3d(*mesh.xyz, *rd.Drst)[-1]
         J_q = rd.Vq @ J
-        g = _gf3d(N_to_N1 @ mesh.xyz[0], N_to_N1 @ mesh.xyz[1], N_to_N1 @ mesh.xyz[2],
-                  Vr1 @ V1pinv, Vs1 @ V1pinv, Vt1 @ V1pinv)
+        g = _gf3d(N_to_N1 @ mesh.xyz[0], N_to_N1 @ mesh.xyz[1], N_to_N1 @ mesh.xyz[2], *D1)
         rxJ, sxJ, txJ, ryJ, syJ, tyJ, rzJ, szJ, tzJ = g[:9]
         comps = {(0, 0): rxJ, (1, 0): sxJ, (2, 0): txJ, (0, 1): ryJ, (1, 1): syJ, (2, 1): tyJ,
                  (0, 2): rzJ, (1, 2): szJ, (2, 2): tzJ}
@@ -170,12 +178,15 @@ def geometric_factors_curl(mesh: MeshData, ref: ReferenceApproximation) -> Geome
     return GeometricFactors(_f(J_q), _f(Lq), J_f, nJf, nJq)
 
 
-def project_jacobian(J_q: np.ndarray, ref: ReferenceApproximation) -> np.ndarray:
-    """(``SpatialDiscretizations.jl:310-317``)."""
+def _jacobian_projector(ref: ReferenceApproximation) -> np.ndarray:
     V = ref.Vdense()
     W = ref.W
-    proj = V @ np.linalg.solve(V.T @ (W[:, None] * V), V.T * W[None, :])
-    return _f(proj @ J_q)
+    return V @ np.linalg.solve(V.T @ (W[:, None] * V), V.T * W[None, :])
+
+
+def project_jacobian(J_q: np.ndarray, ref: ReferenceApproximation) -> np.ndarray:
+    """(``SpatialDiscretizations.jl:310-317``)."""
+    return _f(_jacobian_projector(ref) @ J_q)
 
 
 @dataclass
@@ -184,6 +195,7 @@ class SpatialDiscretization:
     mesh: MeshData
     reference_approximation: ReferenceApproximation
     geometric_factors: GeometricFactors
+    backend: str = "host"   # where set-up steps run ("hip": sse_* set-up entries on the GPU)
 
     @property
     def N_e(self) -> int:
@@ -194,24 +206,99 @@ class SpatialDiscretization:
         return self.reference_approximation.d
 
 
+def _is_exact(metric_type) -> bool:
+    return metric_type is None or isinstance(metric_type, ExactMetrics) or metric_type is ExactMetrics
+
+
 def make_spatial_discretization(mesh: MeshData, ref: ReferenceApproximation, metric_type=None,
-                                project_jacobian_flag: bool = True) -> SpatialDiscretization:
-    if metric_type is None or isinstance(metric_type, ExactMetrics) or metric_type is ExactMetrics:
+                                project_jacobian_flag: bool = True, backend: str = "host") -> SpatialDiscretization:
+    """`backend="hip"` computes the geometric factors on the GPU (``sse_geometric_factors``)
+    and marks the discretisation so that the later set-up steps (the reference mapping
+    of StandardForm, the projection of the initial data) run there too."""
+    if backend == "hip":
+        gf = geometric_factors_hip(mesh, ref, metric_type, project_jacobian_flag)
+    elif backend != "host":
+        raise ValueError(f"unknown set-up backend {backend!r}")
+    elif _is_exact(metric_type):
         gf = geometric_factors_exact(mesh, ref)
         if project_jacobian_flag:
             gf.J_q = project_jacobian(gf.J_q, ref)
     else:
         gf = geometric_factors_curl(mesh, ref)
-    return SpatialDiscretization(mesh, ref, gf)
+    return SpatialDiscretization(mesh, ref, gf, backend)
 
 
-def apply_reference_mapping(gf: GeometricFactors, ref: ReferenceApproximation) -> GeometricFactors:
+def geometric_factors_hip(mesh: MeshData, ref: ReferenceApproximation, metric_type=None,
+                          project_jacobian_flag: bool = True) -> GeometricFactors:
+    """Exact (``mesh.jl:229-282``) or conservative-curl (``mesh.jl:284-339``,
+    ``:410-506``) geometric factors computed by ``sse_geometric_factors`` on the GPU,
+    one workgroup per element; the Jacobian of exact metrics projected as by
+    ``project_jacobian!``. Same arrays as the host functions above."""
+    from . import abi
+    from .hip import check, load_library
+    lib = load_library()
+    rd = ref.reference_element
+    d = ref.d
+    exact = _is_exact(metric_type)
+    if not exact and rd.element_type not in (TRI, TET):
+        raise NotImplementedError(f"curl metrics on {rd.element_type}")
+    keep = []
+
+    def ptr(a):
+        if a is None:
+            return None
+        a = np.asfortranarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    N_p, N_e = mesh.xyz[0].shape
+    N_q, N_f = rd.Vq.shape[0], rd.Vf.shape[0]
+    N_fac = num_faces(rd.element_type)
+    ops = abi.SseGeometryOps()
+    ops.d, ops.N_p, ops.N_q, ops.N_f, ops.N_fac = d, N_p, N_q, N_f, N_fac
+    ops.metrics = abi.SSE_METRICS_EXACT if exact else abi.SSE_METRICS_CURL
+    ops.D = ptr(np.stack(rd.Drst, axis=2))
+    ops.Vq, ops.Vf = ptr(rd.Vq), ptr(rd.Vf)
+    ops.nrstJ = ptr(np.stack(rd.nrstJ, axis=1))
+    ops.P_J = ptr(_jacobian_projector(ref)) if (exact and project_jacobian_flag) else None
+    if not exact and rd.element_type == TET:
+        N_to_N1, D1, Vq1, Vf1 = _tet_curl_operators(rd)
+        ops.N_p1 = N_to_N1.shape[0]
+        ops.I1, ops.D1 = ptr(N_to_N1), ptr(np.stack(D1, axis=2))
+        ops.Vq1, ops.Vf1 = ptr(Vq1), ptr(Vf1)
+    xyz = [np.asfortranarray(c, dtype=np.float64) for c in mesh.xyz] + [None] * (3 - d)
+    J_q = np.empty((N_q, N_e), order="F")
+    Lq = np.empty((N_q, d, d, N_e), order="F")
+    J_f = np.empty((N_f, N_e), order="F")
+    nJf = np.empty((d, N_f, N_e), order="F")
+    nJq = np.empty((d, N_fac, N_q, N_e), order="F")
+    vp = lambda a: None if a is None else a.ctypes.data
+    check(lib, None, lib.sse_geometric_factors(C.byref(ops), N_e, vp(xyz[0]), vp(xyz[1]), vp(xyz[2]),
+                                               vp(J_q), vp(Lq), vp(J_f), vp(nJf), vp(nJq), abi.SSE_HOST))
+    return GeometricFactors(J_q, Lq, J_f, nJf, nJq)
+
+
+def apply_reference_mapping(gf: GeometricFactors, ref: ReferenceApproximation,
+                            backend: str = "host") -> GeometricFactors:
     """Lambda_eta[i, m, n, k] = sum_l Lambda_ref[i, m, l] Lambda_q[i, l, n, k] / J_ref[i]
-    (``SpatialDiscretizations.jl:395-411``)."""
+    (``SpatialDiscretizations.jl:395-411``); on the GPU with `backend="hip"`
+    (``sse_apply_reference_mapping``)."""
     if ref.reference_mapping is None:
         return gf
     Jr = ref.reference_mapping.J_ref
     Lr = ref.reference_mapping.Lambda_ref
+    if backend == "hip":
+        from . import abi
+        from .hip import check, load_library
+        lib = load_library()
+        N_q, d, _, N_e = gf.Lambda_q.shape
+        Lr_f = np.asfortranarray(Lr, dtype=np.float64)
+        Jr_f = np.ascontiguousarray(Jr, dtype=np.float64)
+        Lq = np.asfortranarray(gf.Lambda_q, dtype=np.float64)
+        Le = np.empty((N_q, d, d, N_e), order="F")
+        check(lib, None, lib.sse_apply_reference_mapping(N_q, d, N_e, Lr_f.ctypes.data, Jr_f.ctypes.data,
+                                                         Lq.ctypes.data, Le.ctypes.data, abi.SSE_HOST))
+        return GeometricFactors(gf.J_q, Le, gf.J_f, gf.nJf, gf.nJq)
     Le = np.einsum("iml,ilnk->imnk", Lr, gf.Lambda_q) / Jr[:, None, None, None]
     return GeometricFactors(gf.J_q, _f(Le), gf.J_f, gf.nJf, gf.nJq)
 

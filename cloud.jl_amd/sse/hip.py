@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .abi import EXPORTED_SYMBOLS, SSE_ABI_VERSION, SseDesc
+from .abi import EXPORTED_SYMBOLS, SSE_ABI_VERSION, SseDesc, SseGeometryOps
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "csrc", "libsse_hip.so"))
@@ -72,6 +72,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_residual_vjp.restype = C.c_int
     lib.sse_rk_adjoint.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, C.c_int32, dp, vp, dp, C.c_int32]
     lib.sse_rk_adjoint.restype = C.c_int
+    lib.sse_geometric_factors.argtypes = [C.POINTER(SseGeometryOps), C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                          C.c_int32]
+    lib.sse_geometric_factors.restype = C.c_int
+    lib.sse_apply_reference_mapping.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32]
+    lib.sse_apply_reference_mapping.restype = C.c_int
+    lib.sse_project_l2.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.c_int32]
+    lib.sse_project_l2.restype = C.c_int
     lib.sse_abi_version.argtypes = []
     lib.sse_abi_version.restype = C.c_int32
     for name in EXPORTED_SYMBOLS:

@@ -119,7 +119,7 @@ class DescArrays:
             # the reference mapping and the reference derivative operators D
             # (operators.jl:1-37, 83-160)
             S, S_sparse, Cmat, C_kind = None, False, None, abi.SSE_C_NONE
-            gf = apply_reference_mapping(gf, ref)
+            gf = apply_reference_mapping(gf, ref, getattr(sd, "backend", "host"))
             self.D = np.asfortranarray(np.stack([np.asarray(Dm, dtype=np.float64) for Dm in ref.D], axis=2))
         else:
             S, S_sparse, Cmat, C_kind = flux_differencing_operators(ref)
@@ -349,6 +349,8 @@ def project_function(initial_data, sd: SpatialDiscretization) -> np.ndarray:
     if ref.V is None:
         return np.asfortranarray(uq)
     V = ref.V
+    if getattr(sd, "backend", "host") == "hip":
+        return _project_l2_hip(V, ref.W, sd.geometric_factors.J_q, uq)
     WJ = ref.W[:, None] * sd.geometric_factors.J_q  # N_q x N_e
     N_q, N_p = V.shape
     N_e = WJ.shape[1]
@@ -363,6 +365,23 @@ def project_function(initial_data, sd: SpatialDiscretization) -> np.ndarray:
         y = np.linalg.solve(L, rhs)
         u0[k0:k1] = np.linalg.solve(np.transpose(L, (0, 2, 1)), y)
     return np.asfortranarray(np.transpose(u0, (1, 2, 0)))
+
+
+def _project_l2_hip(V, W, J_q, uq) -> np.ndarray:
+    """The per-element projection of `project_function` on the GPU (``sse_project_l2``:
+    Cholesky of V' W J V in LDS, one workgroup per element)."""
+    from .hip import check, load_library
+    lib = load_library()
+    N_q, N_p = V.shape
+    _, N_c, N_e = uq.shape
+    Vf = np.asfortranarray(V, dtype=np.float64)
+    Wc = np.ascontiguousarray(W, dtype=np.float64)
+    Jf = np.asfortranarray(J_q, dtype=np.float64)
+    uqf = np.asfortranarray(uq, dtype=np.float64)
+    u0 = np.empty((N_p, N_c, N_e), order="F")
+    check(lib, None, lib.sse_project_l2(N_q, N_p, N_c, N_e, Vf.ctypes.data, Wc.ctypes.data, Jf.ctypes.data,
+                                        uqf.ctypes.data, u0.ctypes.data, abi.SSE_HOST))
+    return u0
 
 
 def initialize(initial_data, sd: SpatialDiscretization) -> np.ndarray:

@@ -62,17 +62,18 @@ def vortex_2d_config(M=8, p=4, form=None):
     return law, sd, ic, form, semidiscretize(law, sd, ic, form, (0.0, 1.0))
 
 
-def density_wave_3d_config(M=2, p=4, factor=0.05, form=None, ic_strength=0.5):
+def density_wave_3d_config(M=2, p=4, factor=0.05, form=None, ic_strength=0.5, backend="host"):
     """3D Euler on curved tets: periodic [0,2]^3, M^3 hexes x 6 tets, UniformWarping
     factor 0.05, ModalTensor(p) isoparametric, curl metrics,
-    rho = 1 + 0.5 sin(pi (x+y+z)), u=v=w=1, p=1."""
+    rho = 1 + 0.5 sin(pi (x+y+z)), u=v=w=1, p=1. `backend="hip"` runs the set-up
+    (geometric factors, projection of the initial data) on the GPU."""
     law = EulerEquations(3, 1.4)
     ic = EulerPeriodicTest(3, 1.4, ic_strength, 2.0)
     ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=p)
     mesh = uniform_periodic_mesh(ref, ((0.0, 2.0),) * 3, (M, M, M))
     if factor:
         mesh = warp_mesh(mesh, ref, UniformWarping(factor, (2.0, 2.0, 2.0)))
-    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics(), backend=backend)
     form = form or FluxDifferencingForm(inviscid_numerical_flux=LaxFriedrichsNumericalFlux())
     return law, sd, ic, form, semidiscretize(law, sd, ic, form, (0.0, 1.0))
 
@@ -125,18 +126,19 @@ def advection_3d_modal_tensor_tet(M=2, p=4):
     return law, sd, ExactSolution(law, ic), form, ode
 
 
-def advection_diffusion_3d_config(M=4, p=5, b=5.0e-3, sigma=0.1, factor=0.05):
+def advection_diffusion_3d_config(M=4, p=5, b=5.0e-3, sigma=0.1, factor=0.05, backend="host"):
     """3D linear advection-diffusion on curved tets: a = (1, 0.5, 0.25), diffusivity
     b, BR1, periodic [0,1]^3 with M^3 hexes x 6 tets, UniformWarping, ModalTensor(p)
     isoparametric with curl metrics, Gaussian pulse of width sigma at the centre;
-    StandardForm (skew-symmetric, LF) with PhysicalOperator."""
+    StandardForm (skew-symmetric, LF) with PhysicalOperator. `backend="hip"` runs the
+    set-up (geometric factors, reference mapping, projection) on the GPU."""
     law = LinearAdvectionDiffusionEquation((1.0, 0.5, 0.25), b)
     ic = InitialDataGaussian(1.0, sigma, (0.5, 0.5, 0.5))
     ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=p)
     mesh = uniform_periodic_mesh(ref, ((0.0, 1.0),) * 3, (M, M, M))
     if factor:
         mesh = warp_mesh(mesh, ref, UniformWarping(factor, (1.0, 1.0, 1.0)))
-    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics(), backend=backend)
     form = StandardForm(mapping_form=SkewSymmetricMapping(),
                         inviscid_numerical_flux=LaxFriedrichsNumericalFlux(),
                         viscous_numerical_flux=BR1())

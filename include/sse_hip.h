@@ -279,6 +279,60 @@ const char* sse_batch_last_error(const sse_batch* b);
 /* Frees the batch, its groups and its HIP graph. */
 void sse_batch_destroy(sse_batch* b);
 
+/* ------------------------------------------------------------------ set-up on the device
+ * The steps that build a problem's arrays before the residual path: geometric factors of
+ * curved elements, the reference mapping of StandardForm, and the L2 projection of the
+ * initial data. One workgroup per element; the calls block until done and report errors
+ * through sse_last_error(NULL). `where` says where the element arrays live; the reference
+ * operators in sse_geometry_ops are host pointers. All arrays are column-major. */
+
+/* metric types (src/SpatialDiscretizations/SpatialDiscretizations.jl:34-36) */
+typedef enum { SSE_METRICS_EXACT = 1, SSE_METRICS_CURL = 2 } sse_metrics;
+
+/* The RefElemData fields GeometricFactors reads (StartUpDG's RefElemData: Drst, Vq, Vf,
+ * nrstJ), the Jacobian projector of project_jacobian!, and for conservative-curl metrics on
+ * tetrahedra the degree-(N+1) nodal operators of mesh.jl:410-447. */
+typedef struct sse_geometry_ops {
+    int32_t d;           /* 1, 2 or 3 */
+    int32_t metrics;     /* sse_metrics */
+    int32_t N_p;         /* mapping nodes per element (degree-N nodal set) */
+    int32_t N_q, N_f, N_fac;
+    const double* D;     /* N_p x N_p x d: Dr, Ds, Dt at the mapping nodes */
+    const double* Vq;    /* N_q x N_p: mapping nodes -> volume quadrature nodes */
+    const double* Vf;    /* N_f x N_p: mapping nodes -> facet quadrature nodes */
+    const double* nrstJ; /* N_f x d: scaled reference normals at the facet nodes */
+    const double* P_J;   /* N_q x N_q: V (V' W V)^-1 V' W applied to J_q, or NULL */
+    int32_t N_p1;        /* degree-(N+1) nodes (curl metrics on tetrahedra, else 0) */
+    const double* I1;    /* N_p1 x N_p: degree-N nodal -> degree-(N+1) nodal */
+    const double* D1;    /* N_p1 x N_p1 x 3: derivatives at the degree-(N+1) nodes */
+    const double* Vq1;   /* N_q x N_p1: Vq times the degree-(N+1) -> degree-N interpolation */
+    const double* Vf1;   /* N_f x N_p1 */
+} sse_geometry_ops;
+
+/* Replaces GeometricFactors(mesh, reference_element, metric_type)
+ * (src/SpatialDiscretizations/mesh.jl:229-282 exact, :284-339 curl on triangles, :410-506
+ * curl on tetrahedra) followed by project_jacobian! when P_J is given
+ * (SpatialDiscretizations.jl:310-317). x, y, z: N_p x N_e mapping-node coordinates (y, z
+ * NULL below d = 2, 3). Outputs: J_q (N_q, N_e), Lambda_q (N_q, d, d, N_e), J_f (N_f, N_e),
+ * nJf (d, N_f, N_e), nJq (d, N_fac, N_q, N_e). */
+int sse_geometric_factors(const sse_geometry_ops* ops, int32_t N_e, const double* x, const double* y,
+                          const double* z, double* J_q, double* Lambda_q, double* J_f, double* nJf,
+                          double* nJq, int32_t where);
+
+/* Replaces apply_reference_mapping (SpatialDiscretizations.jl:395-411):
+ * out[i, m, n, k] = sum_l Lambda_ref[i, m, l] Lambda_q[i, l, n, k] / J_ref[i].
+ * Lambda_ref (N_q, d, d) and J_ref (N_q) are host pointers. */
+int sse_apply_reference_mapping(int32_t N_q, int32_t d, int32_t N_e, const double* Lambda_ref,
+                                const double* J_ref, const double* Lambda_q, double* out, int32_t where);
+
+/* Replaces project_function with a modal V (src/Solvers/Solvers.jl:396-417): per element,
+ * u = (V' W J V)^-1 V' W J u_q by a Cholesky factorisation. V (N_q, N_p) and W (N_q) are host
+ * pointers; J_q (N_q, N_e), u_q (N_q, N_c, N_e) and u (N_p, N_c, N_e) live at `where`.
+ * SSE_ERR_INVALID_ARGUMENT names the first element whose matrix is not positive definite
+ * (the reference's cholesky throws there). */
+int sse_project_l2(int32_t N_q, int32_t N_p, int32_t N_c, int32_t N_e, const double* V, const double* W,
+                   const double* J_q, const double* u_q, double* u, int32_t where);
+
 #ifdef __cplusplus
 }
 #endif
