@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(SETUP_THREADS) k_geometric_factors(GeoArgs a) 
                 for (int q = 0; q < 4; ++q) sm[L.Cc + q * N_p + i] = Lm[q];
         }
         if (d == 3) {
-            // conservative curl form on the degree-(N+1) nodes (mesh.jl:448-484):
+            // conservative curl form on the degree-(N+1) nodes (mesh.jl:452-487):
             // J dr_l/dx = curl terms of (D y) z, J dr_l/dy of (D x) z, J dr_l/dz of (D y) x
             const int N1 = a.N_p1;
             for (int t = tt; t < 3 * N1; t += SETUP_THREADS) {

@@ -129,7 +129,7 @@ def _gf3d(x, y, z, Dr, Ds, Dt):
 
 def _tet_curl_operators(rd):
     """The degree-(N+1) nodal operators of the tetrahedral curl metrics
-    (``mesh.jl:418-447``): degree-N -> degree-(N+1) interpolation, the three
+    (``mesh.jl:416-433``): degree-N -> degree-(N+1) interpolation, the three
     derivative matrices at the degree-(N+1) nodes, and Vq, Vf composed with the
     degree-(N+1) -> degree-N interpolation."""
     N = rd.N

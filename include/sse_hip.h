@@ -291,7 +291,7 @@ typedef enum { SSE_METRICS_EXACT = 1, SSE_METRICS_CURL = 2 } sse_metrics;
 
 /* The RefElemData fields GeometricFactors reads (StartUpDG's RefElemData: Drst, Vq, Vf,
  * nrstJ), the Jacobian projector of project_jacobian!, and for conservative-curl metrics on
- * tetrahedra the degree-(N+1) nodal operators of mesh.jl:410-447. */
+ * tetrahedra the degree-(N+1) nodal operators of mesh.jl:416-433. */
 typedef struct sse_geometry_ops {
     int32_t d;           /* 1, 2 or 3 */
     int32_t metrics;     /* sse_metrics */
