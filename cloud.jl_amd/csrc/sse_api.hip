@@ -93,6 +93,7 @@ struct sse_handle {
     double* stage_dudt = nullptr;
     double* fpartial = nullptr;  // functionals: per-element partial sums and the result
     double* fout = nullptr;
+    double* ferr = nullptr;      // staging of u_exact for sse_l2_error with host arrays
     double *vjp_rbar = nullptr, *vjp_fbar = nullptr, *vjp_lam = nullptr, *vjp_lamout = nullptr;  // VJP scratch
     double* vjp_fext = nullptr;  // fbar in the neighbour's exterior slot (tensor VJP kernels)
     bool vjp_tensor = false;
@@ -124,6 +125,18 @@ int fail(sse_handle* h, int code, const std::string& msg) {
         if (_e != hipSuccess)                                                            \
             return fail((h), SSE_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
+
+template <typename T>
+int alloc(sse_handle* h, size_t n, T** out);
+
+// per-element partial sums and the result of sse_functional / sse_l2_error
+int functional_buffers(sse_handle* h) {
+    int rc;
+    if (!h->fpartial && ((rc = alloc(h, (size_t)h->P.N_e * (h->P.N_c + 1), &h->fpartial)) ||
+                         (rc = alloc(h, (size_t)h->P.N_c + 1, &h->fout))))
+        return rc;
+    return 0;
+}
 
 template <typename T>
 int upload(sse_handle* h, const std::vector<T>& host, const T** dev) {
@@ -968,6 +981,36 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     return 0;
 }
 
+int sse_l2_error(sse_handle* h, const double* u, const double* u_exact, int32_t normalize, int32_t where,
+                 double* out) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !u_exact || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u, u_exact or out is NULL");
+    if (where != SSE_DEVICE && where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
+    if (h->members != 1) return fail(h, SSE_ERR_UNSUPPORTED, "the error of an ensemble handle");
+    if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
+    const size_t n = state_size(h), nq = (size_t)h->P.N_q * h->P.N_c * h->P.N_e;
+    int rc;
+    if ((rc = functional_buffers(h))) return rc;
+    const double* ud = u;
+    const double* xd = u_exact;
+    if (where == SSE_HOST) {
+        if (!h->stage_u && ((rc = alloc(h, n, &h->stage_u)) || (rc = alloc(h, n, &h->stage_dudt)))) return rc;
+        if (!h->ferr && (rc = alloc(h, nq, &h->ferr))) return rc;
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_u, u, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->ferr, u_exact, nq * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ud = h->stage_u;
+        xd = h->ferr;
+    }
+    hipError_t e = sse::launch_functionals(h->P, ud, xd, 6, h->fpartial, h->fout, h->stream);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    const int nc = h->P.N_c;
+    std::vector<double> sums(nc + 1);
+    SSE_HIP_TRY(h, hipMemcpyAsync(sums.data(), h->fout, (nc + 1) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int c = 0; c < nc; ++c) out[c] = std::sqrt(normalize ? sums[c] / sums[nc] : sums[c]);
+    return 0;
+}
+
 int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t which, int32_t where, double* out) {
     if (!h) return SSE_ERR_INVALID_ARGUMENT;
     if (!u || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u or out is NULL");
@@ -976,9 +1019,7 @@ int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t w
     if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
     const size_t n = state_size(h);
     int rc;
-    if (!h->fpartial && ((rc = alloc(h, (size_t)h->P.N_e * h->P.N_c, &h->fpartial)) ||
-                         (rc = alloc(h, (size_t)h->P.N_c, &h->fout))))
-        return rc;
+    if ((rc = functional_buffers(h))) return rc;
     const double* ud = u;
     const double* vd = dudt;
     if (where == SSE_HOST) {

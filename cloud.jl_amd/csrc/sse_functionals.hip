@@ -7,6 +7,10 @@
 //   mass       sum_k 1' WJ_k V dudt_k
 //   energy     sum_k u_k' M_k dudt_k
 //   entropy    sum_k (P_k w_q)' M_k dudt_k = sum_k w_q' WJ_k V dudt_k,  P_k = M_k^{-1} V' WJ_k
+// and the squared L2 error against exact values at the volume quadrature nodes
+// (the default ErrorAnalysis, src/Analysis/error.jl:13-57, 58-91):
+//   error      sum_k sum_i W_i J_k,i (u_exact_k,i - (V u_k)_i)^2   (per variable)
+//   volume     sum_k sum_i W_i J_k,i                                (for normalize)
 // M_k is the mass matrix of the handle's solver: (V' diag(W / J_k) V)^{-1} for the
 // weight-adjusted solver with M^{-1} = I (src/Solvers/mass_matrix.jl:150-153), W J_k
 // for the diagonal one. One workgroup per element writes its partial sums; a
@@ -20,6 +24,7 @@ namespace sse {
 namespace {
 
 constexpr int FT = 128;  // threads per element
+constexpr int FUNC_L2_ERROR = 6;
 
 __device__ double block_sum(double v, double* red) {
     red[threadIdx.x] = v;
@@ -50,8 +55,9 @@ __device__ double entropy_of(const LawParams& P, const double* u) {
 
 }  // namespace
 
-// which: 0 mass, 1 energy, 2 entropy, 3 mass rate, 4 energy rate, 5 entropy rate.
-// partial has N_e x ncomp entries.
+// which: 0 mass, 1 energy, 2 entropy, 3 mass rate, 4 energy rate, 5 entropy rate,
+// 6 squared L2 error and volume (dudt then holds u_exact, N_q x N_c per element).
+// partial has N_e x ncomp entries, ncomp = N_c (N_c + 1 for the error).
 template <int D, int LAW>
 __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* __restrict__ u,
                                                     const double* __restrict__ dudt, int which,
@@ -70,13 +76,43 @@ __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* _
     double* s_y = s_M + Np * Np;     // Np x NC
     const double* Jk = p.Jq + (size_t)k * Nq;
     const size_t base = (size_t)k * Np * NC;
-    const bool rate = which >= 3;
+    const bool error = which == FUNC_L2_ERROR;
+    const bool rate = which >= 3 && !error;
     const int kind = which % 3;
     for (int t = threadIdx.x; t < Np * NC; t += FT) {
         s_u[t] = u[base + t];
         s_v[t] = rate ? dudt[base + t] : 0.0;
     }
     __syncthreads();
+    if (error) {
+        for (int i = threadIdx.x; i < Nq; i += FT)
+            for (int e = 0; e < NC; ++e) {
+                double a = 0.0;
+                if (p.has_V) {
+                    for (int j = 0; j < Np; ++j) a += p.V[i + (size_t)Nq * j] * s_u[j + Np * e];
+                } else {
+                    a = s_u[i + Np * e];
+                }
+                s_uq[i + Nq * e] = a;
+            }
+        __syncthreads();
+        const double* ex = dudt + (size_t)k * Nq * NC;
+        double* out = partial + (size_t)k * (NC + 1);
+        for (int e = 0; e < NC; ++e) {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < Nq; i += FT) {
+                const double r = ex[i + Nq * e] - s_uq[i + Nq * e];
+                s += p.W[i] * Jk[i] * r * r;
+            }
+            const double tot = block_sum(s, red);
+            if (threadIdx.x == 0) out[e] = tot;
+        }
+        double s = 0.0;
+        for (int i = threadIdx.x; i < Nq; i += FT) s += p.W[i] * Jk[i];
+        const double tot = block_sum(s, red);
+        if (threadIdx.x == 0) out[NC] = tot;
+        return;
+    }
     double* out = partial + (size_t)k * NC;
     if (kind != 1) {  // mass and entropy work at the volume nodes
         for (int i = threadIdx.x; i < Nq; i += FT)
@@ -207,7 +243,8 @@ hipError_t launch_functionals_t(const DevParams& p, const double* u, const doubl
                        partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FT), 0, s, partial, p.N_e, p.N_c, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FT), 0, s, partial, p.N_e,
+                       which == FUNC_L2_ERROR ? p.N_c + 1 : p.N_c, out);
     return hipGetLastError();
 }
 

@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = (
     "sse_synchronize", "sse_last_error", "sse_destroy", "sse_abi_version", "sse_describe",
     "sse_batch_create", "sse_batch_rk_steps", "sse_batch_describe", "sse_batch_last_error",
     "sse_batch_destroy", "sse_functional", "sse_residual_vjp", "sse_rk_adjoint", "sse_batch_rk_tangent",
-    "sse_geometric_factors", "sse_apply_reference_mapping", "sse_project_l2",
+    "sse_geometric_factors", "sse_apply_reference_mapping", "sse_project_l2", "sse_l2_error",
 )
 
 SSE_METRICS_EXACT, SSE_METRICS_CURL = 1, 2

@@ -79,6 +79,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_apply_reference_mapping.restype = C.c_int
     lib.sse_project_l2.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.c_int32]
     lib.sse_project_l2.restype = C.c_int
+    lib.sse_l2_error.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, dp]
+    lib.sse_l2_error.restype = C.c_int
     lib.sse_abi_version.argtypes = []
     lib.sse_abi_version.restype = C.c_int32
     for name in EXPORTED_SYMBOLS:

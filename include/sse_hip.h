@@ -210,6 +210,16 @@ typedef enum {
  * Blocks until the values are in out. */
 int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t which, int32_t where, double* out);
 
+/* Replaces analyze(ErrorAnalysis(results_path, law, spatial_discretization), sol,
+ * exact_solution, t; normalize) with the default error quadrature, the volume quadrature
+ * (src/Analysis/error.jl:13-57, 58-91): out[e] = sqrt(sum_k sum_i W_i J_k,i
+ * (u_exact[i, e, k] - (V u_k)_{i,e})^2), the sum divided by the total volume sum W J
+ * inside the square root when normalize != 0. u is (N_p, N_c, N_e); u_exact is
+ * (N_q, N_c, N_e), the exact solution evaluated at mesh.xyzq at time t; both at `where`.
+ * out (host) receives N_c values. Blocks until they are there. */
+int sse_l2_error(sse_handle* h, const double* u, const double* u_exact, int32_t normalize, int32_t where,
+                 double* out);
+
 /* Uses the given hipStream_t (0 = the handle's own stream) for later calls. */
 int sse_set_stream(sse_handle* h, void* hip_stream);
 
