@@ -76,6 +76,15 @@ namespace {
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
+// waves per SIMD that `lds_doubles` of LDS per workgroup of `threads` allow (160 KB
+// per CU, 4 SIMDs), capped at 7: the register budget a kernel is held to, so that
+// the compiler does not trade the occupancy the LDS allows for early loads
+constexpr int lds_waves(int lds_doubles, int threads) {
+    const int wg = (160 * 1024) / (8 * lds_doubles);
+    const int w = wg * (threads / 64) / 4;
+    return w < 1 ? 1 : (w > 7 ? 7 : w);
+}
+
 // Node i of the tensor grid has coordinate (i / stride_l) % N1 in direction l
 // (the last direction fastest).
 template <int ELEM, int P>
@@ -187,7 +196,7 @@ struct StdAuxLayout {
     using Sh = Shape<ELEM, P>;
     static constexpr int E = StdTune<ELEM, P>::E2, D = Sh::D;
     static constexpr int A = cmax(D * Sh::NQ, D * (Sh::NZ + Sh::NP));  // D1_l u, then SumFact Z and y
-    static constexpr int B = cmax(D * D * Sh::NQ, D * Sh::NW);         // g_ln, then SumFact W
+    static constexpr int B = cmax(D * Sh::NQ, D * Sh::NW);             // g_ln of one n, then SumFact W
     // u^- only for the standard mapping
     static constexpr int uq = 0, up = uq + E * Sh::NQ, um = up + E * Sh::NF, a = um + (SKEW ? 0 : E * Sh::NF);
     static constexpr int b = a + E * A, r = b + E * B, tf = r + E * D * Sh::NQ;
@@ -196,7 +205,8 @@ struct StdAuxLayout {
 };
 
 template <int ELEM, int P, bool SKEW>
-__global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(TensorParams tp, Ensemble en) {
+__global__ void __launch_bounds__((StdTune<ELEM, P>::TS), (lds_waves(StdAuxLayout<ELEM, P, SKEW>::size, StdTune<ELEM, P>::TS)))
+    k_std_aux_tensor(TensorParams tp, Ensemble en) {
     using Sh = Shape<ELEM, P>;
     using T = StdTune<ELEM, P>;
     using L = StdAuxLayout<ELEM, P, SKEW>;
@@ -251,29 +261,6 @@ __global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(Tenso
         }
         __syncthreads();
     }
-    // g_ln = h W Lt_ln u_q; r_n starts as h W sum_l Lt_ln (D1_l u_q)
-    for (int t = tl; t < E * NQ; t += TS) {
-        const int s = t / NQ, i = t - s * NQ;
-        const long long k = (g0 + (s < nvalid ? s : 0)) / R;
-        const double* Lk = p.Lam + (size_t)k * NQ * D * D;
-        const double hw = h * p.W[i];
-        const double x = suq[t];
-        double lam[D][D];
-#pragma unroll
-        for (int n = 0; n < D; ++n)
-#pragma unroll
-            for (int l = 0; l < D; ++l) lam[l][n] = Lk[i + NQ * (l + D * n)];
-#pragma unroll
-        for (int n = 0; n < D; ++n) {
-            double dterm = 0.0;
-#pragma unroll
-            for (int l = 0; l < D; ++l) {
-                sg[((s * D + l) * D + n) * NQ + i] = hw * lam[l][n] * x;
-                if constexpr (SKEW) dterm += lam[l][n] * sDu[(s * D + l) * NQ + i];
-            }
-            sr[(s * D + n) * NQ + i] = hw * dterm;
-        }
-    }
     // facet: 1/2 B nJf_n u^+ (skew) or 1/2 B nJf_n (u^- + u^+)
     for (int t = tl; t < E * NF; t += TS) {
         const int s = t / NF, j = t - s * NF;
@@ -283,24 +270,48 @@ __global__ void __launch_bounds__((StdTune<ELEM, P>::TS)) k_std_aux_tensor(Tenso
 #pragma unroll
         for (int n = 0; n < D; ++n) stf[(s * D + n) * NF + j] = nJ[n] * x;
     }
-    __syncthreads();
-    // r_n -= sum_l D1_l' g_ln;  r_n += R' tf_n
-    for (int i = tl; i < NQ; i += TS) {
-        double acc[E * D];
+    // one direction n at a time (three g_ln arrays in LDS instead of nine):
+    //   g_ln = h W Lt_ln u_q;  r_n = h W sum_l Lt_ln (D1_l u_q) - sum_l D1_l' g_ln + R' tf_n
+    // node i's sums stay in registers across the passes
+    static_assert(NQ <= TS, "one volume node per thread");
+    double acc[E * D];
 #pragma unroll
-        for (int c = 0; c < E * D; ++c) acc[c] = 0.0;
+    for (int c = 0; c < E * D; ++c) acc[c] = 0.0;
 #pragma unroll 1
-        for (int l = 0; l < D; ++l) {
-            const int sl = LN::stride(l), a = LN::coord(i, l), base = i - a * sl;
+    for (int n = 0; n < D; ++n) {
+        for (int t = tl; t < E * NQ; t += TS) {
+            const int s = t / NQ, i = t - s * NQ;
+            const long long k = (g0 + (s < nvalid ? s : 0)) / R;
+            const double* Lk = p.Lam + (size_t)k * NQ * D * D;
+            const double hw = h * p.W[i];
+            const double x = suq[t];
+            double dterm = 0.0;
 #pragma unroll
-            for (int bb = 0; bb < N1; ++bb) {
-                const double dv = sD1[(l * N1 + bb) * N1 + a];
+            for (int l = 0; l < D; ++l) {
+                const double lam = Lk[i + NQ * (l + D * n)];
+                sg[(s * D + l) * NQ + i] = hw * lam * x;
+                if constexpr (SKEW) dterm += lam * sDu[(s * D + l) * NQ + i];
+            }
+            sr[(s * D + n) * NQ + i] = hw * dterm;
+        }
+        __syncthreads();
+        if (tl < NQ) {
+            const int i = tl;
+#pragma unroll 1
+            for (int l = 0; l < D; ++l) {
+                const int sl = LN::stride(l), a = LN::coord(i, l), base = i - a * sl;
 #pragma unroll
-                for (int s = 0; s < E; ++s)
+                for (int bb = 0; bb < N1; ++bb) {
+                    const double dv = sD1[(l * N1 + bb) * N1 + a];
 #pragma unroll
-                    for (int n = 0; n < D; ++n) acc[s * D + n] -= dv * sg[((s * D + l) * D + n) * NQ + base + bb * sl];
+                    for (int s = 0; s < E; ++s) acc[s * D + n] -= dv * sg[(s * D + l) * NQ + base + bb * sl];
+                }
             }
         }
+        __syncthreads();
+    }
+    if (tl < NQ) {
+        const int i = tl;
         ell_apply<E * D, NF>(tp.rt_idx, tp.rt_val, tp.wrt, NQ, i, stf, acc);
 #pragma unroll
         for (int c = 0; c < E * D; ++c) sr[c * NQ + i] += acc[c];
