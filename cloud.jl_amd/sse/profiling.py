@@ -38,7 +38,7 @@ SLOTS = {
 
 
 def short_name(kernel_name: str) -> str:
-    name = kernel_name.split("(")[0]
+    name = kernel_name.replace("(anonymous namespace)::", "").split("(")[0]
     return name.replace("void ", "").replace("sse::", "")
 
 
