@@ -391,8 +391,34 @@ def advdiff_sweep(args, law, solver, u):
         fd_errs.append(abs(fd - grads[i]) / abs(grads[i]))
     log(f"tangent: {t_grad:.2f} s ({t_grad / t_sweep:.2f}x the sweep); dJ/db in [{min(grads):.4e}, {max(grads):.4e}]; "
         f"FD rel err {max(fd_errs):.2e}")
+    # the same gradient by the discrete adjoint (reverse sweep on the forward kernels
+    # with the velocity reversed) on a few members, one at a time
+    from sse.adjoint import energy_adjoint_b, energy_adjoint_helpers
+    A_n = min(args.ad_adjoint, R)
+    adj_errs, t_adj = [], 0.0
+    helpers = energy_adjoint_helpers(solver) if A_n else None
+    if A_n:  # handles, and the mass-matrix factors the first energy functional keeps
+        energy_adjoint_b(members[0], u, dt, 1, helpers)
+    for i in range(A_n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, g_adj, _ = energy_adjoint_b(members[i], u, dt, n, helpers)
+        torch.cuda.synchronize()
+        t_adj += time.perf_counter() - t0
+        adj_errs.append(abs(g_adj - grads[i]) / abs(grads[i]))
+    if helpers:
+        for h_ in helpers:
+            h_.close()
+    adjoint = None
+    if A_n:
+        per_member_fwd = loop_total / R
+        adjoint = {"members": A_n, "s_per_member": t_adj / A_n,
+                   "over_forward_solve": t_adj / A_n / per_member_fwd,
+                   "max_rel_diff_vs_tangent": max(adj_errs)}
+        log(f"adjoint: {t_adj / A_n:.2f} s per member ({adjoint['over_forward_solve']:.2f}x its forward solve); "
+            f"dJ/db vs tangent max rel diff {max(adj_errs):.2e}")
     batch.close()
-    return {"members": R, "steps": n, "dt": dt, "b_range": [1e-3, 1e-2],
+    return {"members": R, "steps": n, "dt": dt, "b_range": [1e-3, 1e-2], "adjoint": adjoint,
             "sweep_s": t_sweep, "dof_steps_per_s": dof * R * n / t_sweep,
             "loop_s_extrapolated": loop_total, "loop_members_timed": L, "speedup_vs_loop": loop_total / t_sweep,
             "energy_0": J0, "energy_N_range": [min(JN), max(JN)],
@@ -420,6 +446,7 @@ def main():
     ap.add_argument("--ad-steps", type=int, default=10)
     ap.add_argument("--ad-dt", type=float, default=1e-4)
     ap.add_argument("--ad-loop", type=int, default=4, help="members timed as plain per-problem solves")
+    ap.add_argument("--ad-adjoint", type=int, default=2, help="members whose dJ/db is also taken by the adjoint")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-directions", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=200)

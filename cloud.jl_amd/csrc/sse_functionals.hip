@@ -13,8 +13,10 @@
 //   volume     sum_k sum_i W_i J_k,i                                (for normalize)
 // M_k is the mass matrix of the handle's solver: (V' diag(W / J_k) V)^{-1} for the
 // weight-adjusted solver with M^{-1} = I (src/Solvers/mass_matrix.jl:150-153), W J_k
-// for the diagonal one. One workgroup per element writes its partial sums; a
-// second kernel adds them in a fixed order (deterministic, no atomics).
+// for the diagonal one. The energy kinds solve with the Cholesky factor of M_k^{-1};
+// the caller may keep the factors of every element (`chol`) so that later calls skip
+// forming and factoring M_k^{-1}. One workgroup per element writes its partial sums;
+// a second kernel adds them in a fixed order (deterministic, no atomics).
 #include <hip/hip_runtime.h>
 
 #include "sse_kernels.h"
@@ -61,7 +63,8 @@ __device__ double entropy_of(const LawParams& P, const double* u) {
 template <int D, int LAW>
 __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* __restrict__ u,
                                                     const double* __restrict__ dudt, int which,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, double* __restrict__ chol,
+                                                    int chol_ready) {
     using Ph = Physics<D, LAW>;
     constexpr int NC = Ph::NC;
     extern __shared__ double sm[];
@@ -168,6 +171,12 @@ __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* _
         return;
     }
     // energy: y = M_k x by solving (M_k^{-1}) y = x, x = u (energy) or dudt (rate)
+    double* ck = chol ? chol + (size_t)k * Np * Np : nullptr;
+    if (ck && chol_ready) {  // the factor of M_k^{-1} from an earlier call
+        for (int t = threadIdx.x; t < Np * Np; t += FT) s_M[t] = ck[t];
+        for (int t = threadIdx.x; t < Np * NC; t += FT) s_y[t] = rate ? s_v[t] : s_u[t];
+        __syncthreads();
+    } else {
     if (p.has_V) {
         for (int t = threadIdx.x; t < Np * Np; t += FT) {  // M_k^{-1} = V' diag(W/J) V
             const int a = t % Np, b = t / Np;
@@ -195,6 +204,9 @@ __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* _
             if (b <= a) s_M[a + Np * b] -= s_M[a + Np * j] * s_M[b + Np * j];
         }
         __syncthreads();
+    }
+    if (ck)
+        for (int t = threadIdx.x; t < Np * Np; t += FT) ck[t] = s_M[t];
     }
     // forward and back substitution, one variable per thread
     for (int e = threadIdx.x; e < NC; e += FT) {
@@ -238,9 +250,9 @@ size_t lds_functionals(const DevParams& p) {
 
 template <int D, int LAW>
 hipError_t launch_functionals_t(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
-                                double* out, hipStream_t s) {
+                                double* out, hipStream_t s, double* chol, int chol_ready) {
     hipLaunchKernelGGL((k_functionals<D, LAW>), dim3(p.N_e), dim3(FT), lds_functionals(p), s, p, u, dudt, which,
-                       partial);
+                       partial, chol, chol_ready);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FT), 0, s, partial, p.N_e,
@@ -249,15 +261,15 @@ hipError_t launch_functionals_t(const DevParams& p, const double* u, const doubl
 }
 
 hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
-                              double* out, hipStream_t s) {
+                              double* out, hipStream_t s, double* chol, int chol_ready) {
     const bool euler = (p.N_c != 1);
     switch (p.d) {
-        case 1: return euler ? launch_functionals_t<1, LAW_EULER>(p, u, dudt, which, partial, out, s)
-                             : launch_functionals_t<1, LAW_ADVECTION>(p, u, dudt, which, partial, out, s);
-        case 2: return euler ? launch_functionals_t<2, LAW_EULER>(p, u, dudt, which, partial, out, s)
-                             : launch_functionals_t<2, LAW_ADVECTION>(p, u, dudt, which, partial, out, s);
-        case 3: return euler ? launch_functionals_t<3, LAW_EULER>(p, u, dudt, which, partial, out, s)
-                             : launch_functionals_t<3, LAW_ADVECTION>(p, u, dudt, which, partial, out, s);
+        case 1: return euler ? launch_functionals_t<1, LAW_EULER>(p, u, dudt, which, partial, out, s, chol, chol_ready)
+                             : launch_functionals_t<1, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, chol, chol_ready);
+        case 2: return euler ? launch_functionals_t<2, LAW_EULER>(p, u, dudt, which, partial, out, s, chol, chol_ready)
+                             : launch_functionals_t<2, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, chol, chol_ready);
+        case 3: return euler ? launch_functionals_t<3, LAW_EULER>(p, u, dudt, which, partial, out, s, chol, chol_ready)
+                             : launch_functionals_t<3, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, chol, chol_ready);
         default: return hipErrorInvalidValue;
     }
 }

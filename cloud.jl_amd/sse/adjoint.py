@@ -10,7 +10,10 @@ a ``ChainRulesCore.rrule`` of ``semi_discrete_residual!`` (the analogue of a
 * ``rrule_semi_discrete_residual(solver, u)`` -> (dudt, pullback);
 * ``rk_adjoint(solver, u0, dt, nsteps, checkpoint_every)`` -> (J, dJ/du0, dJ/dλ)
   for the entropy integral J of the final CK2N54 state, with checkpointing and
-  one recomputation per segment (``sse_rk_adjoint``).
+  one recomputation per segment (``sse_rk_adjoint``);
+* ``energy_adjoint_b(solver, u0, dt, nsteps)`` -> (J, dJ/db, M^{-1} dJ/du0) for
+  linear advection-diffusion and J = 1/2 u_N' M u_N, by the discrete adjoint of
+  the CK2N54 solve run on the forward kernels (see its docstring).
 """
 
 from __future__ import annotations
@@ -81,3 +84,89 @@ def rk_adjoint(solver: Solver, u0, dt: float, nsteps: int, checkpoint_every: int
     check(lib, h, lib.sse_rk_adjoint(h, pu, float(dt), int(nsteps), int(checkpoint_every), abi.SSE_ENTROPY,
                                      C.byref(J), pg, C.byref(lam), where))
     return J.value, grad, lam.value
+
+
+def energy_adjoint_helpers(solver: Solver):
+    """The three solvers ``energy_adjoint_b`` runs, the same for every diffusion
+    coefficient on this mesh (reuse them across a sweep): the velocity a and the
+    velocity reversed, both at b = 0, and a = 0 at b = 1 (the viscous operator dR/db)."""
+    from .conservation_laws import LinearAdvectionDiffusionEquation
+    law = solver.conservation_law
+    if not isinstance(law, LinearAdvectionDiffusionEquation):
+        raise NotImplementedError("the velocity-reversal adjoint is for linear advection-diffusion")
+    return (solver.with_law(LinearAdvectionDiffusionEquation(tuple(law.a), 0.0)),
+            solver.with_law(LinearAdvectionDiffusionEquation(tuple(-x for x in law.a), 0.0)),
+            solver.with_law(LinearAdvectionDiffusionEquation(tuple(0.0 for _ in law.a), 1.0)))
+
+
+def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
+    """Discrete adjoint of ``nsteps`` CK2N54 steps of the linear advection-diffusion
+    residual R(u) = A(a, b) u (StandardForm, skew-symmetric mapping, LF, BR1) for
+    J = 1/2 u_N' M u_N, M the solver's mass matrix. Returns (J, dJ/db, g) with
+    g = M^{-1} dJ/du0 (the gradient as a state: dJ/du0 . du = <g, du>_M).
+
+    The weak-form operator M A splits into skew-symmetric advection and symmetric
+    LF and BR1 terms, so (M A(a, b))' = M A(-a, b): the transposed residual is the
+    residual with the velocity reversed, conjugated by M. In the variables
+    u~ = M^{-1} ubar, k~ = M^{-1} kbar the reverse of a stage (k = A_s k + dt R(u_s);
+    u = u + B_s k) is
+        k~ += B_s u~;  u~ += dt R_{-a}(k~);  dJ/db += dt <k~, dR/db(u_s)>_M;  k~ *= A_s
+    with R = A(a, 0) + b dR/db, R_{-a} = A(-a, 0) + b dR/db and dR/db(u) = R(u; a = 0,
+    b = 1) (R is affine in b), evaluated by the three solvers of
+    ``energy_adjoint_helpers`` (pass them to reuse them across members; `solver` only
+    supplies a and b), and <x, y>_M from ``sse_functional``. u~ starts at u_N.
+    Every operator application is a forward residual on the GPU; step states are
+    kept and each step's stage states are recomputed once. `u0` is a device tensor
+    (N_e, 1, N_p).
+
+    Each solver's handle has its own stream, ordered against torch's default stream
+    but not against the other handles' streams; the functionals synchronise the
+    device before they read."""
+    import torch
+
+    from .analysis import EnergyConservationAnalysis
+    from .time_integration import CarpenterKennedy2N54 as CK
+    own = helpers is None
+    fwd0, rev0, visc = energy_adjoint_helpers(solver) if own else helpers
+    b = float(solver.conservation_law.b)
+    energy = EnergyConservationAnalysis(visc)  # the mass matrix is the same for every law on the mesh
+    r = torch.empty_like(u0)
+
+    def step(u, keep=None):
+        """One step; with `keep`, the viscous terms dR/db(u_s) of the stage states."""
+        k = torch.zeros_like(u)
+        for s in range(5):
+            dv = keep[s] if keep is not None else r
+            semi_discrete_residual(dv, u, visc)
+            k.mul_(CK.A[s]).add_(dv, alpha=dt * b)
+            semi_discrete_residual(r, u, fwd0)
+            k.add_(r, alpha=dt)
+            u.add_(k, alpha=CK.B[s])
+
+    try:
+        states = [u0.clone()]
+        u = u0.clone()
+        for _ in range(nsteps):
+            step(u)
+            states.append(u.clone())
+        J = float(energy.evaluate_conservation(u)[0])
+        ut = u.clone()
+        kt = torch.empty_like(u)
+        dvis = [torch.empty_like(u) for _ in range(5)]
+        dJdb = 0.0
+        for n in range(nsteps - 1, -1, -1):
+            step(states[n].clone(), dvis)
+            kt.zero_()
+            for s in range(4, -1, -1):
+                kt.add_(ut, alpha=CK.B[s])
+                semi_discrete_residual(r, kt, rev0)
+                ut.add_(r, alpha=dt)
+                semi_discrete_residual(r, kt, visc)
+                ut.add_(r, alpha=dt * b)
+                dJdb += dt * float(energy.evaluate_conservation_residual(kt, dvis[s])[0])
+                kt.mul_(CK.A[s])
+        return J, dJdb, ut
+    finally:
+        if own:
+            for h in (fwd0, rev0, visc):
+                h.close()

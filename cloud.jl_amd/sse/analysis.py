@@ -43,6 +43,8 @@ class _Analysis:
             code = lib.sse_functional(h, u.ctypes.data, None if du is None else du.ctypes.data, self.kind,
                                       abi.SSE_HOST, out.ctypes.data_as(C.POINTER(C.c_double)))
         else:
+            import torch
+            torch.cuda.synchronize()  # inputs may come from another handle's stream; the call blocks anyway
             code = lib.sse_functional(h, u.data_ptr(), None if dudt is None else dudt.data_ptr(), self.kind,
                                       abi.SSE_DEVICE, out.ctypes.data_as(C.POINTER(C.c_double)))
         check(lib, h, code)
@@ -108,8 +110,11 @@ def _l2_error(analysis: ErrorAnalysis, sol, exact_solution, t: float, normalize:
         code = lib.sse_l2_error(h, u.ctypes.data, ue.ctypes.data, int(normalize), abi.SSE_HOST,
                                 out.ctypes.data_as(C.POINTER(C.c_double)))
     else:
+        import torch
+
         from .time_integration import to_device
         ued = to_device(ue)
+        torch.cuda.synchronize()  # sol may come from another handle's stream; the call blocks anyway
         code = lib.sse_l2_error(h, sol.data_ptr(), ued.data_ptr(), int(normalize), abi.SSE_DEVICE,
                                 out.ctypes.data_as(C.POINTER(C.c_double)))
     check(lib, h, code)

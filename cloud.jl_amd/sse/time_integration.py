@@ -17,7 +17,14 @@ from .solvers import ODEProblem, Solver
 
 
 class CarpenterKennedy2N54:
-    """Five-stage, fourth-order, 2N-storage RK of Carpenter & Kennedy (1994)."""
+    """Five-stage, fourth-order, 2N-storage RK of Carpenter & Kennedy (1994), NASA
+    TM-109112 (the coefficients of OrdinaryDiffEq's ``CarpenterKennedy2N54``): stage s
+    sets k = A[s] k + dt R(u), then u = u + B[s] k. The kernels hold the same table."""
+    A = (0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+         -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0)
+    B = (1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+         1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+         2277821191437.0 / 14882151754819.0)
 
 
 def to_device(u: np.ndarray):

@@ -546,3 +546,27 @@ def test_rk_adjoint_matches_oracle_forward_mode():
     dJ_fwd = float(np.einsum("qk,qkc,qp,pck->", WJ, wq, V, duN))
     dJ_adj = float(np.sum(g * du)) + glam * dlam
     assert abs(dJ_fwd - dJ_adj) <= 1e-10 * (abs(dJ_fwd) + 1e-3), (dJ_fwd, dJ_adj)
+
+
+def test_energy_adjoint_in_b_matches_oracle_forward_mode():
+    # the discrete adjoint of the BR1 solve (run on the forward kernels with the
+    # velocity reversed) against the oracle's forward-mode duals: dJ/db for
+    # J = 1/2 u_N' M u_N, and the gradient state through a dot-product test
+    from oracle import functionals as F
+    from sse.adjoint import energy_adjoint_b
+    from sse.time_integration import to_device, to_host
+    solvers, u0s = _advdiff_ensemble(3, (4e-3,))
+    s, u0 = solvers[0], u0s[0]
+    dt, n = 1e-3, 6
+    J, dJdb, g = energy_adjoint_b(s, to_device(u0), dt, n)
+    uN, duN = O.rk_jvp(s.arrays, u0, np.zeros_like(u0), dt, n, db=1.0)
+    sd = s.spatial_discretization
+    assert abs(J - 0.5 * F.energy_rate(sd, uN, uN)[0]) <= 1e-12 * abs(J)
+    want = F.energy_rate(sd, uN, duN)[0]
+    assert abs(dJdb - want) <= 1e-10 * abs(want), (dJdb, want)
+    rng = np.random.default_rng(11)
+    du0 = np.asfortranarray(rng.standard_normal(u0.shape))
+    uN2, dN = O.rk_jvp(s.arrays, u0, du0, dt, n, db=0.0)
+    lhs = F.energy_rate(sd, to_host(g), du0)[0]   # <g, du0>_M = dJ/du0 . du0
+    rhs = F.energy_rate(sd, uN2, dN)[0]           # u_N' M (dU_N/du0 du0)
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs), (lhs, rhs)

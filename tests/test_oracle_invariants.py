@@ -423,3 +423,38 @@ def test_reference_scaling_benchmark_problem():
     mass = np.einsum("qk,qp,pck->c", WJ, ref.Vdense(), dudt)
     scale = np.einsum("qk,qp,pck->c", WJ, np.abs(ref.Vdense()), np.abs(dudt))
     assert np.all(np.abs(mass) <= 1e-13 * scale), (mass, scale)
+
+
+def test_br1_weak_form_transpose_is_the_velocity_reversed_weak_form():
+    """(M A(a, b))' = M A(-a, b) for the skew-symmetric StandardForm with LF and BR1 on
+    curved tets: advection is skew in M, LF and BR1 symmetric. This is what the
+    velocity-reversal adjoint of the advection-diffusion solve rests on; a planted
+    error (the LF coefficient changed on one side) must break it."""
+    from sse.conservation_laws import LinearAdvectionDiffusionEquation
+    from sse.workloads import advection_diffusion_3d_config
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=1, p=2)
+    s = ode.p
+    shape = ode.u0.shape
+    ref = sd.reference_approximation
+    V, W, Jq = ref.V, ref.W, sd.geometric_factors.J_q
+    N_p, _, N_e = shape
+    n = N_p * N_e
+    M = np.zeros((n, n))
+    for k in range(N_e):
+        M[k * N_p:(k + 1) * N_p, k * N_p:(k + 1) * N_p] = np.linalg.inv(V.T @ ((W / Jq[:, k])[:, None] * V))
+
+    def weak(arrays):
+        A = np.empty((n, n))
+        for i in range(n):
+            e = np.zeros(n)
+            e[i] = 1.0
+            A[:, i] = O.residual(arrays, np.asfortranarray(e.reshape(shape, order="F"))).reshape(-1, order="F")
+        return M @ A
+
+    B = weak(s.arrays)
+    rev = s.with_law(LinearAdvectionDiffusionEquation(tuple(-x for x in law.a), law.b))
+    Brev = weak(rev.arrays)
+    assert np.abs(B.T - Brev).max() <= 1e-12 * np.abs(B).max()
+    planted = rev.arrays
+    planted.desc.half_lambda = 0.5 * planted.desc.half_lambda
+    assert np.abs(B.T - weak(planted)).max() > 1e-6 * np.abs(B).max()
