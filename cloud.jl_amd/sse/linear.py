@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 
 from .conservation_laws import LinearAdvectionDiffusionEquation, LinearAdvectionEquation
-from .solvers import Solver, semi_discrete_residual
+from .solvers import SkewSymmetricMapping, Solver, StandardForm, semi_discrete_residual
 
 
 class LinearResidual:
@@ -31,3 +31,18 @@ class LinearResidual:
 
     def __matmul__(self, x):
         return self.matvec(x)
+
+    def adjoint(self) -> "LinearResidual":
+        """The adjoint in the mass-matrix inner product, M^{-1} R' M, as a residual. For
+        the skew-symmetric StandardForm of linear advection(-diffusion) it is the
+        residual with the velocity reversed: advection is skew in M and the LF and BR1
+        terms are symmetric, so <v, R w>_M = <R_{-a} v, w>_M. This is the operator of
+        the discrete adjoint in ``adjoint.energy_adjoint_b``."""
+        form = self.solver.form
+        if not (isinstance(form, StandardForm) and isinstance(form.mapping_form, SkewSymmetricMapping)):
+            raise NotImplementedError("the velocity-reversal adjoint needs the skew-symmetric StandardForm")
+        law = self.solver.conservation_law
+        a = tuple(-x for x in law.a)
+        rev = (LinearAdvectionDiffusionEquation(a, law.b) if isinstance(law, LinearAdvectionDiffusionEquation)
+               else LinearAdvectionEquation(a))
+        return LinearResidual(self.solver.with_law(rev))

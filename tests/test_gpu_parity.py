@@ -410,6 +410,26 @@ def test_linear_residual_is_linear_and_skew_adjoint():
     assert abs(lhs - rhs) < 1e-11 * (abs(lhs) + 1.0), (lhs, rhs)
 
 
+def test_linear_residual_adjoint_dot_product():
+    # <v, R w>_M = <R* v, w>_M with R* the velocity-reversed residual, for BR1
+    # advection-diffusion with LF on curved tets (the operator of the adjoint in b)
+    from oracle import functionals as F
+    from sse.linear import LinearResidual
+    law, sd, ex, form, ode = advection_diffusion_3d_config(M=2, p=3)
+    L = LinearResidual(ode.p)
+    La = L.adjoint()
+    rng = np.random.default_rng(5)
+    v, w = rng.standard_normal(L.shape[0]), rng.standard_normal(L.shape[0])
+    shape = ode.p.size()
+
+    def minner(a, b):
+        return F.energy_rate(sd, np.asfortranarray(a.reshape(shape, order="F")),
+                             np.asfortranarray(b.reshape(shape, order="F")))[0]
+
+    lhs, rhs = minner(v, L @ w), minner(La @ v, w)
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+
+
 # ----------------------------------------------------------------------------- adjoint
 def _ec_interface(mk):
     from sse.conservation_laws import EntropyConservativeNumericalFlux
