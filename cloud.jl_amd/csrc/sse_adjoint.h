@@ -130,48 +130,61 @@ struct EulerAdj {
         h[H_IP] = ip;
     }
 
-    // ec_dot_adj_side from hoisted states: adds the pull-back of Gb through
-    // G = sum_n a_n F_n(L, R) into the kept side's conservative cotangent ub.
-    __device__ static void ec_dot_adj_h(const LawParams& P, const double* hL, const double* hR, const double* a,
-                                        const double* Gb, bool left, double* ub) {
-        double Vb[D];
-        double vv = 0.0, av = 0.0, avL = 0.0, avR = 0.0;
+    // The parts of the pull-back of Gb through G = sum_n a_n F_n(L, R) (hoisted
+    // states) that both sides share: the averages, the log-means and their gradients.
+    struct EcAdjShared {
+        double f, rhoavgb, avb, Ccb, pavgb, drx, dry, dbx, dby, avL, avR;
+    };
+    __device__ static EcAdjShared ec_dot_adj_shared(const LawParams& P, const double* hL, const double* hR,
+                                                    const double* a, const double* Gb, double* Vb) {
+        EcAdjShared c;
+        double vv = 0.0, av = 0.0;
+        c.avL = 0.0;
+        c.avR = 0.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) {
             const double vl = hL[H_V + m], vr = hR[H_V + m];
             vv += vl * vr;
             Vb[m] = 0.5 * (vl + vr);
             av += a[m] * Vb[m];
-            avL += a[m] * vl;
-            avR += a[m] * vr;
+            c.avL += a[m] * vl;
+            c.avR += a[m] * vr;
         }
         const double pL = hL[H_P], pR = hR[H_P];
-        double drx, dry, dbx, dby;
-        const double rho_avg = logmean_vg(hL[H_RHO], hR[H_RHO], hL[H_LR], hR[H_LR], hL[H_IR], hR[H_IR], drx, dry);
-        const double ilm = inv_logmean_vg(hL[H_B], hR[H_B], hL[H_LB], hR[H_LB], pL * hL[H_IR], pR * hR[H_IR], dbx, dby);
+        const double rho_avg = logmean_vg(hL[H_RHO], hR[H_RHO], hL[H_LR], hR[H_LR], hL[H_IR], hR[H_IR], c.drx, c.dry);
+        const double ilm = inv_logmean_vg(hL[H_B], hR[H_B], hL[H_LB], hR[H_LB], pL * hL[H_IR], pR * hR[H_IR], c.dbx,
+                                          c.dby);
         const double Cc = 0.5 * vv + P.inv_gm1 * ilm;
-        const double f = rho_avg * av;
+        c.f = rho_avg * av;
         double fb = Gb[0] + Gb[D + 1] * Cc;
-        double pavgb = 0.0;
+        c.pavgb = 0.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) {
             fb += Gb[m + 1] * Vb[m];
-            pavgb += Gb[m + 1] * a[m];
+            c.pavgb += Gb[m + 1] * a[m];
         }
-        const double Ccb = Gb[D + 1] * f;
-        const double rhoavgb = fb * av, avb = fb * rho_avg;
+        c.Ccb = Gb[D + 1] * c.f;
+        c.rhoavgb = fb * av;
+        c.avb = fb * rho_avg;
+        return c;
+    }
+
+    // the side-specific rest: adds the pull-back into the conservative cotangent ub of
+    // the left (`left`) or the right state
+    __device__ static void ec_dot_adj_side_h(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                             const double* Gb, const EcAdjShared& c, bool left, double* ub) {
         const double* hs = left ? hL : hR;
         const double* ho = left ? hR : hL;
-        const double po = left ? pR : pL;
-        const double avo = left ? avR : avL;
+        const double po = left ? hR[H_P] : hL[H_P];
+        const double avo = left ? c.avR : c.avL;
         double vb[D];
 #pragma unroll
         for (int m = 0; m < D; ++m)
-            vb[m] = 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * f + avb * a[m]) + 0.5 * Ccb * ho[H_V + m];
-        double pb = 0.5 * Gb[D + 1] * avo + 0.5 * pavgb;
-        const double bb = P.inv_gm1 * Ccb * (left ? dbx : dby);
-        const double ips = left ? hL[H_IP] : hR[H_IP];
-        double rb = rhoavgb * (left ? drx : dry) + bb * ips;
+            vb[m] = 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * c.f + c.avb * a[m]) + 0.5 * c.Ccb * ho[H_V + m];
+        double pb = 0.5 * Gb[D + 1] * avo + 0.5 * c.pavgb;
+        const double bb = P.inv_gm1 * c.Ccb * (left ? c.dbx : c.dby);
+        const double ips = hs[H_IP];
+        double rb = c.rhoavgb * (left ? c.drx : c.dry) + bb * ips;
         pb += -bb * hs[H_B] * ips;
         // (rho, v, p) -> (rho, m, E), p = gm1 (E - 1/2 rho |v|^2), v = m / rho
         const double rho = hs[H_RHO], ir = hs[H_IR];
@@ -187,6 +200,25 @@ struct EulerAdj {
             vdot += vbt * hs[H_V + m];
         }
         ub[0] += rb - vdot * ir;
+    }
+
+    // ec_dot_adj_side from hoisted states: adds the pull-back of Gb through
+    // G = sum_n a_n F_n(L, R) into the kept side's conservative cotangent ub.
+    __device__ static void ec_dot_adj_h(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                        const double* Gb, bool left, double* ub) {
+        double Vb[D];
+        const EcAdjShared c = ec_dot_adj_shared(P, hL, hR, a, Gb, Vb);
+        ec_dot_adj_side_h(P, hL, hR, a, Gb, c, left, ub);
+    }
+
+    // both sides from one evaluation of the shared part: the pull-back into the left
+    // state's cotangent ubL and into the right state's ubR
+    __device__ static void ec_dot_adj_h2(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                         const double* Gb, double* ubL, double* ubR) {
+        double Vb[D];
+        const EcAdjShared c = ec_dot_adj_shared(P, hL, hR, a, Gb, Vb);
+        ec_dot_adj_side_h(P, hL, hR, a, Gb, c, true, ubL);
+        ec_dot_adj_side_h(P, hL, hR, a, Gb, c, false, ubR);
     }
 
     // (rho, v, p) cotangents -> conservative cotangents, added into ub

@@ -19,6 +19,7 @@
 
 #include "../../include/sse_hip.h"
 #include "sse_kernels.h"
+#include "sse_tensor.h"
 
 namespace sse {
 hipError_t launch_generic_dispatch(const DevParams& p, const double* u, double* dudt, const RKStage& rk,
@@ -42,17 +43,6 @@ struct TensorConfig {
 bool tensor_config(int elem, int P, int var, TensorConfig* c);
 hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                          int which);
-struct StructTables {
-    const double* SG;
-    const int* cr_j;
-    const double* cr_v;
-    const int* ci_i;
-    const double* ci_v;
-    const int* ci_j;
-    const int* fi;
-    int wcr, ch, n_items, wfi;
-    const double* uf_ext;
-};
 struct StructConfig {
     int TS, EPB, lds, max_item;
 };
@@ -673,25 +663,19 @@ int build_struct(sse_handle* h, const sse_desc* D) {
             for (int m = 0; m < d; ++m)
                 if (D->S[i + (size_t)N_q * (j + (size_t)N_q * m)] != 0.0) return 0;
         }
-    // C rows (volume side) and column chunks (facet side)
-    std::vector<std::vector<std::pair<int, double>>> crows(N_q);
+    // C by column chunks: every nonzero once, grouped by facet node
     std::vector<std::vector<std::pair<int, double>>> ccols(N_f);
     for (int j = 0; j < N_f; ++j)
         for (int i = 0; i < N_q; ++i) {
             const double c = D->C[i + (size_t)N_q * j];
             if (c == 0.0) continue;
-            crows[i].push_back({j, c});
             ccols[j].push_back({i, c});
         }
     sse::StructTables& S = h->ST;
-    std::vector<int> cr_j;
-    std::vector<double> cr_v;
-    ell(crows, S.wcr, cr_j, cr_v);
     const int ch = N1;
     const int max_item = cfg.max_item;
     std::vector<int> ci_i((size_t)ch * max_item, -1), ci_j(max_item, -1);
     std::vector<double> ci_v((size_t)ch * max_item, 0.0);
-    std::vector<std::vector<std::pair<int, int>>> fitems(N_f);
     int n_items = 0;
     for (int j = 0; j < N_f; ++j) {
         for (size_t s0 = 0; s0 < ccols[j].size(); s0 += ch) {
@@ -701,20 +685,14 @@ int build_struct(sse_handle* h, const sse_desc* D) {
                 ci_v[(size_t)w * max_item + n_items] = ccols[j][s0 + w].second;
             }
             ci_j[n_items] = j;
-            fitems[j].push_back({n_items, 0});
             ++n_items;
         }
     }
-    std::vector<int> fi;
-    std::vector<int> fi_dummy;
-    ell(fitems, S.wfi, fi, fi_dummy);
-    if (S.wcr > 12 || S.wfi > 8 || ch > 8) return 0;  // kernel register tables
     S.ch = ch;
     S.n_items = n_items;
     int rc;
-    if ((rc = upload(h, SG, &S.SG)) || (rc = upload(h, cr_j, &S.cr_j)) || (rc = upload(h, cr_v, &S.cr_v)) ||
-        (rc = upload(h, ci_i, &S.ci_i)) || (rc = upload(h, ci_v, &S.ci_v)) || (rc = upload(h, ci_j, &S.ci_j)) ||
-        (rc = upload(h, fi, &S.fi)))
+    if ((rc = upload(h, SG, &S.SG)) || (rc = upload(h, ci_i, &S.ci_i)) || (rc = upload(h, ci_v, &S.ci_v)) ||
+        (rc = upload(h, ci_j, &S.ci_j)))
         return rc;
     double* ext = nullptr;
     if ((rc = alloc(h, (size_t)N_f * D->N_c * N_e, &ext))) return rc;
@@ -964,8 +942,8 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     if (h->use_struct) {
         const sse::TensorParams& T = h->TP;
         const sse::StructTables& S = h->ST;
-        snprintf(buf, len, "struct elem=%d P=%d variant=%d blocks=%d wcr=%d items=%d ch=%d wfi=%d wrt=%d",
-                 T.elem, T.P, T.variant, h->struct_blocks, S.wcr, S.n_items, S.ch, S.wfi, T.wrt);
+        snprintf(buf, len, "struct elem=%d P=%d variant=%d blocks=%d items=%d ch=%d wrt=%d", T.elem, T.P, T.variant,
+                 h->struct_blocks, S.n_items, S.ch, T.wrt);
     } else if (h->use_tensor) {
         const sse::TensorParams& T = h->TP;
         snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",

@@ -10,10 +10,12 @@
 //   r_q[i,e] -= sum_{f, b != a_f(i)} sum_n a_n F(u_i, u_j)[e,n],
 //   a_n = sum_m S_m[i,j] (Lambda_i + Lambda_j)[m,n]
 // (the reference's flux_difference!, src/Solvers/flux_differencing_form.jl:37-75,
-// summed by rows; S antisymmetric and F symmetric make the two equal), and its
-// row of the facet correction (C nonzeros). The facet side of the facet
-// correction is gathered by column chunks (balanced across the team) and summed
-// through a small LDS stage. No atomics: every sum has a fixed order.
+// summed by rows; S antisymmetric and F symmetric make the two equal). Each
+// facet-correction pair (C nonzero) is evaluated once, by column chunks balanced
+// across the team: the same two-point term enters the volume node's row and the
+// facet node's column, and both sums are accumulated in LDS with atomic adds.
+// Those are the only sums whose order varies from run to run; the Hadamard rows
+// and everything else have a fixed order.
 //
 // A team of TS threads owns one element. The exterior facet state is read
 // contiguously: phase 1 writes each facet state also into the neighbour's
@@ -49,8 +51,9 @@ struct StructLayout {
     static constexpr int off_L = off_h + NH * Sh::NQ;          // D*D x NQ
     static constexpr int off_hf = off_L + D * D * Sh::NQ;      // NH x NF
     static constexpr int off_nJ = off_hf + NH * Sh::NF;        // D x NF
-    static constexpr int off_st = off_nJ + D * Sh::NF;         // NC x MAXITEM partials
-    static constexpr int size = off_st + NC * MAXITEM;
+    static constexpr int off_st = off_nJ + D * Sh::NF;         // row and facet accumulators of C
+    static constexpr int ST = NC * (Sh::NQ + Sh::NF) > NC * MAXITEM ? NC * (Sh::NQ + Sh::NF) : NC * MAXITEM;
+    static constexpr int size = off_st + ST;
     static constexpr int off_ff = off_h;                       // NC x NF (after the pair phase)
     static constexpr int off_r = off_hf;                       // NC x NQ
     static constexpr int off_rb = off_L;                       // NC x NQ partial rows of split threads
@@ -58,7 +61,7 @@ struct StructLayout {
     static constexpr int off_z = off_st;                       // NC x NZ
     static constexpr int off_W = off_z + NC * Sh::NZ;          // NC x NW
     static_assert(NC * Sh::NF <= NH * Sh::NQ, "f_f alias");
-    static_assert(NC * Sh::NQ <= NH * Sh::NF + D * Sh::NF + NC * MAXITEM, "r_q alias");
+    static_assert(NC * Sh::NQ <= NH * Sh::NF + D * Sh::NF + ST, "r_q alias");
     static_assert(NC * Sh::NP <= D * D * Sh::NQ, "y alias");
     static_assert(NC * Sh::NQ <= D * D * Sh::NQ, "split partial-row alias");
     static_assert(off_W + NC * Sh::NW <= size, "sum-factorisation temporaries alias");
@@ -74,7 +77,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
     using L = StructLayout<ELEM, P>;
     constexpr int D = Sh::D, NC = Sh::NC, NH = Sh::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
     constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
-    constexpr int MAXCR = 12, MAXCH = 8, MAXFI = 8, MAXR = TENSOR_MAXR;
+    constexpr int MAXR = TENSOR_MAXR;
     constexpr int NPART = D * (N1 - 1), QH = (NPART + SPLIT - 1) / SPLIT;  // volume partners per thread
     static_assert(NQ <= NT && NF <= NT, "one node per thread");
     extern __shared__ double smem[];
@@ -126,7 +129,10 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         const int k = min(kb + team, last);
         const bool valid = kb + team < p.N_e;
         if (half == 0) prefetch(k);
-        // ---- phase A: hoist, interface flux
+        // ---- phase A: hoist, interface flux; clear the facet-correction accumulators
+        double* srow = sst;            // NC x NQ: C-row sums of the volume nodes
+        double* sfac = sst + NC * NQ;  // NC x NF: C-column sums of the facet nodes
+        for (int t = tt; t < NC * (NQ + NF); t += TS) sst[t] = 0.0;
         double hi[NH], Li[D * D], r[NC], ff[NC], Ji = 1.0;
 #pragma unroll
         for (int e = 0; e < NC; ++e) r[e] = ff[e] = 0.0;
@@ -166,7 +172,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         }
         __syncthreads();
 
-        // ---- phase B: volume Hadamard row and facet-correction row of node tl
+        // ---- phase B: volume Hadamard row of node tl
         if (SPLIT > 1 && half > 0 && tl < NQ) {
 #pragma unroll
             for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
@@ -205,28 +211,8 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                     ++f;
                 }
             }
-            const int wh = (st.wcr + SPLIT - 1) / SPLIT;
-#pragma unroll 1
-            for (int w = half * wh; w < (half + 1) * wh && w < st.wcr; ++w) {
-                const int j = st.cr_j[w * NQ + tl];
-                if (j < 0) continue;
-                const double crv = st.cr_v[w * NQ + tl];
-                const int fc = j / NPF;
-#pragma unroll
-                for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
-#pragma unroll
-                for (int n = 0; n < D; ++n) {
-                    double s = snJ[n * NF + j];
-#pragma unroll
-                    for (int l = 0; l < D; ++l) s += Li[l + D * n] * p.fn[l + D * fc];
-                    av[n] = 0.5 * crv * s;
-                }
-                ec_dot<D>(p.law, hi, hj, av, c);
-#pragma unroll
-                for (int e = 0; e < NC; ++e) r[e] -= c[e];
-            }
         }
-        // ---- phase C: facet-correction column chunks (facet side)
+        // ---- phase C: facet-correction pairs by column chunks, both sums by LDS atomics
 #pragma unroll 1
         for (int it = 0; it < MAXITEM / TS; ++it) {
             const int item = it * TS + tt;
@@ -254,29 +240,29 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 }
                 ec_dot<D>(p.law, hv, hj, av, c);
 #pragma unroll
-                for (int e = 0; e < NC; ++e) part[e] += c[e];
+                for (int e = 0; e < NC; ++e) {
+                    part[e] += c[e];
+                    atomicAdd(&srow[e * NQ + i], c[e]);
+                }
             }
 #pragma unroll
-            for (int e = 0; e < NC; ++e) sst[e * MAXITEM + item] = part[e];
+            for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
         }
         __syncthreads();
-        // ---- phase D: facet node sums its chunks; f_f to LDS
+        // ---- phase D: the C sums into r (volume nodes) and f_f (facet nodes); f_f to LDS
         double* sff = sm + L::off_ff;
         double* srb = sm + L::off_rb;
         if (SPLIT > 1 && half == 1 && tl < NQ) {
 #pragma unroll
             for (int e = 0; e < NC; ++e) srb[e * NQ + tl] = r[e];
         }
+        if (half == 0 && tl < NQ) {  // C-row sums, read before r_q may overwrite them
+#pragma unroll
+            for (int e = 0; e < NC; ++e) r[e] -= srow[e * NQ + tl];
+        }
         if (half == 0 && tl < NF) {
 #pragma unroll
-            for (int w = 0; w < MAXFI; ++w) {
-                const int item = w < st.wfi ? st.fi[w * NF + tl] : -1;
-                if (item < 0) continue;
-#pragma unroll
-                for (int e = 0; e < NC; ++e) ff[e] -= sst[e * MAXITEM + item];
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) sff[e * NF + tl] = ff[e];
+            for (int e = 0; e < NC; ++e) sff[e * NF + tl] = ff[e] - sfac[e * NF + tl];
         }
         __syncthreads();
         // ---- phase E: lift r_q -= R' f_f

@@ -65,13 +65,10 @@ SSE_TUNE(ELEM_TRI, 4, 1, 64, 2, 3, 64, 4)
 // Line-structured operator tables (host-built from the descriptor's S, C and R).
 struct StructTables {
     const double* SG;    // D x N1 x D x NQ: S_m[i, j(i, f, b)], index ((f * N1 + b) * D + m) * NQ + i
-    const int* cr_j;     // WCR x NQ: facet node of each C row entry (-1 pad)
-    const double* cr_v;  // WCR x NQ
     const int* ci_i;     // CH x MAXITEM: volume nodes of each column chunk (-1 pad)
     const double* ci_v;  // CH x MAXITEM
     const int* ci_j;     // MAXITEM: facet node of each chunk (-1 = no chunk)
-    const int* fi;       // WFI x NF: chunks summed into each facet node (-1 pad)
-    int wcr, ch, n_items, wfi;
+    int ch, n_items;
     const double* uf_ext;  // exterior facet states [k][e][j]
 };
 

@@ -9,11 +9,15 @@
 //                                   flux and for the neighbour's flux at the shared node
 //   Hadamard rows along lines   ->  each volume node gathers (dT_ij/du_i)' (rbar_j - rbar_i)
 //                                   over its line partners j (S_m from the line tables)
-//   facet correction            ->  volume side by C rows, facet side by column chunks
+//   facet correction            ->  each C pair once, by column chunks: one evaluation of the
+//                                   shared part of the flux adjoint pulled back to both the
+//                                   volume and the facet state, the two sums accumulated in
+//                                   LDS with atomic adds
 //   u_q = e2c(w_q'), u_f = e2c(R w_q'), w_q' = V M^{-1} V' (WJ c2e(V u))  reversed.
 // The pull-back through u = e2c(w) at the projected states uses du/dw in closed
-// form from the states' (rho, v, p), so the projection is not recomputed. Every
-// sum has a fixed order; no atomics.
+// form from the states' (rho, v, p), so the projection is not recomputed. The
+// facet-correction sums are the only ones whose order varies from run to run
+// (LDS atomics); every other sum has a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "sse_adjoint.h"
@@ -45,8 +49,8 @@ struct VjpLayout {
     static constexpr int off_nJ = off_hf + NH * Sh::NF;      // D x NF
     static constexpr int off_rb = off_nJ + D * Sh::NF;       // NC x NQ rbar
     static constexpr int off_fb = off_rb + NC * Sh::NQ;      // NC x NF fbar
-    static constexpr int off_st = off_fb + NC * Sh::NF;      // NC x MAXITEM chunk partials
-    static constexpr int off_ub = off_st + NC * MAXITEM;     // NC x NQ split partial rows, then w_q' cotangent
+    static constexpr int off_st = off_fb + NC * Sh::NF;      // NC x (NQ + NF) C-row and C-column sums
+    static constexpr int off_ub = off_st + NC * (Sh::NQ + Sh::NF);  // NC x NQ split partial rows, then w_q' cotangent
     static constexpr int off_wf = off_ub + NC * Sh::NQ;      // NC x NF w_f cotangent
     static constexpr int off_red = off_wf + NC * Sh::NF;     // TS reduction buffer
     static constexpr int size = off_red + T::TS;
@@ -146,7 +150,6 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
     constexpr int D = Sh::D, NC = Sh::NC, NH = L::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
     constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
     constexpr int NPART = D * (N1 - 1), QH = (NPART + SPLIT - 1) / SPLIT;
-    constexpr int MAXFI = 8;
     static_assert(NQ <= NT && NF <= NT, "one node per thread");
     extern __shared__ double sm[];
     const DevParams& p = tp.base;
@@ -158,12 +161,14 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
     double* snJ = sm + L::off_nJ;
     double* srb = sm + L::off_rb;
     double* sfb = sm + L::off_fb;
-    double* sst = sm + L::off_st;
+    double* srow = sm + L::off_st;       // NC x NQ: facet-correction sums of the volume nodes
+    double* sfac = srow + NC * NQ;       // NC x NF: facet-correction sums of the facet nodes
     double* sub = sm + L::off_ub;
     double* swf = sm + L::off_wf;
     double* red = sm + L::off_red;
 
-    // ---- phase A: loads, hoisting, interface flux adjoints
+    // ---- phase A: loads, hoisting, interface flux adjoints; clear the C-pair sums
+    for (int t = tt; t < NC * (NQ + NF); t += TS) srow[t] = 0.0;
     double hi[NH], ubq[NC];
     double lam = 0.0;
 #pragma unroll
@@ -268,34 +273,14 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
                 ++f;
             }
         }
-        const int wh = (st.wcr + SPLIT - 1) / SPLIT;
-#pragma unroll 1
-        for (int w = half * wh; w < (half + 1) * wh && w < st.wcr; ++w) {
-            const int j = st.cr_j[w * NQ + tl];
-            if (j < 0) continue;
-            const double crv = st.cr_v[w * NQ + tl];
-            const int fc = j / NPF;
-#pragma unroll
-            for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
-#pragma unroll
-            for (int n = 0; n < D; ++n) {
-                double s = snJ[n * NF + j];
-#pragma unroll
-                for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + tl] * p.fn[l + D * fc];
-                av[n] = 0.5 * crv * s;
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) gb[e] = -(srb[e * NQ + tl] + sfb[e * NF + j]);
-            A::ec_dot_adj_h(p.law, hi, hj, av, gb, true, ubq);
-        }
     }
-    // ---- phase C: facet-correction column chunks (facet side)
+    // ---- phase C: facet-correction pairs by column chunks, both sides at once
 #pragma unroll 1
     for (int it = 0; it < MAXITEM / TS; ++it) {
         const int item = it * TS + tt;
         const int j = item < st.n_items ? st.ci_j[item] : -1;
         if (j < 0) continue;
-        double hj[NH], hv[NH], av[D], gb[NC], part[NC];
+        double hj[NH], hv[NH], av[D], gb[NC], part[NC], uv[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) part[e] = 0.0;
 #pragma unroll
@@ -316,11 +301,16 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
                 av[n] = 0.5 * cv * s;
             }
 #pragma unroll
-            for (int e = 0; e < NC; ++e) gb[e] = -(srb[e * NQ + i] + sfb[e * NF + j]);
-            A::ec_dot_adj_h(p.law, hv, hj, av, gb, false, part);
+            for (int e = 0; e < NC; ++e) {
+                gb[e] = -(srb[e * NQ + i] + sfb[e * NF + j]);
+                uv[e] = 0.0;
+            }
+            A::ec_dot_adj_h2(p.law, hv, hj, av, gb, uv, part);
+#pragma unroll
+            for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + i], uv[e]);
         }
 #pragma unroll
-        for (int e = 0; e < NC; ++e) sst[e * MAXITEM + item] = part[e];
+        for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
     }
     if (SPLIT > 1 && half == 1 && tl < NQ) {
 #pragma unroll
@@ -333,18 +323,13 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
 #pragma unroll
             for (int e = 0; e < NC; ++e) ubq[e] += sub[e * NQ + tl];
         }
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ubq[e] += srow[e * NQ + tl];
     }
     if (half == 0 && tl < NF) {
         double ubf[NC], hf[NH];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl];
-#pragma unroll
-        for (int w = 0; w < MAXFI; ++w) {
-            const int item = w < st.wfi ? st.fi[w * NF + tl] : -1;
-            if (item < 0) continue;
-#pragma unroll
-            for (int e = 0; e < NC; ++e) ubf[e] += sst[e * MAXITEM + item];
-        }
+        for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl] + sfac[e * NF + tl];
 #pragma unroll
         for (int v = 0; v < NH; ++v) hf[v] = shf[v * NF + tl];
         double wb[NC];

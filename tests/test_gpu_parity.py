@@ -75,12 +75,13 @@ def test_residual_matches_oracle(name):
 
 
 def test_residual_host_memory_path_equals_device_path():
+    # equal up to the order of the facet-correction sums (LDS atomic adds)
     from sse import semi_discrete_residual
     ode = vortex_2d_config(M=4, p=4)[4]
     dev = _gpu_residual(ode)
     host = np.zeros_like(ode.u0, order="F")
     semi_discrete_residual(host, ode.u0, ode.p)
-    assert np.array_equal(dev, host)
+    assert _rel(host, dev) <= 1e-14
 
 
 def test_rk_100_steps_config1_matches_oracle():
