@@ -1208,12 +1208,33 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
     ADJ_TRY(hip_step(hipMemcpyAsync(u, u0, n * sizeof(double),
                                     where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream),
                      "copy u0"));
-    // forward sweep with checkpoints at the start of each segment
+    // one stage of a segment that keeps what its VJP reads: the stage state and, on the
+    // fast path, the phase-1 outputs
+    auto stage_keep = [&](int step, int first, int s) -> int {
+        const int idx = 5 * (step - first) + s;
+        double* keep = stages + n * (size_t)idx;
+        if (keep_p1) {  // the stage kernel also stores the stage state
+            RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr, keep};
+            return launch_struct_into(h, u, rk, p1_of(idx));
+        }
+        int r = hip_step(hipMemcpyAsync(keep, u, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
+                         "stage state");
+        if (r) return r;
+        RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
+        return launch(h, u, nullptr, rk, 3);
+    };
+    // forward sweep with checkpoints at the start of each segment; the last segment
+    // keeps its stages as it goes, so the reverse sweep does not recompute it
+    const int last_first = (nseg - 1) * checkpoint_every;
     for (int step = 0; step < nsteps; ++step) {
         if (step % checkpoint_every == 0)
             ADJ_TRY(hip_step(hipMemcpyAsync(ckpt + n * (size_t)(step / checkpoint_every), u, n * sizeof(double),
                                             hipMemcpyDeviceToDevice, h->stream), "checkpoint"));
         for (int s = 0; s < 5; ++s) {
+            if (step >= last_first) {
+                ADJ_TRY(stage_keep(step, last_first, s));
+                continue;
+            }
             RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
             ADJ_TRY(launch(h, u, nullptr, rk, 3));
         }
@@ -1230,22 +1251,12 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int first = seg * checkpoint_every;
         const int last = std::min(nsteps, first + checkpoint_every);
-        ADJ_TRY(hip_step(hipMemcpyAsync(u, ckpt + n * (size_t)seg, n * sizeof(double), hipMemcpyDeviceToDevice,
-                                        h->stream), "restore"));
-        for (int step = first; step < last; ++step)
-            for (int s = 0; s < 5; ++s) {
-                const int idx = 5 * (step - first) + s;
-                double* keep = stages + n * (size_t)idx;
-                if (keep_p1) {  // the stage kernel also stores the stage state
-                    RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr, keep};
-                    ADJ_TRY(launch_struct_into(h, u, rk, p1_of(idx)));
-                    continue;
-                }
-                ADJ_TRY(hip_step(hipMemcpyAsync(keep, u, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
-                                 "stage state"));
-                RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
-                ADJ_TRY(launch(h, u, nullptr, rk, 3));
-            }
+        if (seg != nseg - 1) {  // the last segment's stages are still there from the forward sweep
+            ADJ_TRY(hip_step(hipMemcpyAsync(u, ckpt + n * (size_t)seg, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                            h->stream), "restore"));
+            for (int step = first; step < last; ++step)
+                for (int s = 0; s < 5; ++s) ADJ_TRY(stage_keep(step, first, s));
+        }
         // adjoint of the 2N stages: kbar += B_s ubar; ubar += dt R'(u_s)' kbar; kbar *= A_s
         for (int step = last - 1; step >= first; --step) {
             if (fast) {
