@@ -54,10 +54,17 @@ struct VjpLayout {
     static constexpr int off_wf = off_ub + NC * Sh::NQ;      // NC x NF w_f cotangent
     static constexpr int off_red = off_wf + NC * Sh::NF;     // TS reduction buffer
     static constexpr int size = off_red + T::TS;
-    // reverse projection temporaries over the (dead) hoisted states
+};
+
+// LDS of the reverse projection (phase V3)
+template <int ELEM, int P>
+struct VjpProjLayout {
+    using Sh = Shape<ELEM, P>;
+    static constexpr int NC = Sh::NC;
     static constexpr int off_z = 0, off_W = off_z + NC * Sh::NZ, off_y = off_W + NC * (Sh::NW > 0 ? Sh::NW : 1);
     static constexpr int off_x = off_y + NC * Sh::NP, off_u = off_x + NC * Sh::NQ, off_u0 = off_u + NC * Sh::NP;
-    static_assert(off_u0 + NC * Sh::NQ <= off_nJ, "reverse-projection temporaries alias");
+    static constexpr int off_w = off_u0 + NC * Sh::NQ;  // w_q' cotangent
+    static constexpr int size = off_w + NC * Sh::NQ;
 };
 
 }  // namespace
@@ -136,12 +143,13 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
     }
 }
 
-// V2: everything else, one element per workgroup
+// V2: the two-point terms and the entropy variables, one element per workgroup. It
+// leaves the cotangent of the projected volume states w_q' in the element's slice of
+// rbar (read into LDS at its start) for V3.
 template <int ELEM, int P>
 __global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
-k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, const double* __restrict__ rbar,
-                const double* __restrict__ fbar, const double* __restrict__ fbar_ext, double* __restrict__ ubar,
-                double* __restrict__ lam_partial, VjpOut out) {
+k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, const double* __restrict__ fbar,
+                const double* __restrict__ fbar_ext, double* __restrict__ lam_partial, VjpOut out) {
     using Sh = Shape<ELEM, P>;
     using T = VjpTune<ELEM, P>;
     using L = VjpLayout<ELEM, P>;
@@ -358,19 +366,51 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
 #pragma unroll
         for (int e = 0; e < NC; ++e) sub[e * NQ + tl] = wb[e];
     }
-    // ---- phase E: w_q' = V M^{-1} V' (WJ c2e(V u)) reversed
+    __syncthreads();  // the w_q' cotangent is complete
+    double* wk = rbar + (size_t)k * NQ * NC;
+    for (int t = tt; t < NQ * NC; t += TS) wk[t] = sub[t];
+    // λ cotangent of this element, summed in a fixed order
+    red[tt] = lam;
+    __syncthreads();
+    for (int s = TS / 2; s > 0; s >>= 1) {
+        if (tt < s) red[tt] += red[tt + s];
+        __syncthreads();
+    }
+    if (tt == 0) lam_partial[k] = (out.accumulate ? lam_partial[k] : 0.0) + out.scale * red[0];
+}
+
+// V3: the reverse of w_q' = V M^{-1} V' (WJ c2e(V u)), sum-factorised, one element per
+// workgroup: ubar (+)= scale * V' c2e'(V u)' (WJ V M^{-1} V' wbar), M^{-1} = V' (W/J V .).
+// Its own kernel so that these low-register stages run at higher occupancy than V2.
+template <int ELEM, int P>
+__global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
+k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __restrict__ wbar,
+                double* __restrict__ ubar, VjpOut out) {
+    using Sh = Shape<ELEM, P>;
+    using T = VjpTune<ELEM, P>;
+    using L = VjpProjLayout<ELEM, P>;
+    using A = EulerAdj<Sh::D>;
+    using SF = SumFact<ELEM, P, T::TS>;
+    constexpr int NC = Sh::NC, NP = Sh::NP, NQ = Sh::NQ, TS = T::TS;
+    extern __shared__ double sm[];
+    const DevParams& p = tp.base;
+    const int k = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap);
+    const int tt = threadIdx.x;
     double* sZ = sm + L::off_z;
     double* sW = sm + L::off_W;
     double* sy = sm + L::off_y;
     double* sx = sm + L::off_x;
     double* su = sm + L::off_u;
     double* su0 = sm + L::off_u0;
+    double* sw = sm + L::off_w;
     const double* uk = u + (size_t)k * NP * NC;
+    const double* wk = wbar + (size_t)k * NQ * NC;
     for (int t = tt; t < NP * NC; t += TS) su[t] = uk[t];
+    for (int t = tt; t < NQ * NC; t += TS) sw[t] = wk[t];
     __syncthreads();
-    SF::V(tp, tt, su, su0, sZ, sW);    // V u
-    SF::Vt(tp, tt, sub, sy, sZ, sW);   // V' (w_q' cotangent)
-    SF::V(tp, tt, sy, sx, sZ, sW);     // M^{-1} = V' (W/J V .)
+    SF::V(tp, tt, su, su0, sZ, sW);   // V u
+    SF::Vt(tp, tt, sw, sy, sZ, sW);   // V' (w_q' cotangent)
+    SF::V(tp, tt, sy, sx, sZ, sW);    // M^{-1} = V' (W/J V .)
     const double* Jk = p.Jq + (size_t)k * NQ;
     for (int i = tt; i < NQ; i += TS) {
         const double jw = p.W[i] / Jk[i];
@@ -379,7 +419,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
     }
     __syncthreads();
     SF::Vt(tp, tt, sx, sy, sZ, sW);
-    SF::V(tp, tt, sy, sx, sZ, sW);     // V (M^{-1} V' w_q'bar), then times WJ and through c2e
+    SF::V(tp, tt, sy, sx, sZ, sW);    // V (M^{-1} V' w_q'bar), then times WJ and through c2e
     for (int i = tt; i < NQ; i += TS) {
         const double wj = p.W[i] * Jk[i];
         double u0[NC], wb[NC], ub[NC];
@@ -391,20 +431,12 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, const double* __restrict__ u, 
         }
         A::c2e_adj(p.law, u0, wb, ub);
 #pragma unroll
-        for (int e = 0; e < NC; ++e) sub[e * NQ + i] = ub[e];
+        for (int e = 0; e < NC; ++e) sw[e * NQ + i] = ub[e];
     }
     __syncthreads();
-    SF::Vt(tp, tt, sub, sy, sZ, sW);
+    SF::Vt(tp, tt, sw, sy, sZ, sW);
     double* outk = ubar + (size_t)k * NP * NC;
     for (int t = tt; t < NP * NC; t += TS) outk[t] = (out.accumulate ? outk[t] : 0.0) + out.scale * sy[t];
-    // λ cotangent of this element, summed in a fixed order
-    red[tt] = lam;
-    __syncthreads();
-    for (int s = TS / 2; s > 0; s >>= 1) {
-        if (tt < s) red[tt] += red[tt + s];
-        __syncthreads();
-    }
-    if (tt == 0) lam_partial[k] = (out.accumulate ? lam_partial[k] : 0.0) + out.scale * red[0];
 }
 
 // ----------------------------------------------------------------------------- launch
@@ -412,6 +444,7 @@ template <int ELEM, int P>
 size_t vjp_tensor_lds(int which) {
     using Sh = Shape<ELEM, P>;
     if (which == 1) return sizeof(double) * Sh::NC * (Sh::NP + Sh::NQ + (Sh::NW > 0 ? Sh::NW : 1));
+    if (which == 3) return sizeof(double) * VjpProjLayout<ELEM, P>::size;
     return sizeof(double) * VjpLayout<ELEM, P>::size;
 }
 
@@ -421,13 +454,16 @@ hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, c
                                double scale, int accumulate, const VjpIn& in, hipStream_t s) {
     using T = VjpTune<ELEM, P>;
     static bool attr = false;
-    const size_t l1 = vjp_tensor_lds<ELEM, P>(1), l2 = vjp_tensor_lds<ELEM, P>(2);
+    const size_t l1 = vjp_tensor_lds<ELEM, P>(1), l2 = vjp_tensor_lds<ELEM, P>(2), l3 = vjp_tensor_lds<ELEM, P>(3);
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)k_vjp_v1_tensor<ELEM, P>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_vjp_v2_tensor<ELEM, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)l2);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_vjp_v3_tensor<ELEM, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)l3);
         if (e != hipSuccess) return e;
         attr = true;
     }
@@ -435,8 +471,12 @@ hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, c
     hipLaunchKernelGGL((k_vjp_v1_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l1, s, tp, ybar, rbar, fbar, fbar_ext, in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l2, s, tp, st, u, rbar, fbar, fbar_ext, ubar,
+    hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l2, s, tp, st, rbar, fbar, fbar_ext,
                        lam_partial, VjpOut{scale, accumulate});
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_vjp_v3_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l3, s, tp, u, rbar, ubar,
+                       VjpOut{scale, accumulate});
     return hipGetLastError();
 }
 
@@ -449,7 +489,8 @@ size_t vjp_tensor_config(int elem, int P) {
 #define SSE_CASE(E_, PP)                                                                         \
     if (elem == E_ && P == PP) {                                                                 \
         const size_t a = vjp_tensor_lds<E_, PP>(1), b = vjp_tensor_lds<E_, PP>(2);               \
-        return a > b ? a : b;                                                                    \
+        const size_t c = vjp_tensor_lds<E_, PP>(3), ab = a > b ? a : b;                          \
+        return ab > c ? ab : c;                                                                  \
     }
     SSE_VJP_CASES
 #undef SSE_CASE
