@@ -1036,17 +1036,20 @@ int vjp_check(sse_handle* h) {
     if (P.standard || P.N_c == 1) return fail(h, SSE_ERR_UNSUPPORTED, "the VJP covers the Euler flux-differencing residual");
     if (P.law.two_point_flux != SSE_TWO_POINT_EC)
         return fail(h, SSE_ERR_UNSUPPORTED, "the VJP covers the entropy-conservative two-point flux");
-    if (sse::lds_vjp_v2(P) > 160 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the VJP kernels");
+    // compile-time-shaped VJP on the line-structured tables of the forward kernel, where it
+    // fits; the general kernels otherwise (or with SSE_VJP=generic)
+    const char* env = getenv("SSE_VJP");
+    const size_t lds = h->use_struct ? sse::vjp_tensor_config(h->TP.elem, h->TP.P) : 0;
+    const bool tensor = lds > 0 && lds <= 160 * 1024 && !(env && !strcmp(env, "generic"));
+    if (!tensor && sse::lds_vjp_v2(P) > 160 * 1024)
+        return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the VJP kernels");
+    h->vjp_tensor = tensor;
     const size_t Ne = P.N_e;
     int rc;
     if (!h->vjp_rbar && ((rc = alloc(h, (size_t)P.N_q * P.N_c * Ne, &h->vjp_rbar)) ||
                          (rc = alloc(h, (size_t)P.N_f * P.N_c * Ne, &h->vjp_fbar)) || (rc = alloc(h, Ne, &h->vjp_lam)) ||
                          (rc = alloc(h, 1, &h->vjp_lamout))))
         return rc;
-    // compile-time-shaped VJP on the line-structured tables of the forward kernel
-    const char* env = getenv("SSE_VJP");
-    const size_t lds = h->use_struct ? sse::vjp_tensor_config(h->TP.elem, h->TP.P) : 0;
-    h->vjp_tensor = lds > 0 && lds <= 160 * 1024 && !(env && !strcmp(env, "generic"));
     if (h->vjp_tensor && !h->vjp_fext && (rc = alloc(h, (size_t)P.N_f * P.N_c * Ne, &h->vjp_fext))) return rc;
     return 0;
 }

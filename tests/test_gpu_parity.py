@@ -453,6 +453,9 @@ VJP_CASES = {
     "vortex2d_tri_p2_lf": lambda: vortex_2d_config(M=3, p=2)[4],
     "wave3d_tet_p3_ec": lambda: _ec_interface(lambda f: density_wave_3d_config(M=2, p=3, ic_strength=0.2, form=f))[4],
     "vortex2d_tri_p4_lf": lambda: vortex_2d_config(M=3, p=4)[4],
+    "vortex2d_tri_p6_lf": lambda: vortex_2d_config(M=2, p=6)[4],
+    "wave3d_tet_p5_lf": lambda: density_wave_3d_config(M=1, p=5, ic_strength=0.2)[4],
+    "wave3d_tet_p2_lf": lambda: density_wave_3d_config(M=2, p=2, ic_strength=0.2)[4],
     "euler1d_gauss_nodal_ec": lambda: euler_1d_gauss()[4],
 }
 
