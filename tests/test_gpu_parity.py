@@ -55,7 +55,9 @@ CASES = {
     "vortex2d_modal_p3_warped": lambda: euler_vortex_2d_modal()[4],
     "vortex2d_tri_p4_8x8": lambda: vortex_2d_config(M=8, p=4)[4],
     "vortex2d_tri_p2": lambda: vortex_2d_config(M=4, p=2)[4],
+    "vortex2d_tri_p5": lambda: vortex_2d_config(M=3, p=5)[4],
     "vortex2d_tri_p6": lambda: vortex_2d_config(M=3, p=6)[4],
+    "wave3d_tet_p2_curved": lambda: density_wave_3d_config(M=2, p=2, ic_strength=0.2)[4],
     "wave3d_tet_p4_curved": lambda: density_wave_3d_config(M=2, p=4)[4],
     "wave3d_tet_p3_curved": lambda: density_wave_3d_config(M=3, p=3, ic_strength=0.2)[4],
     "wave3d_tet_p5_curved": lambda: density_wave_3d_config(M=2, p=5, ic_strength=0.2)[4],
@@ -247,6 +249,14 @@ STD_CASES = {
     "advdiff3d_tet_p5_br1": lambda: advection_diffusion_3d_config(M=2, p=5)[4],
     "advdiff3d_tet_p3_standard_mapping": _advdiff_standard_mapping,
     "euler2d_tri_standard_skew": _euler_tri_standard,
+    # every other degree the compile-time-shaped scalar kernels are built for
+    "adv2d_tri_p2": lambda: advection_2d_modal_tensor_tri(M=3, p=2)[4],
+    "adv2d_tri_p3": lambda: advection_2d_modal_tensor_tri(M=3, p=3)[4],
+    "adv2d_tri_p5": lambda: advection_2d_modal_tensor_tri(M=2, p=5)[4],
+    "adv2d_tri_p6": lambda: advection_2d_modal_tensor_tri(M=2, p=6)[4],
+    "advdiff3d_tet_p2_br1": lambda: advection_diffusion_3d_config(M=2, p=2)[4],
+    "advdiff3d_tet_p3_br1": lambda: advection_diffusion_3d_config(M=2, p=3)[4],
+    "advdiff3d_tet_p4_br1": lambda: advection_diffusion_3d_config(M=2, p=4)[4],
 }
 
 
