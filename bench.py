@@ -247,6 +247,76 @@ def bench_advdiff(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_multi(args):
+    """The dense-operator (ModalMulti) schemes of the reference's recorded notebook
+    timings: Euler on M=4 curved tets, ModalMulti p=3, EC interface flux, curl
+    metrics (N_e=384; examples/euler_3d.ipynb), and the isentropic vortex on M=4
+    warped triangles, ModalMulti p=4, Lax-Friedrichs (N_e=32;
+    examples/euler_vortex_2d.ipynb); then the tet scheme at M=16 (24,576 elements)
+    for throughput. Time per residual on the GPU beside the CPU oracle on one
+    thread (the reference's Serial() runs). Prints one JSON line."""
+    import torch
+
+    from sse import semi_discrete_residual
+    from sse.conservation_laws import EntropyConservativeNumericalFlux
+    from sse.reference_approximation import ModalMulti
+    from sse.solvers import FluxDifferencingForm
+    from sse.time_integration import to_device
+    from sse.workloads import density_wave_3d_config, euler_vortex_2d_modal
+
+    ec = FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux())
+    cases = [("tet_p3_M4_ec", lambda: density_wave_3d_config(M=4, p=3, approx=ModalMulti, form=ec), 31.8e-3,
+              "examples/euler_3d.ipynb:141-143"),
+             ("tri_p4_M4_lf", lambda: euler_vortex_2d_modal(M=4, p=4, approx=ModalMulti)[:5], 541e-6,
+              "examples/euler_vortex_2d.ipynb:1274-1276"),
+             ("tet_p3_M16_ec", lambda: density_wave_3d_config(M=16, p=3, approx=ModalMulti, form=ec), None, None)]
+    out = {}
+    for name, make, ref_s, where in cases:
+        law, sd, ic, form, ode = make()
+        solver = ode.p
+        N_p, N_c, N_e = solver.size()
+        dof = N_p * N_c * N_e
+        u = to_device(ode.u0)
+        dudt = torch.empty_like(u)
+        for _ in range(args.warmup):
+            semi_discrete_residual(dudt, u, solver)
+        solver.lib.sse_synchronize(solver.handle)
+        nrep = max(args.steps, 200 if N_e < 1000 else 20)
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            semi_discrete_residual(dudt, u, solver)
+        solver.lib.sse_synchronize(solver.handle)
+        gpu_s = (time.perf_counter() - t0) / nrep
+        rec = {"N_e": N_e, "N_p": N_p, "N_q": sd.reference_approximation.N_q, "DOF": dof,
+               "gpu_ms_per_residual": 1e3 * gpu_s, "gpu_dof_res_per_s": dof / gpu_s, "kernels": solver.describe()}
+        if ref_s is not None:
+            rec["reference_recorded_ms"] = 1e3 * ref_s
+            rec["reference_recorded_at"] = where
+            from oracle import oracle as O
+            O.set_threads(1)
+            t0 = time.perf_counter()
+            O.residual(solver.arrays, ode.u0)
+            one = time.perf_counter() - t0
+            reps = max(1, int(2.0 / max(one, 1e-4)))
+            t0 = time.perf_counter()
+            O.residual_repeat(solver.arrays, ode.u0, reps)
+            rec["cpu_oracle_1thread_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+        log(name, json.dumps(rec))
+        out[name] = rec
+        del solver, ode, u, dudt
+    big = out["tet_p3_M16_ec"]
+    line = {
+        "metric": "DOF-residuals/s (3D Euler ModalMulti p=3 FP64, dense operators)", "value": big["gpu_dof_res_per_s"],
+        "unit": "DOF-residuals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": big["gpu_ms_per_residual"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D Euler density wave on curved tets, ModalMulti p=3 (collapsed degree-6 "
+                               "quadrature), EC flux, curl metrics; notebook sizes and 16^3 hexes x 6 tets"},
+        "cases": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_setup(args):
     """Set-up of the 3D advection-diffusion problem (24^3 hexes x 6 curved tets, p=5):
     host (NumPy) against GPU (sse_geometric_factors, sse_apply_reference_mapping,
@@ -435,7 +505,7 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint", "advdiff", "setup"],
+    ap.add_argument("--config", default="residual", choices=["residual", "batched", "adjoint", "advdiff", "setup", "multi"],
                     help="residual: the curved-tet Euler p=4 residual (default); batched: the "
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
@@ -468,6 +538,8 @@ def main():
         return bench_advdiff(args)
     if args.config == "setup":
         return bench_setup(args)
+    if args.config == "multi":
+        return bench_multi(args)
 
     import torch
 

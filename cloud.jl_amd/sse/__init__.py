@@ -13,7 +13,8 @@ from .geometry import (ChanWilcoxMetrics, ConservativeCurlMetrics, ExactMetrics,
 from .mesh import (ChanWarping, DelReyWarping, UniformWarping, uniform_periodic_mesh,  # noqa: F401
                    warp_mesh)
 from .reference_approximation import (GaussQuadrature, LGLQuadrature, LGQuadrature,  # noqa: F401
-                                      LGRQuadrature, ModalTensor, NodalTensor,
+                                      LGRQuadrature, ModalMulti, ModalTensor, NodalMulti,
+                                      NodalTensor,
                                       ReferenceApproximationFor, ReferenceApproximationLine,
                                       ReferenceApproximationTet, ReferenceApproximationTri,
                                       check_sbp_property)

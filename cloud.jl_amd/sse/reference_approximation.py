@@ -393,8 +393,15 @@ class ModalTensor:
 
 @dataclass
 class ModalMulti:
-    """Modal basis on a multidimensional (here: Line) quadrature
-    (``src/SpatialDiscretizations/multidimensional.jl:1-41``)."""
+    """Modal basis of total degree p on a volume quadrature of degree 2p, with dense
+    operators (``src/SpatialDiscretizations/multidimensional.jl:1-41``)."""
+    p: int
+
+
+@dataclass
+class NodalMulti:
+    """Nodal (collocated) values at a volume quadrature of degree 2p, with dense
+    degree-p operators (``src/SpatialDiscretizations/multidimensional.jl:43-75``)."""
     p: int
 
 
@@ -496,12 +503,41 @@ def ReferenceApproximationLine(approx_type, mapping_degree: int = 1,
                                   None, None, sel, (rule,), ())
 
 
+def _multi_approximation(approx_type, rd: RefElemData, vrule, frule):
+    """ModalMulti / NodalMulti operators on a Tri or Tet (``multidimensional.jl:1-75``):
+    P = (V' W V)^{-1} V' W projects quadrature values onto the degree-p modes, D_m =
+    dV_m P, R = Vf P; ModalMulti keeps V (the modal state), NodalMulti the identity.
+    There is no reference mapping: D_m is already the derivative in r, s (, t)."""
+    p = approx_type.p
+    basis = S.basis_tri if rd.element_type == TRI else S.basis_tet
+    vand = S.vandermonde_tri if rd.element_type == TRI else S.vandermonde_tet
+    VDM, *dV = basis(p, *rd.rstq)
+    Vf = vand(p, *rd.rstf)
+    wq = rd.wq
+    Pr = np.linalg.solve(VDM.T @ (wq[:, None] * VDM), VDM.T * wq[None, :])
+    D = tuple(dVm @ Pr for dVm in dV)
+    V = VDM if isinstance(approx_type, ModalMulti) else None
+    return ReferenceApproximation(approx_type, rd, D, V, Vf @ Pr, wq.copy(), rd.wf.copy(), None,
+                                  None, None, tuple(vrule), tuple(frule) if isinstance(frule, tuple)
+                                  else (frule,))
+
+
 def ReferenceApproximationTri(approx_type, mapping_degree: int = 1, volume_quadrature_rule=None,
                               facet_quadrature_rule=None):
-    """ModalTensor/NodalTensor on the Tri (``tensor_simplex.jl:158-219``)."""
+    """ModalTensor/NodalTensor on the Tri (``tensor_simplex.jl:158-219``);
+    ModalMulti/NodalMulti (``multidimensional.jl:1-75``).
+
+    The multidimensional types default to quadratures of degree 2p. On the facets
+    that is the reference's own rule, LG with p + 1 nodes (``quadrature_rules.jl:48-50``).
+    In the volume the reference takes StartUpDG's Xiao-Gimbutas table, which is not
+    available to this package: the default here is the collapsed rule LG(p) x LG(p),
+    also exact to degree 2p. Pass `volume_quadrature_rule` to choose another."""
     p = approx_type.p
     vrule = volume_quadrature_rule or (LGQuadrature(p), LGQuadrature(p))
     frule = facet_quadrature_rule or LGQuadrature(p)
+    if isinstance(approx_type, (ModalMulti, NodalMulti)):
+        return _multi_approximation(approx_type, ref_elem_data_tri(mapping_degree, vrule, frule),
+                                    vrule, frule)
     ops = operators_1d(vrule)
     J_ref, L_ref = reference_geometric_factors_tri(vrule)
     e1_to_f = _interp_1d(vrule[0], frule)
@@ -521,10 +557,20 @@ def ReferenceApproximationTri(approx_type, mapping_degree: int = 1, volume_quadr
 
 def ReferenceApproximationTet(approx_type, mapping_degree: int = 1, volume_quadrature_rule=None,
                               facet_quadrature_rule=None):
-    """ModalTensor/NodalTensor on the Tet (``tensor_simplex.jl:221-306``)."""
+    """ModalTensor/NodalTensor on the Tet (``tensor_simplex.jl:221-306``);
+    ModalMulti/NodalMulti (``multidimensional.jl:1-75``).
+
+    The multidimensional types default to quadratures of degree 2p. The reference
+    takes the Jaskowiec-Sukumar (volume) and Xiao-Gimbutas (facet) tables from
+    StartUpDG, which are not available to this package: the defaults here are the
+    collapsed rules LG(p) x LG(p) x GJ(p; 1, 0) and LG(p) x GJ(p; 1, 0), exact to
+    degree 2p as well."""
     p = approx_type.p
     vrule = volume_quadrature_rule or (LGQuadrature(p), LGQuadrature(p), GaussQuadrature(p, 1, 0))
     frule = facet_quadrature_rule or (LGQuadrature(p), GaussQuadrature(p, 1, 0))
+    if isinstance(approx_type, (ModalMulti, NodalMulti)):
+        return _multi_approximation(approx_type, ref_elem_data_tet(mapping_degree, vrule, frule),
+                                    vrule, frule)
     ops = operators_1d(vrule)
     J_ref, L_ref = reference_geometric_factors_tet(vrule)
     e1_f1 = _interp_1d(vrule[0], frule[0])
@@ -578,7 +624,7 @@ def flux_differencing_operators(ref: ReferenceApproximation):
     D_xi = reference_derivative_operators(ref)
     W = np.diag(ref.W)
     S = np.stack([0.5 * (W @ Dm - Dm.T @ W) for Dm in D_xi])
-    sparse = ref.d > 1
+    sparse = ref.d > 1 and isinstance(ref.approx_type, (ModalTensor, NodalTensor))
     if ref.R_selection is not None:
         return S, sparse, None, 0
     C = ref.R.T @ np.diag(ref.B)

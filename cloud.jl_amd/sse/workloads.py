@@ -35,14 +35,15 @@ def euler_1d_gauss():
     return law, sd, exact, form, semidiscretize(law, sd, exact, form, (0.0, 2.0))
 
 
-def euler_vortex_2d_modal(M=4, form=None):
-    """test/euler_vortex_2d_modal.jl:1-73 (golden L2 at test/runtests.jl:111-121)."""
+def euler_vortex_2d_modal(M=4, form=None, p=3, approx=ModalTensor):
+    """test/euler_vortex_2d_modal.jl:1-73 (golden L2 at test/runtests.jl:111-121).
+    `approx=ModalMulti, p=4` is the scheme of examples/euler_vortex_2d.ipynb."""
     g, L, Ma = 1.4, 1.0, 0.4
     T = L / Ma
     strength = math.sqrt(2 / (g - 1) * (1 - 0.75 ** (g - 1)))
     law = EulerEquations(2, g)
     exact = IsentropicVortex(g, Ma, 0.0, 0.1, strength, 1.0, (L / 2, L / 2))
-    ref = ReferenceApproximationTri(ModalTensor(3), mapping_degree=3)
+    ref = ReferenceApproximationTri(approx(p), mapping_degree=p)
     mesh = uniform_periodic_mesh(ref, ((0.0, L), (0.0, L)), (M, M))
     mesh = warp_mesh(mesh, ref, ChanWarping(1.0 / 16.0, (L, L)))
     sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
@@ -62,14 +63,16 @@ def vortex_2d_config(M=8, p=4, form=None):
     return law, sd, ic, form, semidiscretize(law, sd, ic, form, (0.0, 1.0))
 
 
-def density_wave_3d_config(M=2, p=4, factor=0.05, form=None, ic_strength=0.5, backend="host"):
+def density_wave_3d_config(M=2, p=4, factor=0.05, form=None, ic_strength=0.5, backend="host",
+                           approx=ModalTensor):
     """3D Euler on curved tets: periodic [0,2]^3, M^3 hexes x 6 tets, UniformWarping
     factor 0.05, ModalTensor(p) isoparametric, curl metrics,
     rho = 1 + 0.5 sin(pi (x+y+z)), u=v=w=1, p=1. `backend="hip"` runs the set-up
-    (geometric factors, projection of the initial data) on the GPU."""
+    (geometric factors, projection of the initial data) on the GPU. `approx=ModalMulti`
+    with M=4, p=3 is the scheme of examples/euler_3d.ipynb (dense operators)."""
     law = EulerEquations(3, 1.4)
     ic = EulerPeriodicTest(3, 1.4, ic_strength, 2.0)
-    ref = ReferenceApproximationTet(ModalTensor(p), mapping_degree=p)
+    ref = ReferenceApproximationTet(approx(p), mapping_degree=p)
     mesh = uniform_periodic_mesh(ref, ((0.0, 2.0),) * 3, (M, M, M))
     if factor:
         mesh = warp_mesh(mesh, ref, UniformWarping(factor, (2.0, 2.0, 2.0)))
@@ -94,14 +97,15 @@ def advection_diffusion_1d_modal_multi():
     return law, sd, ExactSolution(law, ic), form, ode
 
 
-def advection_2d_modal_tensor_tri(M=2, p=4):
+def advection_2d_modal_tensor_tri(M=2, p=4, approx=ModalTensor):
     """test/runtests.jl:38-61 (golden L2 0.2660013939427627): ModalTensor(4) Tri,
     mapping degree 4, a = (1,1), sine data, StandardForm skew-symmetric with
     LF(0) (central), ReferenceOperator; DelRey-warped (0.1) 2x2 periodic mesh of
-    [0,1]^2 with exact metrics and projected Jacobian; dt = 1/100 to T = 1."""
+    [0,1]^2 with exact metrics and projected Jacobian; dt = 1/100 to T = 1.
+    `approx=NodalMulti, M=4` is the scheme of examples/advection_2d.ipynb."""
     law = LinearAdvectionEquation((1.0, 1.0))
     ic = InitialDataSine(1.0, (2 * math.pi, 2 * math.pi))
-    ref = ReferenceApproximationTri(ModalTensor(p), mapping_degree=p)
+    ref = ReferenceApproximationTri(approx(p), mapping_degree=p)
     mesh = warp_mesh(uniform_periodic_mesh(ref, ((0.0, 1.0), (0.0, 1.0)), (M, M)), ref, 0.1)
     sd = make_spatial_discretization(mesh, ref)
     form = StandardForm(mapping_form=SkewSymmetricMapping(),
