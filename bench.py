@@ -270,6 +270,8 @@ def bench_multi(args):
              ("tri_p4_M4_lf", lambda: euler_vortex_2d_modal(M=4, p=4, approx=ModalMulti)[:5], 541e-6,
               "examples/euler_vortex_2d.ipynb:1274-1276"),
              ("tet_p3_M16_ec", lambda: density_wave_3d_config(M=16, p=3, approx=ModalMulti, form=ec), None, None)]
+    if args.multi_cases:
+        cases = [c for c in cases if c[0] in args.multi_cases.split(",")]
     out = {}
     for name, make, ref_s, where in cases:
         law, sd, ic, form, ode = make()
@@ -304,7 +306,7 @@ def bench_multi(args):
         log(name, json.dumps(rec))
         out[name] = rec
         del solver, ode, u, dudt
-    big = out["tet_p3_M16_ec"]
+    big = out[list(out)[-1]]
     line = {
         "metric": "DOF-residuals/s (3D Euler ModalMulti p=3 FP64, dense operators)", "value": big["gpu_dof_res_per_s"],
         "unit": "DOF-residuals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -509,6 +511,8 @@ def main():
                     help="residual: the curved-tet Euler p=4 residual (default); batched: the "
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--multi-cases", default="",
+                    help="multi: comma-separated subset of tet_p3_M4_ec, tri_p4_M4_lf, tet_p3_M16_ec")
     ap.add_argument("--adj-mesh", type=int, default=12)
     ap.add_argument("--autotune", action="store_true", help="pick the residual's kernel configuration by timing")
     ap.add_argument("--ad-mesh", type=int, default=24)

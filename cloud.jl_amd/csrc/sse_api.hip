@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/sse_hip.h"
@@ -868,6 +869,58 @@ int create_handle(const sse_desc* D, int members, sse_handle** out) {
             return cleanup_fail(rc);
     } else if ((rc = union_rows(D->S, false, true, &P.S_ptr, &P.S_idx, &P.S_val))) {
         return cleanup_fail(rc);
+    }
+    if (!P.standard) {  // flux work items: volume pairs i < j, then the nonzeros of C
+        if (N_q >= 65536 || N_f >= 65536) {
+            h->err = "too many nodes per element";
+            return cleanup_fail(SSE_ERR_UNSUPPORTED);
+        }
+        std::vector<int> ij;
+        std::vector<double> val;
+        for (int i = 0; i < N_q; ++i)
+            for (int j = i + 1; j < N_q; ++j) {
+                bool nz = false;
+                for (int m = 0; m < d; ++m) nz |= D->S[i + (size_t)N_q * (j + (size_t)N_q * m)] != 0.0;
+                if (!nz) continue;
+                ij.push_back(i | j << 16);
+                for (int m = 0; m < d; ++m) val.push_back(D->S[i + (size_t)N_q * (j + (size_t)N_q * m)]);
+            }
+        P.n_vol_items = (int)ij.size();
+        if (D->C_kind != SSE_C_NONE)
+            for (int i = 0; i < N_q; ++i)
+                for (int j = 0; j < N_f; ++j) {
+                    const double c = D->C[i + (size_t)N_q * j];
+                    if (c == 0.0) continue;
+                    ij.push_back(i | j << 16);
+                    val.push_back(c);
+                    for (int m = 1; m < d; ++m) val.push_back(0.0);
+                }
+        P.n_items = (int)ij.size();
+        // Order the items so that consecutive ones (the lanes of a wavefront) touch
+        // different nodes: by the cyclic offset between their two nodes, then by
+        // node. Row-major order would send the atomic adds of a whole wavefront to
+        // one address.
+        {
+            std::vector<int> perm(ij.size());
+            for (size_t t = 0; t < perm.size(); ++t) perm[t] = (int)t;
+            auto key = [&](int t) {
+                const int i = ij[t] & 0xffff, j = ij[t] >> 16;
+                const bool vol = t < P.n_vol_items;
+                const long long off = vol ? ((j - i) % N_q + N_q) % N_q : ((i - j) % N_q + N_q) % N_q;
+                return std::make_tuple(vol ? 0 : 1, off, vol ? i : j);
+            };
+            std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key(a) < key(b); });
+            std::vector<int> ij2(ij.size());
+            std::vector<double> val2(val.size());
+            for (size_t t = 0; t < perm.size(); ++t) {
+                ij2[t] = ij[perm[t]];
+                for (int m = 0; m < d; ++m) val2[t * d + m] = val[(size_t)perm[t] * d + m];
+            }
+            ij.swap(ij2);
+            val.swap(val2);
+        }
+        if (!ij.empty() && ((rc = upload(h, ij, &P.item_ij)) || (rc = upload(h, val, &P.item_val))))
+            return cleanup_fail(rc);
     }
     if (D->C_kind != SSE_C_NONE) {
         Rows Cr = rows_of(D->C, N_q, N_f), Cc = cols_of(D->C, N_q, N_f);

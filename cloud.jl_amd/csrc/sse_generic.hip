@@ -10,11 +10,12 @@
 //   time_derivative  src/Solvers/flux_differencing_form.jl:294-347
 //   mass solve (WA)  src/Solvers/mass_matrix.jl:406-417
 //
-// The Hadamard sum is evaluated in gather form: the thread owning volume node
-// i loops over every j with S_m[i,j] != 0 for some m and accumulates
-//   r_q[i,e] -= sum_m S_m[i,j] sum_n (Lambda_i + Lambda_j)[m,n] F(u_i,u_j)[e,n],
-// which equals the reference's scatter over i < j (S antisymmetric, F symmetric)
-// without atomics; each pair is evaluated from both ends.
+// The time derivative evaluates every two-point flux once: the interface fluxes,
+// the volume pairs i < j of the Hadamard sum and the facet-correction pairs form
+// one flat list of work items shared evenly by the workgroup's threads (dense
+// multidimensional operators have N_q (N_q - 1) / 2 volume and N_q N_f facet
+// pairs per element), and each item adds its contribution to both ends through
+// LDS atomic adds.
 #include <hip/hip_runtime.h>
 
 #include "sse_kernels.h"
@@ -31,6 +32,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
     double* s_u = sm;              // N_p x NC
     double* s_a = s_u + N_p * NC;  // N_q x NC
     double* s_w = s_a + N_q * NC;  // N_p x NC
+    double* s_b = s_w + N_p * NC;  // N_f x NC
     const double* uk = u + (size_t)k * N_p * NC;
     for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) s_u[t] = uk[t];
     __syncthreads();
@@ -80,57 +82,52 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
             for (int e = 0; e < NC; ++e) s_a[i + N_q * e] = wb[e];
         }
     } else {
-        // modal entropy projection
+        // modal entropy projection; the products with V and V' run over (node,
+        // component) pairs so that a workgroup has N_q NC (or N_p NC) lanes of work
         const double* Jk = p.Jq + (size_t)k * N_q;
-        for (int i = threadIdx.x; i < N_q; i += blockDim.x) {
+        for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // u_q = V u
+            const int i = t % N_q, e = t / N_q;
+            double s = 0.0;
+            for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_u[j + N_p * e];
+            s_a[t] = s;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < N_q; i += blockDim.x) {  // WJ c2e(u_q)
             double ub[NC], wb[NC];
 #pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_u[j + N_p * e];
-                ub[e] = s;
-            }
+            for (int e = 0; e < NC; ++e) ub[e] = s_a[i + N_q * e];
             ok &= Ph::c2e(p.law, ub, wb);
             const double wj = p.W[i] * Jk[i];
 #pragma unroll
             for (int e = 0; e < NC; ++e) s_a[i + N_q * e] = wb[e] * wj;
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < N_p; j += blockDim.x) {  // w = V' (WJ w_q)
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int i = 0; i < N_q; ++i) s += p.V[i + (size_t)N_q * j] * s_a[i + N_q * e];
-                s_w[j + N_p * e] = s;
-            }
+        for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {  // w = V' (WJ w_q)
+            const int j = t % N_p, e = t / N_p;
+            double s = 0.0;
+            for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_a[i + N_q * e];
+            s_w[t] = s;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < N_q; i += blockDim.x) {  // J^{-1}W (V w)
-            const double jw = p.W[i] / Jk[i];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_w[j + N_p * e];
-                s_a[i + N_q * e] = s * jw;
-            }
+        for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // J^{-1}W (V w)
+            const int i = t % N_q, e = t / N_q;
+            double s = 0.0;
+            for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_w[j + N_p * e];
+            s_a[t] = s * (p.W[i] / Jk[i]);
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < N_p; j += blockDim.x) {  // w = V' (...)
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int i = 0; i < N_q; ++i) s += p.V[i + (size_t)N_q * j] * s_a[i + N_q * e];
-                s_w[j + N_p * e] = s;
-            }
+        for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {  // w = V' (...)
+            const int j = t % N_p, e = t / N_p;
+            double s = 0.0;
+            for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_a[i + N_q * e];
+            s_w[t] = s;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < N_q; i += blockDim.x) {  // w_q = V w
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_w[j + N_p * e];
-                s_a[i + N_q * e] = s;
-            }
+        for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // w_q = V w
+            const int i = t % N_q, e = t / N_q;
+            double s = 0.0;
+            for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_w[j + N_p * e];
+            s_a[t] = s;
         }
         __syncthreads();
         for (int i = threadIdx.x; i < N_q; i += blockDim.x) {  // u_q = e2c(w_q)
@@ -143,14 +140,17 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
         }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < N_f; j += blockDim.x) {  // u_f = e2c(R w_q)
+    for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) {  // w_f = R w_q
+        const int j = t % N_f, e = t / N_f;
+        double s = 0.0;
+        for (int c = p.R_ptr[j]; c < p.R_ptr[j + 1]; ++c) s += p.R_val[c] * s_a[p.R_idx[c] + N_q * e];
+        s_b[t] = s;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < N_f; j += blockDim.x) {  // u_f = e2c(w_f)
         double ub[NC], wb[NC];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int t = p.R_ptr[j]; t < p.R_ptr[j + 1]; ++t) s += p.R_val[t] * s_a[p.R_idx[t] + N_q * e];
-            wb[e] = s;
-        }
+        for (int e = 0; e < NC; ++e) wb[e] = s_b[j + N_f * e];
         ok &= Ph::e2c(p.law, wb, ub);
 #pragma unroll
         for (int e = 0; e < NC; ++e) ufk[j + N_f * e] = ub[e];
@@ -187,163 +187,134 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
     for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) s_uq[t] = uqk[t];
     for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) s_uf[t] = ufk[t];
     for (int t = threadIdx.x; t < N_q * D * D; t += blockDim.x) s_L[t] = Lk[t];
+    for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) s_rq[t] = 0.0;
+    for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) s_ff[t] = 0.0;
     __syncthreads();
     const double* nJfk = p.nJf + (size_t)k * D * N_f;
     const double* Jfk = p.Jf + (size_t)k * N_f;
 
-    // interface flux and the facet-node side of the facet correction
-    for (int j = threadIdx.x; j < N_f; j += blockDim.x) {
-        double uin[NC], uout[NC], F[D * NC], nrm[D];
-        const int g = p.mapP[j + (size_t)N_f * k];
-        const int kn = g / N_f, jn = g - kn * N_f;
-        const double* ufn = p.uf + (size_t)kn * N_f * NC;
+    // Flux phase over a flat list of work items, spread evenly over the workgroup:
+    //   items 0..N_f-1   the interface flux of facet node j;
+    //   volume pairs     i < j with S_m[i,j] != 0 for some m (the reference's scatter
+    //                    over the upper triangle, flux_differencing_form.jl:37-75):
+    //                    sv = sum_m S_m[i,j] sum_n (Lambda_i + Lambda_j)[m,n] F_n(u_i,u_j)
+    //                    is subtracted from r_q[i] and added to r_q[j] (S antisymmetric,
+    //                    F symmetric);
+    //   facet pairs      C_ij != 0: c_ij nJ(i,j).F(u_i, u_f_j) is subtracted from both
+    //                    r_q[i] and f_f[j] (flux_differencing_form.jl:91-168).
+    // Each flux is evaluated once; the sums go through LDS atomic adds, so their
+    // order (and the last bits of the result) may vary between calls.
+    const int n_items = N_f + p.n_items;
+    for (int t = threadIdx.x; t < n_items; t += blockDim.x) {
+        double F[D * NC];
+        if (t < N_f) {
+            const int j = t;
+            double uin[NC], uout[NC], nrm[D];
+            const double jf = Jfk[j];
+            const int g = p.mapP[j + (size_t)N_f * k];
+            const int kn = g / N_f, jn = g - kn * N_f;
+            const double* ufn = p.uf + (size_t)kn * N_f * NC;
 #pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            uin[e] = s_uf[j + N_f * e];
-            uout[e] = ufn[jn + N_f * e];
-        }
-        const double jf = Jfk[j];
-#pragma unroll
-        for (int m = 0; m < D; ++m) nrm[m] = nJfk[m + D * j] / jf;
-        Ph::two_point(p.law, uin, uout, F);
-        double a = 0.0;
-        const bool lf = (p.law.inviscid_flux == 1);
-        if (lf) a = p.law.half_lambda * Ph::wave_speed(p.law, uin, uout, nrm);
-        const double bj = p.B[j] * jf;
-        double ff[NC];
-#pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            double fnv = 0.0;
-#pragma unroll
-            for (int m = 0; m < D; ++m) fnv += F[m * NC + e] * nrm[m];
-            if (lf) fnv += a * (uin[e] - uout[e]);
-            ff[e] = fnv * bj;
-        }
-        if (p.C_kind) {
-            const int f = j / npf;
-            double hf[D];
-#pragma unroll
-            for (int m = 0; m < D; ++m) hf[m] = 0.5 * nJfk[m + D * j];
-            for (int t = p.Cc_ptr[j]; t < p.Cc_ptr[j + 1]; ++t) {
-                const int i = p.Cc_idx[t];
-                const double cij = p.Cc_val[t];
-                double ui[NC];
-#pragma unroll
-                for (int e = 0; e < NC; ++e) ui[e] = s_uq[i + N_q * e];
-                Ph::two_point(p.law, ui, uin, F);
-                double nJ[D];
-#pragma unroll
-                for (int m = 0; m < D; ++m) nJ[m] = hf[m] + half_nJq<D>(s_L, N_q, p.fn, m, f, i);
-#pragma unroll
-                for (int e = 0; e < NC; ++e) {
-                    double fnv = 0.0;
-#pragma unroll
-                    for (int m = 0; m < D; ++m) fnv += nJ[m] * F[m * NC + e];
-                    ff[e] -= cij * fnv;
-                }
+            for (int e = 0; e < NC; ++e) {
+                uin[e] = s_uf[j + N_f * e];
+                uout[e] = ufn[jn + N_f * e];
             }
+#pragma unroll
+            for (int m = 0; m < D; ++m) nrm[m] = nJfk[m + D * j] / jf;
+            Ph::two_point(p.law, uin, uout, F);
+            double a = 0.0;
+            const bool lf = (p.law.inviscid_flux == 1);
+            if (lf) a = p.law.half_lambda * Ph::wave_speed(p.law, uin, uout, nrm);
+            const double bj = p.B[j] * jf;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+                double fnv = 0.0;
+#pragma unroll
+                for (int m = 0; m < D; ++m) fnv += F[m * NC + e] * nrm[m];
+                if (lf) fnv += a * (uin[e] - uout[e]);
+                atomicAdd(&s_ff[j + N_f * e], fnv * bj);
+            }
+            continue;
         }
+        const int c = t - N_f;
+        const int ij = p.item_ij[c];
+        const int i = ij & 0xffff, j = ij >> 16;
+        double ui[NC], uj[NC];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) s_ff[j + N_f * e] = ff[e];
-    }
-
-    // volume Hadamard sum and the volume-node side of the facet correction
-    for (int i = threadIdx.x; i < N_q; i += blockDim.x) {
-        double ui[NC], uj[NC], F[D * NC], r[NC];
-        double Li[D * D];
-#pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            ui[e] = s_uq[i + N_q * e];
-            r[e] = 0.0;
-        }
-#pragma unroll
-        for (int mn = 0; mn < D * D; ++mn) Li[mn] = s_L[i + N_q * mn];
-        for (int t = p.S_ptr[i]; t < p.S_ptr[i + 1]; ++t) {
-            const int j = p.S_idx[t];
+        for (int e = 0; e < NC; ++e) ui[e] = s_uq[i + N_q * e];
+        if (c < p.n_vol_items) {
 #pragma unroll
             for (int e = 0; e < NC; ++e) uj[e] = s_uq[j + N_q * e];
             Ph::two_point(p.law, ui, uj, F);
-            // a_n = sum_m S_m[i,j] (Lambda_i + Lambda_j)[m,n]
             double an[D];
 #pragma unroll
             for (int n = 0; n < D; ++n) {
-                double s = 0.0;
+                double sv = 0.0;
 #pragma unroll
-                for (int m = 0; m < D; ++m) s += p.S_val[t * D + m] * (Li[m + D * n] + s_L[j + N_q * (m + D * n)]);
-                an[n] = s;
+                for (int m = 0; m < D; ++m)
+                    sv += p.item_val[(size_t)c * D + m] * (s_L[i + N_q * (m + D * n)] + s_L[j + N_q * (m + D * n)]);
+                an[n] = sv;
             }
 #pragma unroll
             for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
+                double sv = 0.0;
 #pragma unroll
-                for (int n = 0; n < D; ++n) s += an[n] * F[n * NC + e];
-                r[e] -= s;
+                for (int n = 0; n < D; ++n) sv += an[n] * F[n * NC + e];
+                atomicAdd(&s_rq[i + N_q * e], -sv);
+                atomicAdd(&s_rq[j + N_q * e], sv);
+            }
+        } else {
+            const double cij = p.item_val[(size_t)c * D];
+            const int f = j / npf;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) uj[e] = s_uf[j + N_f * e];
+            Ph::two_point(p.law, ui, uj, F);
+            double nJ[D];
+#pragma unroll
+            for (int m = 0; m < D; ++m) nJ[m] = 0.5 * nJfk[m + D * j] + half_nJq<D>(s_L, N_q, p.fn, m, f, i);
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+                double fnv = 0.0;
+#pragma unroll
+                for (int m = 0; m < D; ++m) fnv += nJ[m] * F[m * NC + e];
+                fnv *= cij;
+                atomicAdd(&s_rq[i + N_q * e], -fnv);
+                atomicAdd(&s_ff[j + N_f * e], -fnv);
             }
         }
-        if (p.C_kind) {
-            for (int t = p.Cr_ptr[i]; t < p.Cr_ptr[i + 1]; ++t) {
-                const int j = p.Cr_idx[t];
-                const double cij = p.Cr_val[t];
-                const int f = j / npf;
-#pragma unroll
-                for (int e = 0; e < NC; ++e) uj[e] = s_uf[j + N_f * e];
-                Ph::two_point(p.law, ui, uj, F);
-                double nJ[D];
-#pragma unroll
-                for (int m = 0; m < D; ++m) nJ[m] = 0.5 * nJfk[m + D * j] + half_nJq<D>(s_L, N_q, p.fn, m, f, i);
-#pragma unroll
-                for (int e = 0; e < NC; ++e) {
-                    double fnv = 0.0;
-#pragma unroll
-                    for (int m = 0; m < D; ++m) fnv += nJ[m] * F[m * NC + e];
-                    r[e] -= cij * fnv;
-                }
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < NC; ++e) s_rq[i + N_q * e] = r[e];
     }
     __syncthreads();
 
     // r_q -= R' f_f
-    for (int i = threadIdx.x; i < N_q; i += blockDim.x) {
-#pragma unroll
-        for (int e = 0; e < NC; ++e) {
-            double s = 0.0;
-            for (int t = p.Rt_ptr[i]; t < p.Rt_ptr[i + 1]; ++t) s += p.Rt_val[t] * s_ff[p.Rt_idx[t] + N_f * e];
-            s_rq[i + N_q * e] -= s;
-        }
+    for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {
+        const int i = t % N_q, e = t / N_q;
+        double s = 0.0;
+        for (int c = p.Rt_ptr[i]; c < p.Rt_ptr[i + 1]; ++c) s += p.Rt_val[c] * s_ff[p.Rt_idx[c] + N_f * e];
+        s_rq[t] -= s;
     }
     __syncthreads();
 
     const double* Jk = p.Jq + (size_t)k * N_q;
-    if (p.has_V) {
-        for (int j = threadIdx.x; j < N_p; j += blockDim.x) {  // V' r_q
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int i = 0; i < N_q; ++i) s += p.V[i + (size_t)N_q * j] * s_rq[i + N_q * e];
-                s_o[j + N_p * e] = s;
-            }
+    if (p.has_V) {  // over (mode or node, component) pairs, as in nodal_values
+        for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {  // V' r_q
+            const int j = t % N_p, e = t / N_p;
+            double s = 0.0;
+            for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_rq[i + N_q * e];
+            s_o[t] = s;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < N_q; i += blockDim.x) {  // J^{-1}W (V rhs)
-            const double jw = p.W[i] / Jk[i];
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_o[j + N_p * e];
-                s_t[i + N_q * e] = s * jw;
-            }
+        for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // J^{-1}W (V rhs)
+            const int i = t % N_q, e = t / N_q;
+            double s = 0.0;
+            for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_o[j + N_p * e];
+            s_t[t] = s * (p.W[i] / Jk[i]);
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < N_p; j += blockDim.x) {
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double s = 0.0;
-                for (int i = 0; i < N_q; ++i) s += p.V[i + (size_t)N_q * j] * s_t[i + N_q * e];
-                s_o[j + N_p * e] = s;
-            }
+        for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {
+            const int j = t % N_p, e = t / N_p;
+            double s = 0.0;
+            for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_t[i + N_q * e];
+            s_o[t] = s;
         }
     } else {
         for (int j = threadIdx.x; j < N_p; j += blockDim.x) {  // (W J)^{-1} r_q
@@ -366,7 +337,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
 }
 
 size_t lds_nodal_values_generic(const DevParams& p) {
-    return sizeof(double) * (size_t)(2 * p.N_p + p.N_q) * p.N_c;
+    return sizeof(double) * (size_t)(2 * p.N_p + p.N_q + p.N_f) * p.N_c;
 }
 
 size_t lds_time_derivative_generic(const DevParams& p) {

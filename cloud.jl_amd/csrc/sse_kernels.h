@@ -33,6 +33,11 @@ struct DevParams {
     // C rows: volume node i -> (facet node j, C_ij); C columns: facet node j -> (volume node i, C_ij)
     const int* Cr_ptr; const int* Cr_idx; const double* Cr_val;
     const int* Cc_ptr; const int* Cc_idx; const double* Cc_val;
+    // flux work items of the general time-derivative kernel: volume pairs i < j
+    // (item_val = S_m[i,j], m < d) then facet-correction pairs (item_val[0] = C_ij);
+    // item_ij = i | j << 16
+    int n_items, n_vol_items;
+    const int* item_ij; const double* item_val;
     // StandardForm: union sparsity of D_1..D_d by rows (node i -> (node j, D_m[i,j] for m < d))
     // and of D_1^T..D_d^T (node i -> (node j, D_m[j,i] for m < d))
     const int* D_ptr; const int* D_idx; const double* D_val;
