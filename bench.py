@@ -252,23 +252,27 @@ def bench_multi(args):
     timings: Euler on M=4 curved tets, ModalMulti p=3, EC interface flux, curl
     metrics (N_e=384; examples/euler_3d.ipynb), and the isentropic vortex on M=4
     warped triangles, ModalMulti p=4, Lax-Friedrichs (N_e=32;
-    examples/euler_vortex_2d.ipynb); then the tet scheme at M=16 (24,576 elements)
+    examples/euler_vortex_2d.ipynb), linear advection on M=4 curved triangles,
+    NodalMulti p=4, skew-symmetric StandardForm with the central flux
+    (examples/advection_2d.ipynb); then the tet scheme at M=16 (24,576 elements)
     for throughput. Time per residual on the GPU beside the CPU oracle on one
     thread (the reference's Serial() runs). Prints one JSON line."""
     import torch
 
     from sse import semi_discrete_residual
     from sse.conservation_laws import EntropyConservativeNumericalFlux
-    from sse.reference_approximation import ModalMulti
+    from sse.reference_approximation import ModalMulti, NodalMulti
     from sse.solvers import FluxDifferencingForm
     from sse.time_integration import to_device
-    from sse.workloads import density_wave_3d_config, euler_vortex_2d_modal
+    from sse.workloads import advection_2d_modal_tensor_tri, density_wave_3d_config, euler_vortex_2d_modal
 
     ec = FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux())
     cases = [("tet_p3_M4_ec", lambda: density_wave_3d_config(M=4, p=3, approx=ModalMulti, form=ec), 31.8e-3,
               "examples/euler_3d.ipynb:141-143"),
              ("tri_p4_M4_lf", lambda: euler_vortex_2d_modal(M=4, p=4, approx=ModalMulti)[:5], 541e-6,
               "examples/euler_vortex_2d.ipynb:1274-1276"),
+             ("adv_tri_p4_M4_nodal", lambda: advection_2d_modal_tensor_tri(M=4, p=4, approx=NodalMulti), 68.7e-3,
+              "examples/advection_2d.ipynb:1260 (Threaded)"),
              ("tet_p3_M16_ec", lambda: density_wave_3d_config(M=16, p=3, approx=ModalMulti, form=ec), None, None)]
     if args.multi_cases:
         cases = [c for c in cases if c[0] in args.multi_cases.split(",")]
@@ -512,7 +516,8 @@ def main():
                          "batched-vs-loop comparison over 256 vortex problems")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--multi-cases", default="",
-                    help="multi: comma-separated subset of tet_p3_M4_ec, tri_p4_M4_lf, tet_p3_M16_ec")
+                    help="multi: comma-separated subset of tet_p3_M4_ec, tri_p4_M4_lf, adv_tri_p4_M4_nodal, "
+                         "tet_p3_M16_ec")
     ap.add_argument("--adj-mesh", type=int, default=12)
     ap.add_argument("--autotune", action="store_true", help="pick the residual's kernel configuration by timing")
     ap.add_argument("--ad-mesh", type=int, default=24)

@@ -48,6 +48,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
                 double s;
                 if (p.has_V) {
                     s = 0.0;
+#pragma unroll 8
                     for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_u[j + N_p * e];
                 } else {
                     s = s_u[i + N_p * e];
@@ -88,6 +89,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
         for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // u_q = V u
             const int i = t % N_q, e = t / N_q;
             double s = 0.0;
+#pragma unroll 8
             for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_u[j + N_p * e];
             s_a[t] = s;
         }
@@ -105,6 +107,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
         for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {  // w = V' (WJ w_q)
             const int j = t % N_p, e = t / N_p;
             double s = 0.0;
+#pragma unroll 8
             for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_a[i + N_q * e];
             s_w[t] = s;
         }
@@ -112,6 +115,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
         for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // J^{-1}W (V w)
             const int i = t % N_q, e = t / N_q;
             double s = 0.0;
+#pragma unroll 8
             for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_w[j + N_p * e];
             s_a[t] = s * (p.W[i] / Jk[i]);
         }
@@ -119,6 +123,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
         for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {  // w = V' (...)
             const int j = t % N_p, e = t / N_p;
             double s = 0.0;
+#pragma unroll 8
             for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_a[i + N_q * e];
             s_w[t] = s;
         }
@@ -126,6 +131,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
         for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // w_q = V w
             const int i = t % N_q, e = t / N_q;
             double s = 0.0;
+#pragma unroll 8
             for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_w[j + N_p * e];
             s_a[t] = s;
         }
@@ -143,6 +149,7 @@ __global__ void __launch_bounds__(256) k_nodal_values_generic(DevParams p, const
     for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) {  // w_f = R w_q
         const int j = t % N_f, e = t / N_f;
         double s = 0.0;
+#pragma unroll 8
         for (int c = p.R_ptr[j]; c < p.R_ptr[j + 1]; ++c) s += p.R_val[c] * s_a[p.R_idx[c] + N_q * e];
         s_b[t] = s;
     }
@@ -289,6 +296,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
     for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {
         const int i = t % N_q, e = t / N_q;
         double s = 0.0;
+#pragma unroll 8
         for (int c = p.Rt_ptr[i]; c < p.Rt_ptr[i + 1]; ++c) s += p.Rt_val[c] * s_ff[p.Rt_idx[c] + N_f * e];
         s_rq[t] -= s;
     }
@@ -299,6 +307,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
         for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {  // V' r_q
             const int j = t % N_p, e = t / N_p;
             double s = 0.0;
+#pragma unroll 8
             for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_rq[i + N_q * e];
             s_o[t] = s;
         }
@@ -306,6 +315,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
         for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) {  // J^{-1}W (V rhs)
             const int i = t % N_q, e = t / N_q;
             double s = 0.0;
+#pragma unroll 8
             for (int j = 0; j < N_p; ++j) s += p.V[i + (size_t)N_q * j] * s_o[j + N_p * e];
             s_t[t] = s * (p.W[i] / Jk[i]);
         }
@@ -313,6 +323,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
         for (int t = threadIdx.x; t < N_p * NC; t += blockDim.x) {
             const int j = t % N_p, e = t / N_p;
             double s = 0.0;
+#pragma unroll 8
             for (int i = 0; i < N_q; ++i) s += p.Vt[j + (size_t)N_p * i] * s_t[i + N_q * e];
             s_o[t] = s;
         }
