@@ -871,7 +871,7 @@ int create_handle(const sse_desc* D, int members, sse_handle** out) {
         return cleanup_fail(rc);
     }
     if (!P.standard) {  // flux work items: volume pairs i < j, then the nonzeros of C
-        if (N_q >= 65536 || N_f >= 65536) {
+        if (N_q >= 32768 || N_f >= 32768) {  // item_ij packs i | j << 16 into an int
             h->err = "too many nodes per element";
             return cleanup_fail(SSE_ERR_UNSUPPORTED);
         }

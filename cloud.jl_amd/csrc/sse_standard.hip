@@ -54,6 +54,7 @@ __device__ void volume_term(const DevParams& p, int k, int n, const double* s_x,
         for (int e = 0; e < NC; ++e) {
             // sum_m D_m' (h W Lambda_mn x): row i gathers column i of each D_m
             double s = 0.0;
+#pragma unroll 4
             for (int t = p.Dt_ptr[i]; t < p.Dt_ptr[i + 1]; ++t) {
                 const int j = p.Dt_idx[t];
                 double a = 0.0;
@@ -64,6 +65,7 @@ __device__ void volume_term(const DevParams& p, int k, int n, const double* s_x,
             s *= h;
             if (p.skew) {  // - h W Lambda_mn (D_m x)
                 double g = 0.0;
+#pragma unroll 4
                 for (int t = p.D_ptr[i]; t < p.D_ptr[i + 1]; ++t) {
                     const int j = p.D_idx[t];
                     double a = 0.0;
@@ -81,6 +83,7 @@ __device__ void volume_term(const DevParams& p, int k, int n, const double* s_x,
         for (int j = threadIdx.x; j < Nf; j += blockDim.x) {
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
+#pragma unroll 8
                 for (int t = p.R_ptr[j]; t < p.R_ptr[j + 1]; ++t) s += p.R_val[t] * s_x[p.R_idx[t] + Nq * e];
                 s_tf[j + Nf * e] = 0.5 * p.B[j] * nJfk[n + D * j] * s;
             }
@@ -89,6 +92,7 @@ __device__ void volume_term(const DevParams& p, int k, int n, const double* s_x,
         for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
+#pragma unroll 8
                 for (int t = p.Rt_ptr[i]; t < p.Rt_ptr[i + 1]; ++t) s += p.Rt_val[t] * s_tf[p.Rt_idx[t] + Nf * e];
                 s_r[i + Nq * e] += s;
             }
@@ -185,6 +189,7 @@ __global__ void __launch_bounds__(256) k_auxiliary_standard(DevParams p) {
         __syncthreads();
         for (int i = threadIdx.x; i < Nq; i += blockDim.x) {
             double s = 0.0;
+#pragma unroll 8
             for (int t = p.Rt_ptr[i]; t < p.Rt_ptr[i + 1]; ++t) s += p.Rt_val[t] * s_tf[p.Rt_idx[t]];
             s_r[i] = -s_r[i] + s;
         }
@@ -206,6 +211,7 @@ __global__ void __launch_bounds__(256) k_auxiliary_standard(DevParams p) {
         __syncthreads();
         for (int j = threadIdx.x; j < Nf; j += blockDim.x) {
             double s = 0.0;
+#pragma unroll 8
             for (int t = p.R_ptr[j]; t < p.R_ptr[j + 1]; ++t) s += p.R_val[t] * s_x[p.R_idx[t]];
             qfk[j] = s;
         }
@@ -301,6 +307,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_standard(DevParams p, d
 #pragma unroll
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
+#pragma unroll 8
             for (int t = p.Rt_ptr[i]; t < p.Rt_ptr[i + 1]; ++t) s += p.Rt_val[t] * s_ff[p.Rt_idx[t] + Nf * e];
             s_r[i + Nq * e] -= s;
         }

@@ -105,3 +105,13 @@ def test_multi_residual_vjp_dot_product_against_oracle_jvp(name):
     rhs = float(np.sum(ubar * du)) + lam_bar * dlam
     scale = float(np.sum(np.abs(ybar * dr)))
     assert abs(lhs - rhs) <= 1e-11 * scale, (name, lhs, rhs, scale)
+
+
+def test_multi_repeated_residuals_agree_to_rounding():
+    # the general kernels add each flux to both of its nodes with LDS atomic adds,
+    # so the order of those sums may change between calls; nothing else does
+    ode = CASES["wave3d_modal_multi_p3_lf"]()
+    u = _perturbed(ode)
+    first = _gpu_residual(ode, u)
+    for _ in range(3):
+        assert _rel(_gpu_residual(ode, u), first) <= 1e-14
