@@ -6,16 +6,16 @@
 // set of 1D lines (src/Solvers/operators.jl:182-200 with D_xi of
 // src/SpatialDiscretizations/SpatialDiscretizations.jl:414-418): node i pairs
 // with every node j that differs from i in exactly one tensor index (family f).
-// Each volume node's thread gathers its row of the Hadamard sum
-//   r_q[i,e] -= sum_{f, b != a_f(i)} sum_n a_n F(u_i, u_j)[e,n],
-//   a_n = sum_m S_m[i,j] (Lambda_i + Lambda_j)[m,n]
-// (the reference's flux_difference!, src/Solvers/flux_differencing_form.jl:37-75,
-// summed by rows; S antisymmetric and F symmetric make the two equal). Each
-// facet-correction pair (C nonzero) is evaluated once, by column chunks balanced
-// across the team: the same two-point term enters the volume node's row and the
-// facet node's column, and both sums are accumulated in LDS with atomic adds.
-// Those are the only sums whose order varies from run to run; the Hadamard rows
-// and everything else have a fixed order.
+// Each unordered volume pair is evaluated once, by the thread of one of its nodes
+// (cyclic offsets along the line), and its term
+//   c_e = sum_n a_n F(u_i, u_j)[e,n],  a_n = sum_m S_m[i,j] (Lambda_i + Lambda_j)[m,n]
+// enters r_q[i] as -c (in registers) and r_q[j] as +c (S antisymmetric, F
+// symmetric; an LDS atomic add): the reference's scatter over i < j,
+// src/Solvers/flux_differencing_form.jl:37-75. Each facet-correction pair (C
+// nonzero) is evaluated once too, by column chunks balanced across the team: the
+// same two-point term enters the volume node's row and the facet node's column,
+// both accumulated in LDS with atomic adds. These atomic sums are the only ones
+// whose order varies from run to run.
 //
 // A team of TS threads owns one element. The exterior facet state is read
 // contiguously: phase 1 writes each facet state also into the neighbour's
@@ -78,7 +78,12 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
     constexpr int D = Sh::D, NC = Sh::NC, NH = Sh::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
     constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
     constexpr int MAXR = TENSOR_MAXR;
-    constexpr int NPART = D * (N1 - 1), QH = (NPART + SPLIT - 1) / SPLIT;  // volume partners per thread
+    // volume pairs: on each line node a evaluates its pairs with the nodes at cyclic
+    // offsets 1..DH from it, which covers every unordered pair of an odd-length line
+    // once; on an even-length line the offset N1/2 pairs nodes both ways and only
+    // the node in the lower half of the line evaluates it
+    constexpr int DH = (N1 - 1) / 2, EVEN = N1 % 2 == 0 ? 1 : 0, PER_F = DH + EVEN;
+    constexpr int NPART = D * PER_F, QH = (NPART + SPLIT - 1) / SPLIT;  // volume pairs per thread
     static_assert(NQ <= NT && NF <= NT, "one node per thread");
     extern __shared__ double smem[];
     const DevParams& p = tp.base;
@@ -184,12 +189,12 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             // tensor indices of node tl (last index fastest) and the family strides
             const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
             const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
-            int f = q0 / (N1 - 1), dlt = q0 - f * (N1 - 1) + 1;
+            int f = q0 / PER_F, dlt = q0 - f * PER_F + 1;
 #pragma unroll 1
             for (int q = q0; q < q1; ++q) {
-                {
-                    const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
-                    const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
+                const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
+                const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
+                if (!(EVEN && 2 * dlt == N1 && 2 * af >= N1)) {
                     const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
                     const int j = tl + (b - af) * stride;
 #pragma unroll
@@ -203,10 +208,15 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                         for (int n = 0; n < D; ++n) av[n] += s * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
                     }
                     ec_dot<D>(p.law, hi, hj, av, c);
+                    // S antisymmetric, F symmetric: node j's row receives +c, through
+                    // the row accumulator that phase D subtracts
 #pragma unroll
-                    for (int e = 0; e < NC; ++e) r[e] -= c[e];
+                    for (int e = 0; e < NC; ++e) {
+                        r[e] -= c[e];
+                        atomicAdd(&srow[e * NQ + j], -c[e]);
+                    }
                 }
-                if (++dlt == N1) {  // next family
+                if (++dlt == PER_F + 1) {  // next family
                     dlt = 1;
                     ++f;
                 }

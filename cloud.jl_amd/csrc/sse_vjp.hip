@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v1(DevParams p, const double* __rest
     for (int j = threadIdx.x; j < Nf; j += blockDim.x)
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
+#pragma unroll 8
             for (int t = p.R_ptr[j]; t < p.R_ptr[j + 1]; ++t) s += p.R_val[t] * s_r[p.R_idx[t] + Nq * e];
             fk[j + Nf * e] = -s;
         }
@@ -220,6 +221,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
     for (int j = threadIdx.x; j < Nf; j += blockDim.x)
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
+#pragma unroll 8
             for (int t = p.R_ptr[j]; t < p.R_ptr[j + 1]; ++t) s += p.R_val[t] * s_wq[p.R_idx[t] + Nq * e];
             s_wf[j + Nf * e] = s;
         }
@@ -396,6 +398,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
         for (int i = threadIdx.x; i < Nq; i += blockDim.x)
             for (int e = 0; e < NC; ++e) {
                 double s = 0.0;
+#pragma unroll 8
                 for (int t = p.Rt_ptr[i]; t < p.Rt_ptr[i + 1]; ++t) s += p.Rt_val[t] * s_ubf[p.Rt_idx[t] + Nf * e];
                 if (p.has_V) s_t[i + Nq * e] += s;
                 else s_t[i + Nq * e] = s;
@@ -444,6 +447,7 @@ __global__ void __launch_bounds__(VT) k_vjp_v2(DevParams p, const double* __rest
         for (int i = threadIdx.x; i < Nq; i += blockDim.x)
             for (int e = 0; e < NC; ++e) {
                 double s = s_ubq[i + Nq * e];
+#pragma unroll 8
                 for (int t = p.Rt_ptr[i]; t < p.Rt_ptr[i + 1]; ++t) s += p.Rt_val[t] * s_ubf[p.Rt_idx[t] + Nf * e];
                 outk[i + Np * e] = s;
             }
@@ -547,6 +551,7 @@ __global__ void __launch_bounds__(VT) k_entropy_grad(DevParams p, const double* 
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
             if (p.has_V) {
+#pragma unroll 8
                 for (int j = 0; j < Np; ++j) s += p.V[i + (size_t)Nq * j] * s_u[j + Np * e];
             } else {
                 s = s_u[i + Np * e];
@@ -564,6 +569,7 @@ __global__ void __launch_bounds__(VT) k_entropy_grad(DevParams p, const double* 
         for (int e = 0; e < NC; ++e) {
             double s = 0.0;
             if (p.has_V) {
+#pragma unroll 8
                 for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * j] * s_w[i + Nq * e];
             } else {
                 s = s_w[j + Nq * e];

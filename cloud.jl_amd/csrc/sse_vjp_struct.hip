@@ -157,7 +157,10 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     using SF = SumFact<ELEM, P, T::TS>;
     constexpr int D = Sh::D, NC = Sh::NC, NH = L::NH, NP = Sh::NP, NQ = Sh::NQ, NF = Sh::NF, NPF = Sh::NPF;
     constexpr int N1 = Sh::N1, TS = T::TS, MAXITEM = L::MAXITEM, SPLIT = T::SPLIT, NT = TS / SPLIT;
-    constexpr int NPART = D * (N1 - 1), QH = (NPART + SPLIT - 1) / SPLIT;
+    // each unordered line pair once, as in the forward kernel: cyclic offsets
+    // 1..DH from the node, and the offset N1/2 of an even line from its lower half
+    constexpr int DH = (N1 - 1) / 2, EVEN = N1 % 2 == 0 ? 1 : 0, PER_F = DH + EVEN;
+    constexpr int NPART = D * PER_F, QH = (NPART + SPLIT - 1) / SPLIT;
     static_assert(NQ <= NT && NF <= NT, "one node per thread");
     extern __shared__ double sm[];
     const DevParams& p = tp.base;
@@ -257,26 +260,37 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         double hj[NH], av[D], gb[NC];
         const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
         const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
-        int f = q0 / (N1 - 1), dlt = q0 - f * (N1 - 1) + 1;
+        int f = q0 / PER_F, dlt = q0 - f * PER_F + 1;
 #pragma unroll 1
         for (int q = q0; q < q1; ++q) {
             const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
             const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
-            const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
-            const int j = tl + (b - af) * stride;
+            if (!(EVEN && 2 * dlt == N1 && 2 * af >= N1)) {
+                const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+                const int j = tl + (b - af) * stride;
+                double ubj[NC];
 #pragma unroll
-            for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+                for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
 #pragma unroll
-            for (int n = 0; n < D; ++n) av[n] = 0.0;
-            for (int m = f; m < D; ++m) {
-                const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+                for (int n = 0; n < D; ++n) av[n] = 0.0;
+                for (int m = f; m < D; ++m) {
+                    const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
 #pragma unroll
-                for (int n = 0; n < D; ++n) av[n] += s * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
+                    for (int n = 0; n < D; ++n) av[n] += s * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
+                }
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
+                    ubj[e] = 0.0;
+                }
+                // the pair's term enters r_q[i] as -c and r_q[j] as +c: one evaluation
+                // pulls (rbar_j - rbar_i) back to both states; node j's share goes
+                // through the row accumulator that phase D adds
+                A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
             }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
-            A::ec_dot_adj_h(p.law, hi, hj, av, gb, true, ubq);
-            if (++dlt == N1) {
+            if (++dlt == PER_F + 1) {
                 dlt = 1;
                 ++f;
             }
