@@ -202,7 +202,9 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                     // a_n = sum_{m >= f} S_m[i,j] (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
 #pragma unroll
                     for (int n = 0; n < D; ++n) av[n] = 0.0;
-                    for (int m = f; m < D; ++m) {
+#pragma unroll
+                    for (int m = 0; m < D; ++m) {  // static indices into Li (m < f: S_m = 0)
+                        if (m < f) continue;
                         const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
 #pragma unroll
                         for (int n = 0; n < D; ++n) av[n] += s * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
