@@ -1,11 +1,11 @@
 // sse_adjoint.h -- reverse-mode (adjoint) derivatives of the Euler physics of
 // sse_physics.h, written by hand from the forward formulas:
 //   logmean / inv_logmean           src/ConservationLaws/ConservationLaws.jl:132-156
-//   EC two-point flux (contracted)  src/ConservationLaws/euler_navierstokes.jl:358-401
+//   EC two-point flux (contracted)  src/ConservationLaws/euler_navierstokes.jl:171-225
 //   Lax-Friedrichs interface flux   src/ConservationLaws/ConservationLaws.jl:75-101,
-//                                   wave speed euler_navierstokes.jl:339-356
-//   conservative_to_entropy!        euler_navierstokes.jl:306-319
-//   entropy_to_conservative!        euler_navierstokes.jl:321-337
+//                                   wave speed euler_navierstokes.jl:133-150
+//   conservative_to_entropy!        euler_navierstokes.jl:100-113
+//   entropy_to_conservative!        euler_navierstokes.jl:115-131
 // Each *_adj function adds the pull-back of an output cotangent to the input
 // cotangents. Derivatives follow the branch the forward evaluation takes
 // (series or logarithm in logmean; the larger side in the wave speed, the
@@ -420,7 +420,7 @@ struct EulerAdj {
                                     const double* fb, double* uib, double* uob) {
         ec_dot_adj(P, ui, uo, n, fb, uib, uob);
         if (!lf) return 0.0;
-        // wave speed (euler_navierstokes.jl:339-356)
+        // wave speed (euler_navierstokes.jl:133-150)
         double si = 0.0, so = 0.0, vni = 0.0, vno = 0.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) {

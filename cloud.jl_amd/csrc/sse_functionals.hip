@@ -40,7 +40,7 @@ __device__ double block_sum(double v, double* red) {
     return r;
 }
 
-// S(u) (euler_navierstokes.jl:299-304; linear_advection_diffusion.jl:273) and the
+// S(u) (euler_navierstokes.jl:93-98; linear_advection_diffusion.jl:104) and the
 // entropy variables
 template <int D, int LAW>
 __device__ double entropy_of(const LawParams& P, const double* u) {

@@ -8,7 +8,7 @@
 //
 //   nodal_values     src/Solvers/flux_differencing_form.jl:171-292
 //   time_derivative  src/Solvers/flux_differencing_form.jl:294-347
-//   mass solve (WA)  src/Solvers/mass_matrix.jl:406-417
+//   mass solve (WA)  src/Solvers/mass_matrix.jl:185-196
 //
 // The time derivative evaluates every two-point flux once: the interface fluxes,
 // the volume pairs i < j of the Hadamard sum and the facet-correction pairs form

@@ -2,13 +2,13 @@
 //
 // Restates the reference's conservation-law functions:
 //   logmean / inv_logmean           src/ConservationLaws/ConservationLaws.jl:132-156
-//   conservative_to_entropy!        src/ConservationLaws/euler_navierstokes.jl:306-319
-//   entropy_to_conservative!        src/ConservationLaws/euler_navierstokes.jl:321-337
-//   wave_speed (Euler / advection)  src/ConservationLaws/euler_navierstokes.jl:339-356,
-//                                   src/ConservationLaws/linear_advection_diffusion.jl:275-280
+//   conservative_to_entropy!        src/ConservationLaws/euler_navierstokes.jl:100-113
+//   entropy_to_conservative!        src/ConservationLaws/euler_navierstokes.jl:115-131
+//   wave_speed (Euler / advection)  src/ConservationLaws/euler_navierstokes.jl:133-150,
+//                                   src/ConservationLaws/linear_advection_diffusion.jl:106-111
 //   compute_two_point_flux (EC Ranocha, conservative, advection)
-//                                   src/ConservationLaws/euler_navierstokes.jl:358-401,
-//                                   src/ConservationLaws/linear_advection_diffusion.jl:282-288
+//                                   src/ConservationLaws/euler_navierstokes.jl:171-225,
+//                                   src/ConservationLaws/linear_advection_diffusion.jl:113-128
 // Flux tensors are stored F[n * NC + e] (direction n, variable e).
 #pragma once
 
@@ -169,7 +169,7 @@ struct Physics<D, LAW_EULER> {
 //   c_0     = rho_avg (a . v_avg)
 //   c_{m+1} = v_avg_m c_0 + a_m p_avg
 //   c_{D+1} = C c_0 + (p_L (a . v_R) + p_R (a . v_L)) / 2
-// which is the tensor of euler_navierstokes.jl:377-401 contracted with a.
+// which is the tensor of euler_navierstokes.jl:171-195 contracted with a.
 // ---------------------------------------------------------------------------
 template <int D>
 __device__ __forceinline__ void hoist(const LawParams& P, const double* u, double* h) {
@@ -253,7 +253,7 @@ __device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, con
     c[D + 1] = c0 * Cc + 0.5 * (pL * avR + pR * avL);
 }
 
-// conservative_to_entropy! / entropy_to_conservative! (euler_navierstokes.jl:306-337)
+// conservative_to_entropy! / entropy_to_conservative! (euler_navierstokes.jl:100-131)
 // with the powers written through logarithms:
 //   log(p / rho^gamma)                 = log p - gamma log rho
 //   ((gamma-1)/(-w_E)^gamma)^(1/(gamma-1)) exp(-s/(gamma-1))
@@ -292,7 +292,7 @@ __device__ __forceinline__ bool e2c_fast(const LawParams& P, const double* win, 
     return win[D + 1] < 0.0;
 }
 
-// wave_speed from hoisted states (euler_navierstokes.jl:339-356)
+// wave_speed from hoisted states (euler_navierstokes.jl:133-150)
 template <int D>
 __device__ __forceinline__ double wave_speed_h(const LawParams& P, const double* hi, const double* ho,
                                                const double* n) {

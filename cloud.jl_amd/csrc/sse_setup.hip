@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(SETUP_THREADS) k_geometric_factors(GeoArgs a) 
     __syncthreads();
     if (a.curl) {
         // J at the mapping nodes; for triangles also the metric terms (geometric_factors,
-        // mesh.jl:294-299: rxJ = ys, sxJ = -yr, ryJ = -xs, syJ = xr)
+        // mesh.jl:306: rxJ = ys, sxJ = -yr, ryJ = -xs, syJ = xr)
         for (int i = tt; i < N_p; i += SETUP_THREADS) {
             double A[9], Lm[9], J;
             for (int q = 0; q < dd; ++q) A[q] = sm[L.dX + q * N_p + i];

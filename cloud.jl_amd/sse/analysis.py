@@ -7,7 +7,7 @@ entropy), each with ``evaluate_conservation(u)`` and
 ``evaluate_conservation_residual(u, dudt)``, computed by libsse_hip's
 ``sse_functional`` with a fixed summation order; ``ErrorAnalysis``
 (src/Analysis/error.jl:1-91) through ``sse_l2_error``; and ``analyze`` over the
-time steps of a results directory (conservation.jl:224-274), which reads the
+time steps of a results directory (conservation.jl:191-274), which reads the
 solutions that ``io.SaveCallback`` wrote.
 """
 
@@ -126,10 +126,10 @@ def analyze(analysis, *args, normalize: bool = False, **kw):
 
     - ``analyze(ErrorAnalysis, sol, exact_solution, t=0.0; normalize)`` (error.jl:58-91):
       the L2 error of each variable;
-    - ``analyze(ConservationAnalysis, time_steps::list; normalize)`` (conservation.jl:251-274,
-      entropy :190-220): E and dE/dt at the written steps of ``analysis.results_path``;
+    - ``analyze(ConservationAnalysis, time_steps::list; normalize)`` (conservation.jl:236-274,
+      entropy :191-219): E and dE/dt at the written steps of ``analysis.results_path``;
     - ``analyze(ConservationAnalysis, initial_time_step=0, final_time_step="final")``
-      (conservation.jl:234-249): (initial, final, difference).
+      (conservation.jl:221-234): (initial, final, difference).
     """
     if isinstance(analysis, ErrorAnalysis):
         sol, exact_solution = args[0], args[1]

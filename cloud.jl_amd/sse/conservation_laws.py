@@ -36,7 +36,7 @@ class EulerEquations:
 
 @dataclass
 class LinearAdvectionEquation:
-    """``LinearAdvectionEquation(a)`` (``linear_advection_diffusion.jl:179-188``)."""
+    """``LinearAdvectionEquation(a)`` (``linear_advection_diffusion.jl:1-19``)."""
     a: Tuple[float, ...]
 
     @property
@@ -54,7 +54,7 @@ class LinearAdvectionEquation:
 
 @dataclass
 class LinearAdvectionDiffusionEquation:
-    """``LinearAdvectionDiffusionEquation(a, b)`` (``linear_advection_diffusion.jl:199-211``)."""
+    """``LinearAdvectionDiffusionEquation(a, b)`` (``linear_advection_diffusion.jl:21-51``)."""
     a: Tuple[float, ...]
     b: float
 
@@ -104,7 +104,7 @@ class ConservativeFlux:
 
 # ----------------------------------------------------------------------------- physics (host, NumPy)
 def conservative_to_entropy(law: EulerEquations, u: np.ndarray) -> np.ndarray:
-    """(``euler_navierstokes.jl:306-319``); u has shape (..., N_c)."""
+    """(``euler_navierstokes.jl:100-113``); u has shape (..., N_c)."""
     g = law.gamma
     d = law.d
     k = (0.5 / u[..., 0]) * (u[..., 1:d + 1] ** 2).sum(-1)
@@ -118,8 +118,8 @@ def conservative_to_entropy(law: EulerEquations, u: np.ndarray) -> np.ndarray:
 
 
 def entropy(law, u: np.ndarray) -> np.ndarray:
-    """Mathematical entropy S(u) (``euler_navierstokes.jl:299-304``;
-    ``linear_advection_diffusion.jl:273``)."""
+    """Mathematical entropy S(u) (``euler_navierstokes.jl:93-98``;
+    ``linear_advection_diffusion.jl:104``)."""
     if isinstance(law, EulerEquations):
         g = law.gamma
         d = law.d
@@ -131,7 +131,7 @@ def entropy(law, u: np.ndarray) -> np.ndarray:
 # ----------------------------------------------------------------------------- initial data
 @dataclass
 class IsentropicVortex:
-    """``IsentropicVortex`` (``euler_navierstokes.jl:440-472``), p = ρ^γ / γ."""
+    """``IsentropicVortex`` (``euler_navierstokes.jl:232-274``), p = ρ^γ / γ."""
     gamma: float = 1.4
     Ma: float = 0.4
     theta: float = math.pi / 4
@@ -190,7 +190,7 @@ class ShuVortex:
 @dataclass
 class EulerPeriodicTest:
     """ρ = 1 + strength sin(2π Σx / L), unit velocity, p = 1
-    (``euler_navierstokes.jl:482-500``)."""
+    (``euler_navierstokes.jl:276-300``)."""
     d: int
     gamma: float = 1.4
     strength: float = 0.2

@@ -132,7 +132,7 @@ def load_snapshots(results_path: str, time_steps: List[int]):
 
 
 def load_snapshots_with_derivatives(results_path: str, time_steps: List[int]):
-    """(``load.jl:52-70``): (U, dU, t_s)."""
+    """(``load.jl:51-69``): (U, dU, t_s)."""
     U, dU, times = [], [], []
     for s in time_steps:
         u, du, t = load_solution(results_path, s, load_du=True)

@@ -69,16 +69,16 @@ class MI355X:
 
 @dataclass
 class WeightAdjustedSolver:
-    """WADG with M⁻¹ = I for orthonormal bases (``mass_matrix.jl:280-296``)."""
+    """WADG with M⁻¹ = I for orthonormal bases (``mass_matrix.jl:11-17, 59-75``)."""
     Minv: Optional[np.ndarray] = None
 
 
 class DiagonalSolver:
-    """W J diagonal mass matrix for nodal V = I (``mass_matrix.jl:329-336``)."""
+    """W J diagonal mass matrix for nodal V = I (``mass_matrix.jl:7-9, 108-115``)."""
 
 
 def default_mass_matrix_solver(sd: SpatialDiscretization):
-    """(``mass_matrix.jl:19-24, 262-269``): diagonal for nodal V = I, else WADG."""
+    """(``mass_matrix.jl:19-24, 41-48``): diagonal for nodal V = I, else WADG."""
     if sd.reference_approximation.V is None:
         return DiagonalSolver()
     return WeightAdjustedSolver()

@@ -1,7 +1,7 @@
 """Fixed-step time integration of the semi-discrete system on the GPU.
 
 The reference hands its ``ODEProblem`` to OrdinaryDiffEq and integrates with
-``CarpenterKennedy2N54()`` at fixed dt (``test/test_driver.jl:221-227``). Here
+``CarpenterKennedy2N54()`` at fixed dt (``test/test_driver.jl:77-83``). Here
 ``solve`` advances the state with libsse_hip's ``sse_rk_steps``, which fuses
 each stage's low-storage update into the residual kernel.
 """

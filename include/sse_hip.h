@@ -53,7 +53,7 @@ typedef enum {
     SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX = 3
 } sse_inviscid_flux;
 
-/* mapping forms of StandardForm (src/Solvers/Solvers.jl:74-75; operators.jl:1-37, 83-160) */
+/* mapping forms of StandardForm (src/Solvers/Solvers.jl:75-76; operators.jl:1-37, 83-160) */
 typedef enum { SSE_SKEW_SYMMETRIC_MAPPING = 1, SSE_STANDARD_MAPPING = 2 } sse_mapping_form;
 
 /* viscous numerical fluxes (src/ConservationLaws/ConservationLaws.jl:64-67) */
@@ -74,7 +74,7 @@ typedef enum { SSE_DEVICE = 0, SSE_HOST = 1 } sse_memory;
 /*
  * Everything the reference's `Solver{..., FluxDifferencingOperators, ...}`
  * holds for one discretisation (src/Solvers/operators.jl:39-81,
- * src/Solvers/mass_matrix.jl:280-296, src/Solvers/Solvers.jl:311-333).
+ * src/Solvers/mass_matrix.jl:11-17, 59-75, src/Solvers/Solvers.jl:311-333).
  * All pointers are host pointers; sse_create copies what it needs.
  */
 typedef struct sse_desc {
@@ -116,7 +116,7 @@ typedef struct sse_desc {
 
     /* Optional sum factorisation of V: the fields of the reference's
      * WarpedTensorProductMap2D/3D (src/MatrixFreeOperators/warped_product_2d.jl:2-23,
-     * warped_product_3d.jl:91-124, built by src/SpatialDiscretizations/tensor_simplex.jl:84-140),
+     * warped_product_3d.jl:2-35, built by src/SpatialDiscretizations/tensor_simplex.jl:84-140),
      * modes ordered by nested loops over (i, j[, k]) with i + j + k <= V_p.
      * V_p = 0 means "not given" (V is then applied densely). */
     const double* V_A;          /* M1 x (V_p+1) */
@@ -146,7 +146,7 @@ typedef struct sse_handle sse_handle;
 #define SSE_NUM_KERNELS 2
 
 /* Replaces Solver construction (src/Solvers/Solvers.jl:311-333) and
- * WeightAdjustedSolver (src/Solvers/mass_matrix.jl:280-296): validates the
+ * WeightAdjustedSolver (src/Solvers/mass_matrix.jl:11-17, 59-75): validates the
  * descriptor, copies operators and geometry to the device, builds the kernels'
  * sparse tables. On failure *out is NULL and sse_last_error(NULL) explains. */
 int sse_create(const sse_desc* desc, sse_handle** out);
@@ -160,7 +160,7 @@ int sse_residual(sse_handle* h, const double* u, double* dudt, double t, int32_t
 /* Advances u (N_p, N_c, N_e) by nsteps steps of the 5-stage, 4th-order
  * low-storage Runge-Kutta method of Carpenter & Kennedy (1994), the
  * `CarpenterKennedy2N54()` integrator the reference hands its ODEProblem to
- * (test/test_driver.jl:221-227), with the stage update fused into the residual. */
+ * (test/test_driver.jl:77-83), with the stage update fused into the residual. */
 int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t where);
 
 /* Runs the residual nrep times on device buffers and reports the mean duration
@@ -296,7 +296,7 @@ void sse_batch_destroy(sse_batch* b);
  * through sse_last_error(NULL). `where` says where the element arrays live; the reference
  * operators in sse_geometry_ops are host pointers. All arrays are column-major. */
 
-/* metric types (src/SpatialDiscretizations/SpatialDiscretizations.jl:34-36) */
+/* metric types (src/SpatialDiscretizations/SpatialDiscretizations.jl:154-156) */
 typedef enum { SSE_METRICS_EXACT = 1, SSE_METRICS_CURL = 2 } sse_metrics;
 
 /* The RefElemData fields GeometricFactors reads (StartUpDG's RefElemData: Drst, Vq, Vf,

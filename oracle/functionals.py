@@ -43,7 +43,7 @@ def energy(sd, u):
 
 
 def _entropy_pointwise(law, uq):
-    """S(u) (euler_navierstokes.jl:299-304; linear_advection_diffusion.jl:273)."""
+    """S(u) (euler_navierstokes.jl:93-98; linear_advection_diffusion.jl:104)."""
     if hasattr(law, "gamma"):
         g, d = law.gamma, law.d
         p = (g - 1.0) * (uq[..., d + 1] - 0.5 / uq[..., 0] * (uq[..., 1:d + 1] ** 2).sum(-1))

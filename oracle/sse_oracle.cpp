@@ -21,7 +21,7 @@
 //   BR1 auxiliary variable / time deriv.      src/Solvers/standard_form_second_order.jl:1-75
 //   volume operators (skew / standard mapping) src/Solvers/operators.jl:1-37, 83-160
 //   BR1 numerical fluxes                       src/ConservationLaws/linear_advection_diffusion.jl:54-102
-//   CarpenterKennedy2N54 (OrdinaryDiffEq)      used at test/test_driver.jl:221-227
+//   CarpenterKennedy2N54 (OrdinaryDiffEq)      used at test/test_driver.jl:77-83
 //
 // The restatement is templated on the scalar type so that a dual-number
 // instantiation can differentiate it.
@@ -113,8 +113,8 @@ struct Law {
         return log(y / x) / (y - x);
     }
 
-    // compute_two_point_flux, tensor form F[e + N_c*n] (euler_navierstokes.jl:358-401;
-    // linear_advection_diffusion.jl:282-288)
+    // compute_two_point_flux, tensor form F[e + N_c*n] (euler_navierstokes.jl:171-225;
+    // linear_advection_diffusion.jl:113-128)
     void two_point(int flux, const T* uL, const T* uR, T* F) const {
         if (kind == SSE_EULER) {
             if (flux == SSE_TWO_POINT_CONSERVATIVE) {
@@ -168,7 +168,7 @@ struct Law {
         }
     }
 
-    // wave_speed (euler_navierstokes.jl:339-356; linear_advection_diffusion.jl:275-280)
+    // wave_speed (euler_navierstokes.jl:133-150; linear_advection_diffusion.jl:106-111)
     T wave_speed(const T* ui, const T* uo, const double* n) const {
         if (kind != SSE_EULER) {
             T s = 0.0;
@@ -193,7 +193,7 @@ struct Law {
         return av + cm;
     }
 
-    // conservative_to_entropy! (euler_navierstokes.jl:306-319)
+    // conservative_to_entropy! (euler_navierstokes.jl:100-113)
     void c2e(const T* u, T* w) const {
         T s = 0.0;
         for (int m = 0; m < d; ++m) s += u[m + 1] * u[m + 1];
@@ -205,7 +205,7 @@ struct Law {
         w[d + 1] = -u[0] * inv_p;
     }
 
-    // entropy_to_conservative! (euler_navierstokes.jl:321-337)
+    // entropy_to_conservative! (euler_navierstokes.jl:115-131)
     void e2c(const T* w_in, T* u) const {
         T w[5];
         for (int e = 0; e < N_c; ++e) w[e] = w_in[e] * gm1;
@@ -243,6 +243,25 @@ CSC to_csc(const double* A, int n_rows, int n_cols) {
     return c;
 }
 
+// The same nonzeros by rows: entry i of the result lists row i's nonzero columns,
+// ascending (for y = A x, which then sums the nonzero terms in the order the dense
+// loop would; the skipped products are exact zeros).
+CSC to_csr(const double* A, int n_rows, int n_cols) {
+    CSC c;
+    c.colptr.push_back(0);
+    for (int i = 0; i < n_rows; ++i) {
+        for (int j = 0; j < n_cols; ++j) {
+            double v = A[i + (size_t)n_rows * j];
+            if (v != 0.0) {
+                c.rowval.push_back(j);
+                c.nzval.push_back(v);
+            }
+        }
+        c.colptr.push_back((int)c.rowval.size());
+    }
+    return c;
+}
+
 template <typename T>
 struct Oracle {
     sse_desc D;
@@ -250,6 +269,10 @@ struct Oracle {
     int d, N_p, N_q, N_f, N_fac, N_c, N_e, npf;
     vector<CSC> S_csc;
     CSC C_csc;
+    // nonzeros of D_m and R by rows (for D x, R x) and by columns (for D' x, R' x): the
+    // reference applies these tensor-product operators without their zero entries
+    vector<CSC> D_rows, D_cols;
+    CSC R_rows, R_cols;
     vector<double> halfnJq;   // d x N_fac x N_q x N_e
     vector<double> n_f, BJf;  // d x N_f x N_e ; N_f x N_e
     vector<double> WJ, JinvW; // N_q x N_e
@@ -283,6 +306,15 @@ struct Oracle {
         if (D.S_sparse)
             for (int m = 0; m < d; ++m) S_csc.push_back(to_csc(D.S + (size_t)N_q * N_q * m, N_q, N_q));
         if (D.C_kind == SSE_C_SPARSE) C_csc = to_csc(D.C, N_q, N_f);
+        if (D.D)
+            for (int m = 0; m < d; ++m) {
+                D_rows.push_back(to_csr(D.D + (size_t)N_q * N_q * m, N_q, N_q));
+                D_cols.push_back(to_csc(D.D + (size_t)N_q * N_q * m, N_q, N_q));
+            }
+        if (D.R && !D.R_selection) {
+            R_rows = to_csr(D.R, N_f, N_q);
+            R_cols = to_csc(D.R, N_f, N_q);
+        }
         halfnJq.assign((size_t)d * N_fac * N_q * N_e, 0.0);
         if (D.nJq) {
             for (size_t i = 0; i < halfnJq.size(); ++i) halfnJq[i] = 0.5 * D.nJq[i];
@@ -343,6 +375,16 @@ struct Oracle {
                 y[j + (size_t)ldy * e] = s;
             }
     }
+    // y (n_out x N_c) = A x with A's nonzeros by rows (to_csr), or y = A' x with A's
+    // nonzeros by columns (to_csc): the dense loops above without their zero products
+    void matmul_sp(const CSC& A, int n_out, const T* x, int ldx, T* y, int ldy) const {
+        for (int e = 0; e < N_c; ++e)
+            for (int i = 0; i < n_out; ++i) {
+                T s = 0.0;
+                for (int q = A.colptr[i]; q < A.colptr[i + 1]; ++q) s += A.nzval[q] * x[A.rowval[q] + (size_t)ldx * e];
+                y[i + (size_t)ldy * e] = s;
+            }
+    }
     void applyV(const T* x, T* y) const {  // N_p -> N_q
         if (D.V) matmul(D.V, N_q, N_p, x, N_p, y, N_q);
         else for (int e = 0; e < N_c; ++e) for (int i = 0; i < N_q; ++i) y[i + N_q * e] = x[i + N_p * e];
@@ -363,7 +405,7 @@ struct Oracle {
             for (int e = 0; e < N_c; ++e)
                 for (int j = 0; j < N_f; ++j) y[j + (size_t)ldy * e] = x[(D.R_selection[j] - 1) + (size_t)N_q * e];
         } else {
-            matmul(D.R, N_f, N_q, x, N_q, y, ldy);
+            matmul_sp(R_rows, N_f, x, N_q, y, ldy);
         }
     }
     void applyRt(const T* x, T* y) const {  // (N_f x N_c) -> (N_q x N_c)
@@ -373,11 +415,11 @@ struct Oracle {
                 for (int j = 0; j < N_f; ++j) y[(D.R_selection[j] - 1) + (size_t)N_q * e] += x[j + (size_t)N_f * e];
             }
         } else {
-            matmul_t(D.R, N_f, N_q, x, N_f, y, N_q);
+            matmul_sp(R_cols, N_q, x, N_f, y, N_q);
         }
     }
 
-    // mass_matrix_solve! (mass_matrix.jl:398-417); rhs N_p x N_c, temp N_q x N_c
+    // mass_matrix_solve! (mass_matrix.jl:169-196); rhs N_p x N_c, temp N_q x N_c
     void mass_solve(int k, T* rhs, T* temp) const {
         if (D.mass_solver == SSE_MASS_DIAGONAL) {
             for (int e = 0; e < N_c; ++e)
@@ -555,8 +597,8 @@ struct Oracle {
 
     // ------------------------------------------------------------------ StandardForm
     // y = D_m x, y = D_m' x (N_q x N_c)
-    void applyD(int m, const T* x, T* y) const { matmul(D.D + (size_t)N_q * N_q * m, N_q, N_q, x, N_q, y, N_q); }
-    void applyDt(int m, const T* x, T* y) const { matmul_t(D.D + (size_t)N_q * N_q * m, N_q, N_q, x, N_q, y, N_q); }
+    void applyD(int m, const T* x, T* y) const { matmul_sp(D_rows[m], N_q, x, N_q, y, N_q); }
+    void applyDt(int m, const T* x, T* y) const { matmul_sp(D_cols[m], N_q, x, N_q, y, N_q); }
 
     // r += (volume operator of direction n) x, before V' and the mass solve:
     //   SkewSymmetricMapping  sum_m D_m' (1/2 W Lambda_mn x) - 1/2 W Lambda_mn (D_m x)

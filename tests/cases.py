@@ -10,7 +10,7 @@ from sse.workloads import (advection_2d_modal_tensor_tri,  # noqa: F401
 
 
 def l2_error(sd, u, exact, t=0.0):
-    """ErrorAnalysis with the volume quadrature (src/Analysis/error.jl:144-222)."""
+    """ErrorAnalysis with the volume quadrature (src/Analysis/error.jl:58-91)."""
     ref = sd.reference_approximation
     uq = np.einsum("qp,pck->qck", ref.Vdense(), u)
     ue = np.transpose(np.asarray(exact(sd.mesh.xyzq, t)), (0, 2, 1))

@@ -1,6 +1,6 @@
 """Error and conservation analyses on the GPU (sse_l2_error, sse_functional over the
 written time steps of a results directory), run the way the reference's
-test/test_driver.jl:1-111 runs them: solve with the save callback, then
+test/test_driver.jl:1-110 runs them: solve with the save callback, then
 ``analyze(ErrorAnalysis, u_final, exact, t)``, ``analyze(PrimaryConservationAnalysis,
 time_steps)`` and ``analyze(EnergyConservationAnalysis, time_steps)``, checked
 against the reference's golden values and tolerances (test/runtests.jl:12-61).
@@ -78,7 +78,7 @@ def test_l2_error_matches_the_quadrature_sum(normalize):
 
 
 def test_conservation_analyze_initial_and_final(tmp_path):
-    # conservation.jl:234-249: (initial, final, difference) between two written steps
+    # conservation.jl:221-234: (initial, final, difference) between two written steps
     from sse import io
     from sse.analysis import PrimaryConservationAnalysis, analyze
     from sse.time_integration import CarpenterKennedy2N54, solve

@@ -77,7 +77,8 @@ def test_residual_matches_oracle(name):
 
 
 def test_residual_host_memory_path_equals_device_path():
-    # equal up to the order of the facet-correction sums (LDS atomic adds)
+    # equal up to the order of K2's LDS atomic sums (the partner shares of the volume
+    # pairs and the facet-correction row and column sums)
     from sse import semi_discrete_residual
     ode = vortex_2d_config(M=4, p=4)[4]
     dev = _gpu_residual(ode)
@@ -87,8 +88,9 @@ def test_residual_host_memory_path_equals_device_path():
 
 
 def test_repeated_residuals_agree_to_rounding():
-    # the facet-correction sums of K2 are accumulated with LDS atomic adds, so their
-    # order may change between calls; everything else is fixed
+    # K2 accumulates the partner shares of the volume pairs and the facet-correction
+    # sums with LDS atomic adds, so their order may change between calls; everything
+    # else is fixed
     ode = density_wave_3d_config(M=2, p=4)[4]
     first = _gpu_residual(ode)
     for _ in range(3):

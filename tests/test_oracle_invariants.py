@@ -331,7 +331,7 @@ def test_standard_form_br1_dissipates_energy_and_conserves_mass():
 
 def test_standard_form_br1_converges_to_exact_gaussian():
     # advection-diffusion of a Gaussian on curved tets against the closed form of
-    # src/ConservationLaws/linear_advection_diffusion.jl:146-163 (pulse away from the
+    # src/ConservationLaws/linear_advection_diffusion.jl:147-165 (pulse away from the
     # periodic boundary over the short horizon)
     T = 0.05
     errs = []
