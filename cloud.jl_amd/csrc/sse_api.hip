@@ -14,6 +14,7 @@
 #include <cstring>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -90,6 +91,8 @@ struct sse_handle {
     double *vjp_rbar = nullptr, *vjp_fbar = nullptr, *vjp_lam = nullptr, *vjp_lamout = nullptr;  // VJP scratch
     double* vjp_fext = nullptr;  // fbar in the neighbour's exterior slot (tensor VJP kernels)
     bool vjp_tensor = false;
+    double* noalloc_ws = nullptr;  // this device's address of the file-scope workspace `scratch`
+    std::string noalloc_err;       // why that address is missing, if it is
     std::vector<void*> allocations;
     std::string err;
 };
@@ -97,6 +100,38 @@ struct sse_handle {
 namespace {
 
 thread_local std::string g_create_error;
+
+// Workspace of sse_functional_noalloc: the per-element partial sums and their totals.
+// It is a file-scope device array, so the entry point allocates nothing; every GPU the
+// library runs on holds its own copy (8 MiB) until the process ends. Host code never
+// passes the name `scratch` to a kernel (on the host it is only a stand-in symbol): it
+// uses the device address hipGetSymbolAddress returns, looked up once per device.
+static __device__ __align__(256) double scratch[SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_ws_mutex;
+double* g_ws_ptr[kMaxDevices] = {};
+hipError_t g_ws_status[kMaxDevices] = {};
+bool g_ws_done[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call per
+// device, with that device current). Returns the lookup's status.
+hipError_t workspace_address(double** out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if (!g_ws_done[dev]) {
+        void* p = nullptr;
+        g_ws_status[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_ws_ptr[dev] = g_ws_status[dev] == hipSuccess ? static_cast<double*>(p) : nullptr;
+        g_ws_done[dev] = true;
+    }
+    *out = g_ws_ptr[dev];
+    return g_ws_status[dev];
+}
 
 // Carpenter & Kennedy (1994), NASA TM-109112: five-stage fourth-order 2N-storage
 // Runge-Kutta, the coefficients of OrdinaryDiffEq's CarpenterKennedy2N54.
@@ -979,6 +1014,9 @@ int create_handle(const sse_desc* D, int members, sse_handle** out) {
         return cleanup_fail(SSE_ERR_HIP);
     }
     h->stream = h->own_stream;
+    // set-up of sse_functional_noalloc: this device's address of its workspace
+    if (hipError_t e = workspace_address(&h->noalloc_ws); e != hipSuccess)
+        h->noalloc_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
     *out = h;
     return 0;
 }
@@ -1077,6 +1115,29 @@ int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t w
     const int nout = which == SSE_ENTROPY ? 1 : h->P.N_c;
     SSE_HIP_TRY(h, hipMemcpyAsync(out, h->fout, nout * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int sse_functional_noalloc(sse_handle* h, const double* u, const double* dudt, int32_t which, double* out) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u or out is NULL");
+    if (which < SSE_MASS || which > SSE_ENTROPY) return fail(h, SSE_ERR_INVALID_ARGUMENT, "unknown functional");
+    if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
+    const size_t ncomp = h->P.N_c, need = (size_t)h->P.N_e * ncomp + ncomp;
+    if (need > (size_t)SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES)
+        return fail(h, SSE_ERR_INVALID_ARGUMENT,
+                    "sse_functional_noalloc: N_e * N_c + N_c = " + std::to_string(need) +
+                        " exceeds its workspace of " + std::to_string(SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES) +
+                        " doubles; use sse_functional");
+    if (!h->noalloc_ws) return fail(h, SSE_ERR_HIP, "sse_functional_noalloc: " + h->noalloc_err);
+    double* partial = h->noalloc_ws;                     // N_e x N_c, every entry written by the first kernel
+    double* sums = h->noalloc_ws + (size_t)h->P.N_e * ncomp;  // N_c totals, written by the second
+    const int kind = which + (dudt ? 3 : 0);
+    // energy: the mass-matrix factors are formed in LDS on every call (no per-handle cache)
+    hipError_t e = sse::launch_functionals(h->P, u, dudt, kind, partial, sums, h->stream, nullptr, 0);
+    if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    const int nout = which == SSE_ENTROPY ? 1 : h->P.N_c;
+    SSE_HIP_TRY(h, hipMemcpyAsync(out, sums, nout * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     return 0;
 }
 

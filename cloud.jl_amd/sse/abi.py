@@ -54,7 +54,11 @@ EXPORTED_SYMBOLS = (
     "sse_batch_create", "sse_batch_rk_steps", "sse_batch_describe", "sse_batch_last_error",
     "sse_batch_destroy", "sse_functional", "sse_residual_vjp", "sse_rk_adjoint", "sse_batch_rk_tangent",
     "sse_geometric_factors", "sse_apply_reference_mapping", "sse_project_l2", "sse_l2_error",
+    "sse_functional_noalloc",
 )
+
+# Workspace of sse_functional_noalloc, in doubles (SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES).
+SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES = 1 << 20
 
 SSE_METRICS_EXACT, SSE_METRICS_CURL = 1, 2
 

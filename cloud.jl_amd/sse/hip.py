@@ -68,6 +68,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_batch_destroy.restype = None
     lib.sse_functional.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, dp]
     lib.sse_functional.restype = C.c_int
+    lib.sse_functional_noalloc.argtypes = [vp, vp, vp, C.c_int32, vp]
+    lib.sse_functional_noalloc.restype = C.c_int
     lib.sse_residual_vjp.argtypes = [vp, vp, vp, vp, dp, C.c_int32]
     lib.sse_residual_vjp.restype = C.c_int
     lib.sse_rk_adjoint.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, C.c_int32, dp, vp, dp, C.c_int32]

@@ -220,6 +220,23 @@ int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t w
 int sse_l2_error(sse_handle* h, const double* u, const double* u_exact, int32_t normalize, int32_t where,
                  double* out);
 
+/* Largest N_e * N_c + N_c that sse_functional_noalloc accepts (doubles of its workspace). */
+#define SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES (1 << 20)
+
+/* sse_functional for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. Same functionals and the same kernels as sse_functional, but u, dudt
+ * (NULL for the functional itself) and out are DEVICE pointers, out receives N_c values
+ * (SSE_MASS, SSE_ENERGY) or 1 (SSE_ENTROPY), and the call is asynchronous on the
+ * handle's stream: it allocates nothing, copies nothing to the host and does not wait.
+ * The per-element partial sums live in one fixed workspace per GPU inside the library
+ * (SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES doubles, held until the process ends); a handle
+ * with N_e * N_c + N_c above that is rejected with SSE_ERR_INVALID_ARGUMENT. Every
+ * call on one GPU shares that workspace, so calls that may overlap on one GPU (other
+ * handles, other streams) race: order them on one stream. The energy functionals form
+ * each element's mass-matrix factor in LDS on every call (sse_functional keeps them). */
+int sse_functional_noalloc(sse_handle* h, const double* u, const double* dudt, int32_t which, double* out);
+
 /* Uses the given hipStream_t (0 = the handle's own stream) for later calls. */
 int sse_set_stream(sse_handle* h, void* hip_stream);
 
