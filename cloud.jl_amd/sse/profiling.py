@@ -11,9 +11,13 @@ Two summaries, both plain text or JSON so they can be kept beside the code:
   size (``bench.py --copy-probe-mib``), so that the figure is in real bytes
   whatever fraction of the memory channels the counter samples.
 
+* ``counters``: every counter of one or more ``rocprofv3 --pmc`` runs, mean per
+  dispatch, for each kernel slot.
+
 Usage::
 
     python -m sse.profiling kernel-stats <trace_dir>
+    python -m sse.profiling counters <pmc_dir> [<pmc_dir> ...]
     python -m sse.profiling traffic <fetch_dir> <write_dir> --copy-mib 256 \
         --workload NAME [--json profiles/hbm_traffic.json]
 """
@@ -101,6 +105,34 @@ def counter_means(directory: str, counter: str):
     return {k: statistics.fmean(v) for k, v in by_kernel.items()}
 
 
+def all_counters(directories):
+    """{slot: {counter: mean per dispatch}} over the counter-collection runs in `directories`."""
+    per = defaultdict(lambda: defaultdict(float))  # (dir, dispatch) -> counter -> value
+    names = {}
+    for d in directories:
+        for r in _csv_rows(d, "counter_collection.csv"):
+            key = (d, r["Dispatch_Id"])
+            per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+            names[key] = r["Kernel_Name"]
+    acc = defaultdict(lambda: defaultdict(list))
+    for key, cs in per.items():
+        slot = slot_of(names[key])
+        if slot is None:
+            continue
+        for c, v in cs.items():
+            acc[slot][c].append(v)
+    return {s: {c: statistics.fmean(v) for c, v in cs.items()} for s, cs in acc.items()}
+
+
+def format_counters(data) -> str:
+    slots = [s for s in ("nodal_values", "time_derivative") if s in data]
+    names = sorted({c for s in slots for c in data[s]})
+    lines = [f"{'counter (mean per dispatch)':34s}" + "".join(f"{s:>18s}" for s in slots)]
+    for c in names:
+        lines.append(f"{c:34s}" + "".join(f"{data[s].get(c, float('nan')):18.6g}" for s in slots))
+    return "\n".join(lines)
+
+
 def traffic(fetch_dir: str, write_dir: str, copy_mib: int):
     """Calibrated HBM bytes per dispatch for each kernel slot."""
     fetch = counter_means(fetch_dir, "FETCH_SIZE")
@@ -139,6 +171,8 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("kernel-stats")
     a.add_argument("dir")
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -148,6 +182,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "kernel-stats":
         print(format_kernel_stats(kernel_stats(args.dir)))
+        return
+    if args.cmd == "counters":
+        print(format_counters(all_counters(args.dirs)))
         return
     t = traffic(args.fetch_dir, args.write_dir, args.copy_mib)
     print(json.dumps(t, indent=1))
