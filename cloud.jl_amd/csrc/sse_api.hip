@@ -32,7 +32,7 @@ hipError_t launch_standard(const DevParams& p, double* dudt, const RKStage& rk, 
 size_t lds_standard(const DevParams& p);
 hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
                               double* out, hipStream_t s, double* chol = nullptr, int chol_ready = 0);
-size_t lds_functionals(const DevParams& p);
+size_t lds_functionals(const DevParams& p, int which);
 hipError_t launch_vjp(const DevParams& p, const double* u, const double* ybar, double* ubar, double* rbar,
                       double* fbar, double* lam_partial, double* lam_out, double lam_scale, int accumulate,
                       hipStream_t s);
@@ -81,6 +81,15 @@ struct sse_handle {
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     double* rk_tmp = nullptr;
+    double* erk_k = nullptr;     // sse_erk_steps: erk_stages stage derivatives, then the stage state
+    // sse_rk_steps: one CK2N54 step captured as a HIP graph, for the state, step and
+    // stream it was captured with
+    hipGraph_t rk_graph = nullptr;
+    hipGraphExec_t rk_exec = nullptr;
+    const double* rk_graph_u = nullptr;
+    double rk_graph_dt = 0.0;
+    hipStream_t rk_graph_stream = nullptr;
+    int erk_stages = 0;
     double* stage_u = nullptr;
     double* stage_dudt = nullptr;
     double* fpartial = nullptr;  // functionals: per-element partial sums and the result
@@ -1058,7 +1067,7 @@ int sse_l2_error(sse_handle* h, const double* u, const double* u_exact, int32_t 
     if (!u || !u_exact || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u, u_exact or out is NULL");
     if (where != SSE_DEVICE && where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
     if (h->members != 1) return fail(h, SSE_ERR_UNSUPPORTED, "the error of an ensemble handle");
-    if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
+    if (sse::lds_functionals(h->P, 6) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
     const size_t n = state_size(h), nq = (size_t)h->P.N_q * h->P.N_c * h->P.N_e;
     int rc;
     if ((rc = functional_buffers(h))) return rc;
@@ -1087,7 +1096,7 @@ int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t w
     if (!u || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u or out is NULL");
     if (which < SSE_MASS || which > SSE_ENTROPY) return fail(h, SSE_ERR_INVALID_ARGUMENT, "unknown functional");
     if (where != SSE_DEVICE && where != SSE_HOST) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad `where`");
-    if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
+    if (sse::lds_functionals(h->P, which + (dudt ? 3 : 0)) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
     const size_t n = state_size(h);
     int rc;
     if ((rc = functional_buffers(h))) return rc;
@@ -1122,7 +1131,7 @@ int sse_functional_noalloc(sse_handle* h, const double* u, const double* dudt, i
     if (!h) return SSE_ERR_INVALID_ARGUMENT;
     if (!u || !out) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u or out is NULL");
     if (which < SSE_MASS || which > SSE_ENTROPY) return fail(h, SSE_ERR_INVALID_ARGUMENT, "unknown functional");
-    if (sse::lds_functionals(h->P) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
+    if (sse::lds_functionals(h->P, which + (dudt ? 3 : 0)) > 64 * 1024) return fail(h, SSE_ERR_UNSUPPORTED, "element too large for the functionals");
     const size_t ncomp = h->P.N_c, need = (size_t)h->P.N_e * ncomp + ncomp;
     if (need > (size_t)SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES)
         return fail(h, SSE_ERR_INVALID_ARGUMENT,
@@ -1454,11 +1463,140 @@ int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps,
     } else if (where != SSE_DEVICE) {
         return fail(h, SSE_ERR_INVALID_ARGUMENT, "where must be SSE_DEVICE or SSE_HOST");
     }
-    for (int step = 0; step < nsteps; ++step) {
+    auto one_step = [&]() -> int {
         for (int s = 0; s < 5; ++s) {
             RKStage rk{1, CK_A[s], CK_B[s], dt, h->rk_tmp, ud};
-            if ((rc = launch(h, ud, nullptr, rk, 3))) return rc;
+            int r = launch(h, ud, nullptr, rk, 3);
+            if (r) return r;
         }
+        return 0;
+    };
+    // The first step runs eagerly (it also sets kernel attributes); the rest replay one
+    // step captured as a HIP graph (ten kernel launches), recaptured when the state,
+    // the step or the stream differ from the captured ones. SSE_RK_GRAPH=0 launches
+    // every step eagerly.
+    int step = 0;
+    if (nsteps > 0) {
+        if ((rc = one_step())) return rc;
+        step = 1;
+    }
+    const char* genv = getenv("SSE_RK_GRAPH");
+    const bool graph = !(genv && genv[0] == '0') && h->members == 1;
+    if (graph && step < nsteps) {
+        if (!h->rk_exec || h->rk_graph_u != ud || h->rk_graph_dt != dt || h->rk_graph_stream != h->stream) {
+            if (h->rk_exec) (void)hipGraphExecDestroy(h->rk_exec);
+            if (h->rk_graph) (void)hipGraphDestroy(h->rk_graph);
+            h->rk_exec = nullptr;
+            h->rk_graph = nullptr;
+            SSE_HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            const int r = one_step();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(h->stream, &g);
+            if (r) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            SSE_HIP_TRY(h, e);
+            h->rk_graph = g;
+            SSE_HIP_TRY(h, hipGraphInstantiate(&h->rk_exec, g, nullptr, nullptr, 0));
+            h->rk_graph_u = ud;
+            h->rk_graph_dt = dt;
+            h->rk_graph_stream = h->stream;
+        }
+        for (; step < nsteps; ++step) SSE_HIP_TRY(h, hipGraphLaunch(h->rk_exec, h->stream));
+    }
+    for (; step < nsteps; ++step)
+        if ((rc = one_step())) return rc;
+    if (where == SSE_HOST) {
+        SSE_HIP_TRY(h, hipMemcpyAsync(u, h->stage_u, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return check_status(h);
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// out = base + sum_q c[q] K[idx[q]] over n entries (a Runge-Kutta stage combination;
+// out may alias base). The nonzero coefficients only, at most SSE_ERK_MAX_STAGES.
+struct ErkTerms {
+    int count;
+    int idx[SSE_ERK_MAX_STAGES];
+    double c[SSE_ERK_MAX_STAGES];
+};
+
+__global__ void __launch_bounds__(256) k_erk_combine(size_t n, const double* base, const double* __restrict__ K,
+                                                     ErkTerms terms, double* out) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int q = 0; q < terms.count; ++q) acc += terms.c[q] * K[(size_t)terms.idx[q] * n + t];
+        out[t] = base[t] + acc;
+    }
+}
+
+int erk_combine(sse_handle* h, size_t n, const double* base, const double* K, const ErkTerms& terms, double* out) {
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_erk_combine, dim3(blocks), dim3(256), 0, h->stream, n, base, K, terms, out);
+    SSE_HIP_TRY(h, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sse_erk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t stages, const double* A,
+                  const double* b, int32_t where) {
+    (void)t0;  // the residual is autonomous
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!u || !A || !b || nsteps < 0) return fail(h, SSE_ERR_INVALID_ARGUMENT, "u, A or b is NULL or nsteps < 0");
+    if (stages < 1 || stages > SSE_ERK_MAX_STAGES)
+        return fail(h, SSE_ERR_INVALID_ARGUMENT, "stages must be 1.." + std::to_string(SSE_ERK_MAX_STAGES));
+    if (where != SSE_DEVICE && where != SSE_HOST)
+        return fail(h, SSE_ERR_INVALID_ARGUMENT, "where must be SSE_DEVICE or SSE_HOST");
+    if (h->members != 1) return fail(h, SSE_ERR_UNSUPPORTED, "an ensemble handle advances with sse_batch_rk_steps");
+    for (int i = 0; i < stages; ++i)
+        for (int j = i; j < stages; ++j)
+            if (A[i * stages + j] != 0.0)
+                return fail(h, SSE_ERR_INVALID_ARGUMENT, "A must be strictly lower triangular (an explicit method)");
+    const size_t n = state_size(h);
+    int rc;
+    if (h->erk_stages < stages) {  // stage derivatives and the stage state (kept for later calls)
+        if ((rc = alloc(h, (size_t)(stages + 1) * n, &h->erk_k))) return rc;
+        h->erk_stages = stages;
+    }
+    double* K = h->erk_k;
+    double* y = h->erk_k + (size_t)h->erk_stages * n;
+    double* ud = u;
+    if (where == SSE_HOST) {
+        if (!h->stage_u && ((rc = alloc(h, n, &h->stage_u)) || (rc = alloc(h, n, &h->stage_dudt)))) return rc;
+        SSE_HIP_TRY(h, hipMemcpyAsync(h->stage_u, u, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ud = h->stage_u;
+    }
+    ErkTerms fin{};
+    for (int i = 0; i < stages; ++i)
+        if (b[i] != 0.0) {
+            fin.idx[fin.count] = i;
+            fin.c[fin.count++] = dt * b[i];
+        }
+    for (int step = 0; step < nsteps; ++step) {
+        for (int i = 0; i < stages; ++i) {
+            ErkTerms st{};
+            for (int j = 0; j < i; ++j)
+                if (A[i * stages + j] != 0.0) {
+                    st.idx[st.count] = j;
+                    st.c[st.count++] = dt * A[i * stages + j];
+                }
+            const double* yi = ud;
+            if (st.count) {
+                if ((rc = erk_combine(h, n, ud, K, st, y))) return rc;
+                yi = y;
+            }
+            if ((rc = launch(h, yi, K + (size_t)i * n, RKStage{}, 3))) return rc;
+        }
+        if ((rc = erk_combine(h, n, ud, K, fin, ud))) return rc;
     }
     if (where == SSE_HOST) {
         SSE_HIP_TRY(h, hipMemcpyAsync(u, h->stage_u, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1497,6 +1635,8 @@ int sse_profile_residual(sse_handle* h, const double* u, double* dudt, int32_t n
 void sse_destroy(sse_handle* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->rk_exec) (void)hipGraphExecDestroy(h->rk_exec);
+    if (h->rk_graph) (void)hipGraphDestroy(h->rk_graph);
     for (void* p : h->allocations) (void)hipFree(p);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;

@@ -170,6 +170,15 @@ __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* _
         }
         return;
     }
+    if (!p.has_V) {  // nodal: M_k = diag(W J_k), no matrix to factor
+        for (int e = 0; e < NC; ++e) {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < Np; i += FT) s += p.W[i] * Jk[i] * s_u[i + Np * e] * (rate ? s_v : s_u)[i + Np * e];
+            const double tot = block_sum(s, red);
+            if (threadIdx.x == 0) out[e] = rate ? tot : 0.5 * tot;
+        }
+        return;
+    }
     // energy: y = M_k x by solving (M_k^{-1}) y = x, x = u (energy) or dudt (rate)
     double* ck = chol ? chol + (size_t)k * Np * Np : nullptr;
     if (ck && chol_ready) {  // the factor of M_k^{-1} from an earlier call
@@ -243,15 +252,18 @@ __global__ void __launch_bounds__(FT) k_sum_partials(const double* __restrict__ 
     }
 }
 
-size_t lds_functionals(const DevParams& p) {
-    return sizeof(double) * ((size_t)FT + 4 * (size_t)p.N_p * p.N_c + 2 * (size_t)p.N_q * p.N_c +
-                             (size_t)p.N_p * p.N_p);
+// LDS of one functional kind: the mass-matrix factor and the solve buffer only for
+// the energy kinds of a modal (weight-adjusted) solver
+size_t lds_functionals(const DevParams& p, int which) {
+    const bool matrix = (which == 1 || which == 4) && p.has_V;
+    return sizeof(double) * ((size_t)FT + 2 * (size_t)p.N_p * p.N_c + 2 * (size_t)p.N_q * p.N_c +
+                             (matrix ? (size_t)p.N_p * p.N_p + (size_t)p.N_p * p.N_c : 0));
 }
 
 template <int D, int LAW>
 hipError_t launch_functionals_t(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
                                 double* out, hipStream_t s, double* chol, int chol_ready) {
-    hipLaunchKernelGGL((k_functionals<D, LAW>), dim3(p.N_e), dim3(FT), lds_functionals(p), s, p, u, dudt, which,
+    hipLaunchKernelGGL((k_functionals<D, LAW>), dim3(p.N_e), dim3(FT), lds_functionals(p, which), s, p, u, dudt, which,
                        partial, chol, chol_ready);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -226,14 +226,28 @@ __device__ __forceinline__ void logmean_frac(double x, double y, double lx, doub
     den = ser ? poly : (ly - lx);
 }
 
+// 1/x for a nonzero normal x: the hardware reciprocal estimate (v_rcp_f64) refined by
+// two Newton-Raphson steps, e = 1 - x r, r <- r + r e, each of which squares the
+// relative error (the iteration of the IEEE division sequence, without its scaling
+// and rounding fix-up, which only matter for denormal, infinite or near-overflow
+// operands). Five instructions instead of the eleven of a correctly rounded divide;
+// within two ulps of 1/x (checked on the GPU against the divide).
+__device__ __forceinline__ double rcp_refined(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 template <int D>
 __device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, const double* hR, const double* a,
                                        double* c) {
-    // rho_avg = n1/d1 (logmean of rho), ilm = d2/n2 (inverse logmean of rho/p): one divide
+    // rho_avg = n1/d1 (logmean of rho), ilm = d2/n2 (inverse logmean of rho/p): one reciprocal
     double n1, d1, n2, d2;
     logmean_frac(hL[0], hR[0], hL[D + 2], hR[D + 2], n1, d1);
     logmean_frac(hL[D + 3], hR[D + 3], hL[D + 4], hR[D + 4], n2, d2);
-    const double rcp = 1.0 / (d1 * n2);
+    const double rcp = rcp_refined(d1 * n2);
     const double rho_avg = n1 * n2 * rcp;
     const double ilm = d2 * d1 * rcp;
     double vv = 0.0, avL = 0.0, avR = 0.0;

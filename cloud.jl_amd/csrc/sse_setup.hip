@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/sse_hip.h"
+#include "sse_physics.h"
 
 namespace sse {
 int set_global_error(int code, const std::string& msg);
@@ -554,5 +555,69 @@ extern "C" int sse_project_l2(int32_t N_q, int32_t N_p, int32_t N_c, int32_t N_e
                                                               std::to_string(st - 1) + " (J <= 0 at a quadrature node?)");
     if (where == SSE_HOST && (e = hipMemcpy(u, uo, n_u * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(e, "sse_project_l2: copy out");
+    return 0;
+}
+
+// ----------------------------------------------------------------------------- flux evaluator
+namespace {
+
+template <int D>
+__global__ void __launch_bounds__(256) k_ec_flux_batch(sse::LawParams law, int n, const double* __restrict__ uL,
+                                                       const double* __restrict__ uR,
+                                                       const double* __restrict__ a, double* __restrict__ c) {
+    constexpr int NC = D + 2, NH = D + 5;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double ul[NC], ur[NC], hl[NH], hr[NH], av[D], cv[NC];
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+        ul[e] = uL[(size_t)t * NC + e];
+        ur[e] = uR[(size_t)t * NC + e];
+    }
+#pragma unroll
+    for (int m = 0; m < D; ++m) av[m] = a[(size_t)t * D + m];
+    sse::hoist<D>(law, ul, hl);
+    sse::hoist<D>(law, ur, hr);
+    sse::ec_dot<D>(law, hl, hr, av, cv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) c[(size_t)t * NC + e] = cv[e];
+}
+
+}  // namespace
+
+extern "C" int sse_ec_flux_batch(int32_t d, double gamma, int32_t n, const double* uL, const double* uR,
+                                 const double* a, double* c, int32_t where) {
+    using sse::set_global_error;
+    if (d < 1 || d > 3 || !(gamma > 1.0) || n < 0 || !uL || !uR || !a || !c)
+        return set_global_error(SSE_ERR_INVALID_ARGUMENT, "sse_ec_flux_batch: bad arguments");
+    if (where != SSE_DEVICE && where != SSE_HOST)
+        return set_global_error(SSE_ERR_INVALID_ARGUMENT, "sse_ec_flux_batch: where must be SSE_DEVICE or SSE_HOST");
+    if (n == 0) return 0;
+    const size_t nc = (size_t)n * (d + 2), na = (size_t)n * d;
+    DevBuf buf;
+    int rc;
+    const double *l = uL, *r = uR, *av = a;
+    double* out = c;
+    if (where == SSE_HOST) {
+        if ((rc = buf.put(uL, nc, &l)) || (rc = buf.put(uR, nc, &r)) || (rc = buf.put(a, na, &av)) ||
+            (rc = buf.alloc(nc, &out)))
+            return rc;
+    }
+    sse::LawParams law{};
+    law.gamma = gamma;
+    law.gm1 = gamma - 1.0;
+    law.inv_gm1 = 1.0 / (gamma - 1.0);
+    law.log_gm1 = std::log(gamma - 1.0);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    switch (d) {
+        case 1: hipLaunchKernelGGL(k_ec_flux_batch<1>, grid, dim3(256), 0, 0, law, n, l, r, av, out); break;
+        case 2: hipLaunchKernelGGL(k_ec_flux_batch<2>, grid, dim3(256), 0, 0, law, n, l, r, av, out); break;
+        default: hipLaunchKernelGGL(k_ec_flux_batch<3>, grid, dim3(256), 0, 0, law, n, l, r, av, out); break;
+    }
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "k_ec_flux_batch launch");
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(e, "k_ec_flux_batch");
+    if (where == SSE_HOST && (e = hipMemcpy(c, out, nc * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "sse_ec_flux_batch: copy out");
     return 0;
 }

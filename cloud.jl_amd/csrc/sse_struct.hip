@@ -53,7 +53,8 @@ struct StructLayout {
     static constexpr int off_nJ = off_hf + NH * Sh::NF;        // D x NF
     static constexpr int off_st = off_nJ + D * Sh::NF;         // row and facet accumulators of C
     static constexpr int ST = NC * (Sh::NQ + Sh::NF) > NC * MAXITEM ? NC * (Sh::NQ + Sh::NF) : NC * MAXITEM;
-    static constexpr int size = off_st + ST;
+    static constexpr int off_fn = off_st + ST;                 // D x NFAC reference face normals
+    static constexpr int size = off_fn + D * Sh::NFAC;
     static constexpr int off_ff = off_h;                       // NC x NF (after the pair phase)
     static constexpr int off_r = off_hf;                       // NC x NQ
     static constexpr int off_rb = off_L;                       // NC x NQ partial rows of split threads
@@ -64,7 +65,7 @@ struct StructLayout {
     static_assert(NC * Sh::NQ <= NH * Sh::NF + D * Sh::NF + ST, "r_q alias");
     static_assert(NC * Sh::NP <= D * D * Sh::NQ, "y alias");
     static_assert(NC * Sh::NQ <= D * D * Sh::NQ, "split partial-row alias");
-    static_assert(off_W + NC * Sh::NW <= size, "sum-factorisation temporaries alias");
+    static_assert(off_W + NC * Sh::NW <= off_fn, "sum-factorisation temporaries alias");
 };
 
 
@@ -137,7 +138,9 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         // ---- phase A: hoist, interface flux; clear the facet-correction accumulators
         double* srow = sst;            // NC x NQ: C-row sums of the volume nodes
         double* sfac = sst + NC * NQ;  // NC x NF: C-column sums of the facet nodes
+        double* sfn = sm + L::off_fn;
         for (int t = tt; t < NC * (NQ + NF); t += TS) sst[t] = 0.0;
+        if (tt < D * Sh::NFAC) sfn[tt] = p.fn[tt];
         double hi[NH], Li[D * D], r[NC], ff[NC], Ji = 1.0;
 #pragma unroll
         for (int e = 0; e < NC; ++e) r[e] = ff[e] = 0.0;
@@ -185,42 +188,66 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             for (int mn = 0; mn < D * D; ++mn) Li[mn] = sL[mn * NQ + tl];
         }
         if (tl < NQ) {
-            double hj[NH], av[D], c[NC];
             // tensor indices of node tl (last index fastest) and the family strides
             const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
             const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
-            int f = q0 / PER_F, dlt = q0 - f * PER_F + 1;
-#pragma unroll 1
-            for (int q = q0; q < q1; ++q) {
+            // pair q of this node: family f, partner index b on the line, partner node j;
+            // `act` is false for the offset N1/2 of an even line taken by the other node
+            auto pair_of = [&](int q, int& f, int& b, int& j, bool& act) {
+                f = q / PER_F;
+                const int dlt = q - f * PER_F + 1;
                 const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
                 const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
-                if (!(EVEN && 2 * dlt == N1 && 2 * af >= N1)) {
-                    const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
-                    const int j = tl + (b - af) * stride;
+                act = !(EVEN && 2 * dlt == N1 && 2 * af >= N1);
+                b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+                j = tl + (b - af) * stride;
+            };
+            // S_m[i, j] of pair q (the table holds the zeros of m < f too); requested one
+            // pair ahead and unconditionally, so that the loads overlap the flux
+            // evaluation of the pair before
+            auto load_s = [&](int q, double* sv) {
+                int f, b, j;
+                bool act;
+                pair_of(q, f, b, j, act);
+                const double* sg = st.SG + ((f * N1 + b) * D) * NQ + tl;
 #pragma unroll
-                    for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
-                    // a_n = sum_{m >= f} S_m[i,j] (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
+                for (int m = 0; m < D; ++m) sv[m] = sg[m * NQ];
+            };
+            auto eval = [&](int q, const double* sv) {
+                int f, b, j;
+                bool act;
+                pair_of(q, f, b, j, act);
+                if (!act) return;
+                double hj[NH], av[D], c[NC];
 #pragma unroll
-                    for (int n = 0; n < D; ++n) av[n] = 0.0;
+                for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+                // a_n = sum_{m >= f} S_m[i,j] (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
 #pragma unroll
-                    for (int m = 0; m < D; ++m) {  // static indices into Li (m < f: S_m = 0)
-                        if (m < f) continue;
-                        const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+                for (int n = 0; n < D; ++n) av[n] = 0.0;
 #pragma unroll
-                        for (int n = 0; n < D; ++n) av[n] += s * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
-                    }
-                    ec_dot<D>(p.law, hi, hj, av, c);
-                    // S antisymmetric, F symmetric: node j's row receives +c, through
-                    // the row accumulator that phase D subtracts
+                for (int m = 0; m < D; ++m) {  // static indices into Li
+                    if (m < f) continue;
 #pragma unroll
-                    for (int e = 0; e < NC; ++e) {
-                        r[e] -= c[e];
-                        atomicAdd(&srow[e * NQ + j], -c[e]);
-                    }
+                    for (int n = 0; n < D; ++n) av[n] += sv[m] * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
                 }
-                if (++dlt == PER_F + 1) {  // next family
-                    dlt = 1;
-                    ++f;
+                ec_dot<D>(p.law, hi, hj, av, c);
+                // S antisymmetric, F symmetric: node j's row receives +c, through
+                // the row accumulator that phase D subtracts
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    r[e] -= c[e];
+                    atomicAdd(&srow[e * NQ + j], -c[e]);
+                }
+            };
+            double sa[D], sb[D];
+            if (q0 < q1) load_s(q0, sa);
+#pragma unroll 1
+            for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
+                if (q + 1 < q1) load_s(q + 1, sb);
+                eval(q, sa);
+                if (q + 1 < q1) {
+                    if (q + 2 < q1) load_s(q + 2, sa);
+                    eval(q + 1, sb);
                 }
             }
         }
@@ -228,27 +255,39 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
 #pragma unroll 1
         for (int it = 0; it < MAXITEM / TS; ++it) {
             const int item = it * TS + tt;
-            const int j = item < st.n_items ? st.ci_j[item] : -1;
-            if (j < 0) continue;
-            double hj[NH], hv[NH], av[D], c[NC], part[NC];
+            if (item >= st.n_items) continue;  // the chunks are packed: items below n_items are real
+            // the chunk (facet node, N1 volume nodes and C entries) requested at once
+            const int j = st.ci_j[item];
+            int ci[N1];
+            double cv[N1];
+#pragma unroll
+            for (int w = 0; w < N1; ++w) {
+                ci[w] = st.ci_i[w * MAXITEM + item];
+                cv[w] = st.ci_v[w * MAXITEM + item];
+            }
+            const int fc = j / NPF;
+            double hj[NH], fnl[D], nj[D], part[NC];
+#pragma unroll
+            for (int l = 0; l < D; ++l) fnl[l] = sfn[l + D * fc];
 #pragma unroll
             for (int e = 0; e < NC; ++e) part[e] = 0.0;
 #pragma unroll
             for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
-            const int fc = j / NPF;
-#pragma unroll 1
-            for (int w = 0; w < st.ch; ++w) {
-                const int i = st.ci_i[w * MAXITEM + item];
+#pragma unroll
+            for (int n = 0; n < D; ++n) nj[n] = snJ[n * NF + j];
+#pragma unroll
+            for (int w = 0; w < N1; ++w) {
+                const int i = ci[w];
                 if (i < 0) continue;
-                const double cv = st.ci_v[w * MAXITEM + item];
+                double hv[NH], av[D], c[NC];
 #pragma unroll
                 for (int v = 0; v < NH; ++v) hv[v] = sh[v * NQ + i];
 #pragma unroll
                 for (int n = 0; n < D; ++n) {
-                    double s = snJ[n * NF + j];
+                    double s = nj[n];
 #pragma unroll
-                    for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * p.fn[l + D * fc];
-                    av[n] = 0.5 * cv * s;
+                    for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * fnl[l];
+                    av[n] = 0.5 * cv[w] * s;
                 }
                 ec_dot<D>(p.law, hv, hj, av, c);
 #pragma unroll

@@ -53,7 +53,8 @@ struct VjpLayout {
     static constexpr int off_ub = off_st + NC * (Sh::NQ + Sh::NF);  // NC x NQ split partial rows, then w_q' cotangent
     static constexpr int off_wf = off_ub + NC * Sh::NQ;      // NC x NF w_f cotangent
     static constexpr int off_red = off_wf + NC * Sh::NF;     // TS reduction buffer
-    static constexpr int size = off_red + T::TS;
+    static constexpr int off_fn = off_red + T::TS;           // D x NFAC reference face normals
+    static constexpr int size = off_fn + D * Sh::NFAC;
 };
 
 // LDS of the reverse projection (phase V3)
@@ -177,9 +178,11 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     double* sub = sm + L::off_ub;
     double* swf = sm + L::off_wf;
     double* red = sm + L::off_red;
+    double* sfn = sm + L::off_fn;
 
     // ---- phase A: loads, hoisting, interface flux adjoints; clear the C-pair sums
     for (int t = tt; t < NC * (NQ + NF); t += TS) srow[t] = 0.0;
+    if (tt < D * Sh::NFAC) sfn[tt] = p.fn[tt];
     double hi[NH], ubq[NC];
     double lam = 0.0;
 #pragma unroll
@@ -257,42 +260,63 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
     }
     if (tl < NQ) {  // the node's metric and rbar are read from LDS (fewer live registers)
-        double hj[NH], av[D], gb[NC];
         const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
         const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
-        int f = q0 / PER_F, dlt = q0 - f * PER_F + 1;
-#pragma unroll 1
-        for (int q = q0; q < q1; ++q) {
+        auto pair_of = [&](int q, int& f, int& b, int& j, bool& act) {
+            f = q / PER_F;
+            const int dlt = q - f * PER_F + 1;
             const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
             const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
-            if (!(EVEN && 2 * dlt == N1 && 2 * af >= N1)) {
-                const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
-                const int j = tl + (b - af) * stride;
-                double ubj[NC];
+            act = !(EVEN && 2 * dlt == N1 && 2 * af >= N1);
+            b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+            j = tl + (b - af) * stride;
+        };
+        // S_m[i, j] of pair q (zeros for m < f in the table), requested one pair ahead
+        auto load_s = [&](int q, double* sv) {
+            int f, b, j;
+            bool act;
+            pair_of(q, f, b, j, act);
+            const double* sg = st.SG + ((f * N1 + b) * D) * NQ + tl;
 #pragma unroll
-                for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+            for (int m = 0; m < D; ++m) sv[m] = sg[m * NQ];
+        };
+        auto eval = [&](int q, const double* sv) {
+            int f, b, j;
+            bool act;
+            pair_of(q, f, b, j, act);
+            if (!act) return;
+            double hj[NH], av[D], gb[NC], ubj[NC];
 #pragma unroll
-                for (int n = 0; n < D; ++n) av[n] = 0.0;
-                for (int m = f; m < D; ++m) {
-                    const double s = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+            for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
 #pragma unroll
-                    for (int n = 0; n < D; ++n) av[n] += s * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
-                }
+            for (int n = 0; n < D; ++n) av[n] = 0.0;
 #pragma unroll
-                for (int e = 0; e < NC; ++e) {
-                    gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
-                    ubj[e] = 0.0;
-                }
-                // the pair's term enters r_q[i] as -c and r_q[j] as +c: one evaluation
-                // pulls (rbar_j - rbar_i) back to both states; node j's share goes
-                // through the row accumulator that phase D adds
-                A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+            for (int m = 0; m < D; ++m) {
+                if (m < f) continue;
 #pragma unroll
-                for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
+                for (int n = 0; n < D; ++n) av[n] += sv[m] * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
             }
-            if (++dlt == PER_F + 1) {
-                dlt = 1;
-                ++f;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+                gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
+                ubj[e] = 0.0;
+            }
+            // the pair's term enters r_q[i] as -c and r_q[j] as +c: one evaluation
+            // pulls (rbar_j - rbar_i) back to both states; node j's share goes
+            // through the row accumulator that phase D adds
+            A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+#pragma unroll
+            for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
+        };
+        double sa[D], sb[D];
+        if (q0 < q1) load_s(q0, sa);
+#pragma unroll 1
+        for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
+            if (q + 1 < q1) load_s(q + 1, sb);
+            eval(q, sa);
+            if (q + 1 < q1) {
+                if (q + 2 < q1) load_s(q + 2, sa);
+                eval(q + 1, sb);
             }
         }
     }
@@ -300,27 +324,37 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
 #pragma unroll 1
     for (int it = 0; it < MAXITEM / TS; ++it) {
         const int item = it * TS + tt;
-        const int j = item < st.n_items ? st.ci_j[item] : -1;
-        if (j < 0) continue;
-        double hj[NH], hv[NH], av[D], gb[NC], part[NC], uv[NC];
+        if (item >= st.n_items) continue;  // the chunks are packed: items below n_items are real
+        // the chunk (facet node, N1 volume nodes and C entries) requested at once
+        const int j = st.ci_j[item];
+        int ci[N1];
+        double cv[N1];
+#pragma unroll
+        for (int w = 0; w < N1; ++w) {
+            ci[w] = st.ci_i[w * MAXITEM + item];
+            cv[w] = st.ci_v[w * MAXITEM + item];
+        }
+        double hj[NH], part[NC], fnl[D];
 #pragma unroll
         for (int e = 0; e < NC; ++e) part[e] = 0.0;
 #pragma unroll
         for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
         const int fc = j / NPF;
-#pragma unroll 1
-        for (int w = 0; w < st.ch; ++w) {
-            const int i = st.ci_i[w * MAXITEM + item];
+#pragma unroll
+        for (int l = 0; l < D; ++l) fnl[l] = sfn[l + D * fc];
+#pragma unroll
+        for (int w = 0; w < N1; ++w) {
+            const int i = ci[w];
             if (i < 0) continue;
-            const double cv = st.ci_v[w * MAXITEM + item];
+            double hv[NH], av[D], gb[NC], uv[NC];
 #pragma unroll
             for (int v = 0; v < NH; ++v) hv[v] = sh[v * NQ + i];
 #pragma unroll
             for (int n = 0; n < D; ++n) {
                 double s = snJ[n * NF + j];
 #pragma unroll
-                for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * p.fn[l + D * fc];
-                av[n] = 0.5 * cv * s;
+                for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * fnl[l];
+                av[n] = 0.5 * cv[w] * s;
             }
 #pragma unroll
             for (int e = 0; e < NC; ++e) {

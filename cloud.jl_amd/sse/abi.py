@@ -13,7 +13,7 @@ SSE_ERR_HIP = 3
 SSE_ERR_NONPHYSICAL_STATE = 4
 SSE_ERR_OUT_OF_MEMORY = 5
 
-SSE_LINE, SSE_TRI, SSE_TET = 1, 2, 3
+SSE_LINE, SSE_TRI, SSE_TET, SSE_HEX = 1, 2, 3, 5
 SSE_EULER, SSE_LINEAR_ADVECTION, SSE_LINEAR_ADVECTION_DIFFUSION = 1, 2, 3
 SSE_FLUX_DIFFERENCING, SSE_STANDARD = 1, 2
 SSE_LAX_FRIEDRICHS, SSE_CENTRAL, SSE_ENTROPY_CONSERVATIVE_NUMERICAL_FLUX = 1, 2, 3
@@ -54,8 +54,10 @@ EXPORTED_SYMBOLS = (
     "sse_batch_create", "sse_batch_rk_steps", "sse_batch_describe", "sse_batch_last_error",
     "sse_batch_destroy", "sse_functional", "sse_residual_vjp", "sse_rk_adjoint", "sse_batch_rk_tangent",
     "sse_geometric_factors", "sse_apply_reference_mapping", "sse_project_l2", "sse_l2_error",
-    "sse_functional_noalloc",
+    "sse_functional_noalloc", "sse_erk_steps", "sse_ec_flux_batch",
 )
+
+SSE_ERK_MAX_STAGES = 16
 
 # Workspace of sse_functional_noalloc, in doubles (SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES).
 SSE_FUNCTIONAL_NOALLOC_MAX_DOUBLES = 1 << 20

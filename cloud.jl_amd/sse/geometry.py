@@ -25,7 +25,7 @@ import numpy as np
 
 from . import simplex as S
 from .mesh import MeshData
-from .reference_approximation import LINE, TET, TRI, ReferenceApproximation, num_faces
+from .reference_approximation import HEX, LINE, TET, TRI, ReferenceApproximation, num_faces
 
 
 class ExactMetrics:
@@ -144,6 +144,7 @@ def _tet_curl_operators(rd):
 
 def geometric_factors_curl(mesh: MeshData, ref: ReferenceApproximation) -> GeometricFactors:
     """Conservative-curl (Chan-Wilcox) metrics: Tri ``mesh.jl:284-339``,
+    Hex ``mesh.jl:341-408`` (the curl form on the mapping nodes, then interpolated),
     Tet ``mesh.jl:410-506``."""
     rd = ref.reference_element
     elem = rd.element_type
@@ -154,6 +155,18 @@ def geometric_factors_curl(mesh: MeshData, ref: ReferenceApproximation) -> Geome
         N_q, N_e = rd.Vq.shape[0], J.shape[1]
         Lq = np.empty((N_q, 2, 2, N_e))
         Lf = np.empty((rd.Vf.shape[0], 2, 2, N_e))
+        for (l, m), v in comps.items():
+            Lq[:, l, m, :] = rd.Vq @ v
+            Lf[:, l, m, :] = rd.Vf @ v
+        J_q = rd.Vq @ J
+    elif elem == HEX:
+        g = _gf3d(*mesh.xyz, *rd.Drst)
+        rxJ, sxJ, txJ, ryJ, syJ, tyJ, rzJ, szJ, tzJ, J = g
+        comps = {(0, 0): rxJ, (1, 0): sxJ, (2, 0): txJ, (0, 1): ryJ, (1, 1): syJ, (2, 1): tyJ,
+                 (0, 2): rzJ, (1, 2): szJ, (2, 2): tzJ}
+        N_e = J.shape[1]
+        Lq = np.empty((rd.Vq.shape[0], 3, 3, N_e))
+        Lf = np.empty((rd.Vf.shape[0], 3, 3, N_e))
         for (l, m), v in comps.items():
             Lq[:, l, m, :] = rd.Vq @ v
             Lf[:, l, m, :] = rd.Vf @ v

@@ -42,6 +42,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_residual.restype = C.c_int
     lib.sse_rk_steps.argtypes = [vp, vp, C.c_double, C.c_double, C.c_int32, C.c_int32]
     lib.sse_rk_steps.restype = C.c_int
+    lib.sse_ec_flux_batch.argtypes = [C.c_int32, C.c_double, C.c_int32, vp, vp, vp, vp, C.c_int32]
+    lib.sse_ec_flux_batch.restype = C.c_int
+    lib.sse_erk_steps.argtypes = [vp, vp, C.c_double, C.c_double, C.c_int32, C.c_int32, dp, dp, C.c_int32]
+    lib.sse_erk_steps.restype = C.c_int
     lib.sse_profile_residual.argtypes = [vp, vp, vp, C.c_int32, dp]
     lib.sse_profile_residual.restype = C.c_int
     lib.sse_set_stream.argtypes = [vp, vp]

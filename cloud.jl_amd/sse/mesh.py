@@ -21,7 +21,7 @@ from typing import Optional, Tuple
 
 import numpy as np
 
-from .reference_approximation import LINE, TET, TRI, RefElemData
+from .reference_approximation import HEX, LINE, TET, TRI, RefElemData
 
 
 @dataclass
@@ -101,6 +101,21 @@ def uniform_mesh_tet(Kx: int, Ky: int, Kz: int):
         tets.append(np.stack(verts, axis=1))
     EToV = np.stack(tets, axis=1).reshape(-1, 4)  # 6 consecutive tets per hex
     return (VX, VY, VZ), EToV
+
+
+def uniform_mesh_hex(Kx: int, Ky: int, Kz: int):
+    """Hexahedra of a uniform grid on [-1, 1]^3; vertex id of (ix, iy, iz) is
+    ix + (Kx+1)(iy + (Ky+1) iz), element vertices in the reference order (r fastest)."""
+    xs = np.linspace(-1.0, 1.0, Kx + 1)
+    ys = np.linspace(-1.0, 1.0, Ky + 1)
+    zs = np.linspace(-1.0, 1.0, Kz + 1)
+    IZ, IY, IX = np.meshgrid(np.arange(Kz + 1), np.arange(Ky + 1), np.arange(Kx + 1), indexing="ij")
+    VX, VY, VZ = xs[IX.reshape(-1)], ys[IY.reshape(-1)], zs[IZ.reshape(-1)]
+    ez, ey, ex = np.meshgrid(np.arange(Kz), np.arange(Ky), np.arange(Kx), indexing="ij")
+    ex, ey, ez = ex.reshape(-1), ey.reshape(-1), ez.reshape(-1)
+    verts = [(ex + (v & 1)) + (Kx + 1) * ((ey + ((v >> 1) & 1)) + (Ky + 1) * (ez + ((v >> 2) & 1)))
+             for v in range(8)]
+    return (VX, VY, VZ), np.stack(verts, axis=1).astype(np.int64)
 
 
 def collapsed_orientation(VXYZ, EToV):
@@ -198,6 +213,8 @@ def uniform_periodic_mesh(ref, limits, M, tri_diagonal: str = "bl-tr") -> MeshDa
     elif elem == TET:
         VXYZ, EToV = uniform_mesh_tet(M[0], M[1], M[2])
         EToV = collapsed_orientation(VXYZ, EToV)
+    elif elem == HEX:
+        VXYZ, EToV = uniform_mesh_hex(M[0], M[1], M[2])
     else:
         raise NotImplementedError(elem)
     return make_mesh(rd, VXYZ, EToV, limits)

@@ -4,7 +4,7 @@ SBP operators on the Line, Tri and Tet.
 
 Restates ``src/SpatialDiscretizations/quadrature_rules.jl``,
 ``src/SpatialDiscretizations/tensor_simplex.jl``,
-``src/SpatialDiscretizations/tensor_cartesian.jl:1-32`` and
+``src/SpatialDiscretizations/tensor_cartesian.jl:1-32, 85-131`` and
 ``src/SpatialDiscretizations/ref_elem_data.jl`` of the reference.
 All matrices are dense NumPy arrays (the reference builds LinearMaps; the dense
 form of each map is what crosses the C-ABI, and the library recovers the
@@ -325,6 +325,18 @@ def _tet_face_vertices():
     return ((0, 1, 3), (1, 2, 3), (0, 2, 3), (0, 1, 2))
 
 
+def _hex_face_vertices():
+    # vertices (r, s, t) in {-1, 1}^3 numbered with r fastest; faces r = -1, r = 1,
+    # s = -1, s = 1, t = -1, t = 1 (StartUpDG's face order for the Hex)
+    return ((0, 2, 4, 6), (1, 3, 5, 7), (0, 1, 4, 5), (2, 3, 6, 7), (0, 1, 2, 3), (4, 5, 6, 7))
+
+
+def hex_vertices():
+    """Reference vertices of the Hex, r fastest: (8, 3)."""
+    return np.array([[-1.0 + 2 * (v & 1), -1.0 + 2 * ((v >> 1) & 1), -1.0 + 2 * ((v >> 2) & 1)]
+                     for v in range(8)])
+
+
 def ref_elem_data_line(N: int, rq: np.ndarray, wq: np.ndarray) -> RefElemData:
     r = P.gauss_lobatto_legendre_nodes(N)
     VDM = P.vandermonde_1d(N, r)
@@ -378,6 +390,56 @@ def ref_elem_data_tet(N: int, volume_rule, facet_rule) -> RefElemData:
     return RefElemData(TET, N, (r, s, t), VDM, (Vr @ Vinv, Vs @ Vinv, Vt @ Vinv), V1,
                        (rq, sq, tq), wq, Vq, (rf, sf, tf), wf, Vf, (nrJ, nsJ, ntJ),
                        _tet_face_vertices())
+
+
+def ref_elem_data_hex(N: int, volume_rule, facet_rule) -> RefElemData:
+    """StartUpDG's ``RefElemData(Hex(), N; quad_rule_vol, quad_rule_face)``: mapping
+    nodes the tensor Gauss-Lobatto grid of degree N (the first coordinate slowest,
+    as the volume quadrature grid), nodal derivative and interpolation operators
+    built from 1D ones (exact for the tensor-product polynomial space), facet nodes
+    on the faces r = -1, 1, s = -1, 1, t = -1, 1 from the Quad rule (its first and
+    second coordinates are (s, t), (r, t) and (r, s) on those faces)."""
+    r1 = P.gauss_lobatto_legendre_nodes(N)
+    V1d = P.vandermonde_1d(N, r1)
+    V1inv = np.linalg.inv(V1d)
+    D1 = P.grad_vandermonde_1d(N, r1) @ V1inv
+    I1 = np.eye(N + 1)
+    r, s, t = _grid3(r1, r1, r1)
+    VDM = _kron(V1d, V1d, V1d)
+    Drst = (_kron(D1, I1, I1), _kron(I1, D1, I1), _kron(I1, I1, D1))
+    vol = volume_rule if isinstance(volume_rule, tuple) else (volume_rule,) * 3
+    x1, w1 = quadrature_line(vol[0])
+    x2, w2 = quadrature_line(vol[1])
+    x3, w3 = quadrature_line(vol[2])
+    rq, sq, tq = _grid3(x1, x2, x3)
+    wa, wb, wc = _grid3(w1, w2, w3)
+    wq = wa * wb * wc
+    interp = lambda x: P.vandermonde_1d(N, x) @ V1inv  # 1D mapping-node interpolation to points x
+    Vq = _kron(interp(x1), interp(x2), interp(x3))
+    fr = facet_rule if isinstance(facet_rule, tuple) else (facet_rule,) * 2
+    y1, v1 = quadrature_line(fr[0])
+    y2, v2 = quadrature_line(fr[1])
+    a, b = _grid2(y1, y2)
+    va, vb = _grid2(v1, v2)
+    wq2 = va * vb
+    e = np.ones_like(a)
+    z = np.zeros_like(a)
+    rf = np.concatenate([-e, e, a, a, a, a])
+    sf = np.concatenate([a, a, -e, e, b, b])
+    tf = np.concatenate([b, b, b, b, -e, e])
+    wf = np.concatenate([wq2] * 6)
+    nrJ = np.concatenate([-e, e, z, z, z, z])
+    nsJ = np.concatenate([z, z, -e, e, z, z])
+    ntJ = np.concatenate([z, z, z, z, -e, e])
+    L, R = interp([-1.0]), interp([1.0])
+    A, B = interp(y1), interp(y2)
+    Vf = np.vstack([_kron(L, A, B), _kron(R, A, B), _kron(A, L, B), _kron(A, R, B), _kron(A, B, L),
+                    _kron(A, B, R)])
+    hv = hex_vertices()
+    V1 = np.stack([np.prod([0.5 * (1.0 + hv[v, m] * np.array((r, s, t)[m])) for m in range(3)], axis=0)
+                   for v in range(8)], axis=1)
+    return RefElemData(HEX, N, (r, s, t), VDM, Drst, V1, (rq, sq, tq), wq, Vq, (rf, sf, tf), wf, Vf,
+                       (nrJ, nsJ, ntJ), _hex_face_vertices())
 
 
 # ----------------------------------------------------------------------------- ReferenceApproximation
@@ -593,6 +655,42 @@ def ReferenceApproximationTet(approx_type, mapping_degree: int = 1, volume_quadr
                                   ReferenceMapping(J_ref, L_ref), wp, None, tuple(vrule), tuple(frule))
 
 
+def ReferenceApproximationHex(approx_type, mapping_degree: int = 1, volume_quadrature_rule=None,
+                              facet_quadrature_rule=None):
+    """NodalTensor on the Hex (``tensor_cartesian.jl:85-131``): Gauss-Lobatto volume
+    and facet rules by default, D = (D1 x I x I, I x D1 x I, I x I x D1), V = I, and
+    R a selection of the volume nodes on the faces when both rules are the same
+    Gauss-Lobatto rule (diagonal E), else the interpolation to the facet nodes."""
+    if not isinstance(approx_type, NodalTensor):
+        raise NotImplementedError("the Hex is implemented for NodalTensor")
+    p = approx_type.p
+    vrule = volume_quadrature_rule or LGLQuadrature(p)
+    frule = facet_quadrature_rule or LGLQuadrature(p)
+    rd = ref_elem_data_hex(mapping_degree, vrule, frule)
+    x1, _ = quadrature_line(vrule)
+    q = len(x1) - 1
+    V1 = P.vandermonde_1d(q, x1)
+    V1inv = np.linalg.inv(V1)
+    D1 = P.grad_vandermonde_1d(q, x1) @ V1inv
+    I1 = np.eye(q + 1)
+    D = (_kron(D1, I1, I1), _kron(I1, D1, I1), _kron(I1, I1, D1))
+    rq = np.stack(rd.rstq, axis=1)
+    rf = np.stack(rd.rstf, axis=1)
+    sel = None
+    if vrule == frule and vrule.kind == "lobatto":
+        dist = np.abs(rf[:, None, :] - rq[None, :, :]).max(axis=2)
+        sel = np.argmin(dist, axis=1).astype(np.int64)
+        if dist[np.arange(len(sel)), sel].max() > 1e-12:
+            raise ValueError("facet nodes are not volume nodes")
+        R = np.zeros((rf.shape[0], rq.shape[0]))
+        R[np.arange(len(sel)), sel] = 1.0
+    else:
+        interp = lambda x: P.vandermonde_1d(q, [x]) @ V1inv  # 1 x (q + 1)
+        R = np.vstack([_kron(interp(rf[j, 0]), interp(rf[j, 1]), interp(rf[j, 2])) for j in range(rf.shape[0])])
+    return ReferenceApproximation(NodalTensor(q), rd, D, None, R, rd.wq.copy(), rd.wf.copy(), None, None, sel,
+                                  (vrule,) * 3, (frule,) * 2)
+
+
 def ReferenceApproximationFor(approx_type, element_type: str, **kw):
     if element_type == LINE:
         return ReferenceApproximationLine(approx_type, **kw)
@@ -600,6 +698,8 @@ def ReferenceApproximationFor(approx_type, element_type: str, **kw):
         return ReferenceApproximationTri(approx_type, **kw)
     if element_type == TET:
         return ReferenceApproximationTet(approx_type, **kw)
+    if element_type == HEX:
+        return ReferenceApproximationHex(approx_type, **kw)
     raise NotImplementedError(element_type)
 
 

@@ -27,7 +27,7 @@ from .conservation_laws import (BR1, CentralNumericalFlux, ConservativeFlux,
                                 EulerEquations, LaxFriedrichsNumericalFlux,
                                 LinearAdvectionDiffusionEquation, LinearAdvectionEquation)
 from .geometry import SpatialDiscretization, apply_reference_mapping
-from .reference_approximation import (LINE, TET, TRI, face_reference_normals,
+from .reference_approximation import (HEX, LINE, TET, TRI, face_reference_normals,
                                       flux_differencing_operators, num_faces)
 
 
@@ -85,7 +85,7 @@ def default_mass_matrix_solver(sd: SpatialDiscretization):
 
 
 # ----------------------------------------------------------------------------- descriptor
-_ELEM = {LINE: abi.SSE_LINE, TRI: abi.SSE_TRI, TET: abi.SSE_TET}
+_ELEM = {LINE: abi.SSE_LINE, TRI: abi.SSE_TRI, TET: abi.SSE_TET, HEX: abi.SSE_HEX}
 
 
 def _fptr(a: Optional[np.ndarray]):

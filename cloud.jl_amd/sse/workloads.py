@@ -14,8 +14,8 @@ from .conservation_laws import (BR1, CentralNumericalFlux, EntropyConservativeNu
 from .geometry import ChanWilcoxMetrics, make_spatial_discretization
 from .mesh import ChanWarping, UniformWarping, uniform_periodic_mesh, warp_mesh
 from .reference_approximation import (LGQuadrature, ModalMulti, ModalTensor, NodalTensor,
-                                      ReferenceApproximationLine, ReferenceApproximationTet,
-                                      ReferenceApproximationTri)
+                                      ReferenceApproximationHex, ReferenceApproximationLine,
+                                      ReferenceApproximationTet, ReferenceApproximationTri)
 from .solvers import (FluxDifferencingForm, PhysicalOperator, ReferenceOperator,
                       SkewSymmetricMapping, StandardForm, StandardMapping, semidiscretize)
 
@@ -33,6 +33,25 @@ def euler_1d_gauss():
     sd = make_spatial_discretization(mesh, ref)
     form = FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux())
     return law, sd, exact, form, semidiscretize(law, sd, exact, form, (0.0, 2.0))
+
+
+def euler_3d_hex(M=2, p=4):
+    """test/euler_3d.jl:1-90 (golden L2 at test/runtests.jl:131-144): the Euler periodic
+    wave on [0, 2]^3, M^3 hexes curved by ChanWarping(1/16), NodalTensor(p) with
+    Gauss-Lobatto volume and facet rules, conservative-curl metrics, the EC interface
+    flux. Returns (law, sd, exact, form, ode, T, N_t); the reference integrates with
+    DP8 at dt = T / N_t."""
+    L = 2.0
+    T = L
+    law = EulerEquations(3, 1.4)
+    exact = EulerPeriodicTest(3, 1.4)
+    form = FluxDifferencingForm(inviscid_numerical_flux=EntropyConservativeNumericalFlux())
+    ref = ReferenceApproximationHex(NodalTensor(p), mapping_degree=p)
+    mesh = uniform_periodic_mesh(ref, ((0.0, L),) * 3, (M, M, M))
+    mesh = warp_mesh(mesh, ref, ChanWarping(1.0 / 16.0, (L, L, L)))
+    sd = make_spatial_discretization(mesh, ref, ChanWilcoxMetrics())
+    ode = semidiscretize(law, sd, exact, form, (0.0, T))
+    return law, sd, exact, form, ode, T, 25 * M * (p + 1)
 
 
 def euler_vortex_2d_modal(M=4, form=None, p=3, approx=ModalTensor):

@@ -33,8 +33,8 @@ typedef enum {
     SSE_ERR_OUT_OF_MEMORY = 5
 } sse_status;
 
-/* element shapes (StartUpDG Line/Tri/Tet) */
-typedef enum { SSE_LINE = 1, SSE_TRI = 2, SSE_TET = 3 } sse_element;
+/* element shapes (StartUpDG Line/Tri/Tet/Hex) */
+typedef enum { SSE_LINE = 1, SSE_TRI = 2, SSE_TET = 3, SSE_HEX = 5 } sse_element;
 
 /* conservation laws (src/ConservationLaws/ in the reference) */
 typedef enum {
@@ -162,6 +162,20 @@ int sse_residual(sse_handle* h, const double* u, double* dudt, double t, int32_t
  * `CarpenterKennedy2N54()` integrator the reference hands its ODEProblem to
  * (test/test_driver.jl:77-83), with the stage update fused into the residual. */
 int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t where);
+
+/* Largest stage count sse_erk_steps accepts. */
+#define SSE_ERK_MAX_STAGES 16
+
+/* Advances u (N_p, N_c, N_e) by nsteps steps of size dt of the explicit Runge-Kutta
+ * method with the Butcher tableau (A: stages x stages, row-major, strictly lower
+ * triangular; b: stages weights), as OrdinaryDiffEq runs an explicit method with
+ * adaptive=false: k_i = R(u + dt sum_{j<i} A[i][j] k_j), then u += dt sum_i b_i k_i.
+ * With the Dormand-Prince 8(5,3) tableau this replaces solve(ode, DP8(),
+ * adaptive=false, dt=dt) of the reference's 3D Euler test (test/euler_3d.jl:45-53).
+ * The stage states are combined on the device (one kernel per stage); the library
+ * keeps stages + 1 state-sized buffers on the handle. A and b are host arrays. */
+int sse_erk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t stages,
+                  const double* A, const double* b, int32_t where);
 
 /* Runs the residual nrep times on device buffers and reports the mean duration
  * of each kernel slot in milliseconds (ms_out has SSE_NUM_KERNELS entries),
@@ -359,6 +373,16 @@ int sse_apply_reference_mapping(int32_t N_q, int32_t d, int32_t N_e, const doubl
  * (the reference's cholesky throws there). */
 int sse_project_l2(int32_t N_q, int32_t N_p, int32_t N_c, int32_t N_e, const double* V, const double* W,
                    const double* J_q, const double* u_q, double* u, int32_t where);
+
+/* The device's entropy-conservative two-point flux of the Euler equations, contracted with
+ * a vector: c[t] = sum_n a[t, n] F(uL[t], uR[t])[:, n], F the Ranocha flux of
+ * compute_two_point_flux (src/ConservationLaws/euler_navierstokes.jl:171-195), evaluated
+ * by the same device code as the residual kernels (hoisted logarithms, one refined
+ * reciprocal per pair). uL, uR, c are n x (d + 2) and a is n x d, row-major (one pair per
+ * row), at `where`. No counterpart in the reference: a test aid that checks the flux the
+ * kernels evaluate against the oracle's, pair by pair. Blocks until done. */
+int sse_ec_flux_batch(int32_t d, double gamma, int32_t n, const double* uL, const double* uR, const double* a,
+                      double* c, int32_t where);
 
 #ifdef __cplusplus
 }

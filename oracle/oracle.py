@@ -36,6 +36,8 @@ def lib() -> C.CDLL:
         L.sse_oracle_residual_repeat.restype = C.c_int
         L.sse_oracle_rk_steps.argtypes = [vp, vp, C.c_double, C.c_int]
         L.sse_oracle_rk_steps.restype = C.c_int
+        L.sse_oracle_erk_steps.argtypes = [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]
+        L.sse_oracle_erk_steps.restype = C.c_int
         L.sse_oracle_last_error.restype = C.c_char_p
         L.sse_oracle_set_threads.argtypes = [C.c_int]
         L.sse_oracle_max_threads.restype = C.c_int
@@ -82,6 +84,18 @@ def residual_repeat(arrays, u: np.ndarray, nrep: int) -> np.ndarray:
 def rk_steps(arrays, u: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
     u = np.array(u, dtype=np.float64, order="F", copy=True)
     _check(lib().sse_oracle_rk_steps(C.addressof(arrays.desc), u.ctypes.data, float(dt), int(nsteps)))
+    return u
+
+
+def erk_steps(arrays, u: np.ndarray, dt: float, nsteps: int, A: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """nsteps steps of the explicit RK method with Butcher tableau (A, b)."""
+    u = np.array(u, dtype=np.float64, order="F", copy=True)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    s = len(b)
+    assert A.shape == (s, s)
+    _check(lib().sse_oracle_erk_steps(C.addressof(arrays.desc), u.ctypes.data, float(dt), int(nsteps), s,
+                                      A.ctypes.data, b.ctypes.data))
     return u
 
 

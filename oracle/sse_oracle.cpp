@@ -860,6 +860,41 @@ int sse_oracle_rk_steps(const sse_desc* desc, double* u, double dt, int nsteps) 
     return 0;
 }
 
+// nsteps steps of an explicit Runge-Kutta method with s stages given by its Butcher
+// tableau (A: s x s row-major, strictly lower triangular; b: s weights), fixed step
+// dt, as OrdinaryDiffEq's explicit methods take with adaptive=false: stage i is
+// k_i = f(u + dt sum_{j<i} A[i][j] k_j), then u += dt sum_i b_i k_i. With the
+// Dormand-Prince 8(5,3) tableau this is DP8() (test/euler_3d.jl:49-56). The
+// residual is autonomous, so the nodes c_i are not needed.
+int sse_oracle_erk_steps(const sse_desc* desc, double* u, double dt, int nsteps, int s, const double* A,
+                         const double* b) {
+    if (s < 1 || s > 32) {
+        g_err = "stages must be 1..32";
+        return SSE_ERR_INVALID_ARGUMENT;
+    }
+    Oracle<double> o;
+    g_err = o.init(desc);
+    if (!g_err.empty()) return SSE_ERR_UNSUPPORTED;
+    const size_t n = (size_t)desc->N_p * desc->N_c * desc->N_e;
+    vector<double> k((size_t)s * n), y(n);
+    for (int step = 0; step < nsteps; ++step) {
+        for (int i = 0; i < s; ++i) {
+            for (size_t t = 0; t < n; ++t) {
+                double acc = 0.0;
+                for (int j = 0; j < i; ++j) acc += A[i * s + j] * k[(size_t)j * n + t];
+                y[t] = u[t] + dt * acc;
+            }
+            o.residual(y.data(), k.data() + (size_t)i * n, g_threads);
+        }
+        for (size_t t = 0; t < n; ++t) {
+            double acc = 0.0;
+            for (int i = 0; i < s; ++i) acc += b[i] * k[(size_t)i * n + t];
+            u[t] += dt * acc;
+        }
+    }
+    return 0;
+}
+
 // Directional derivative of the residual along (du, dlambda, db), lambda the
 // LaxFriedrichsNumericalFlux coefficient (halfλ = λ / 2) and b the diffusion
 // coefficient: dudt = R(u), ddudt = R'(u) du + dR/dλ dlambda + dR/db db.

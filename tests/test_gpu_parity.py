@@ -356,6 +356,112 @@ def test_gpu_reproduces_reference_golden_advection_2d_tri():
     np.testing.assert_allclose(err, [0.2660013939427627], rtol=0, atol=1e-10)
 
 
+def test_gpu_reproduces_reference_golden_euler_3d_nodal_tensor_hex(tmp_path):
+    # test/runtests.jl:131-144 with the flow of test/euler_3d.jl: NodalTensor(4) Hex
+    # (generic kernels: sparse S, diagonal E, EC interface flux), DP8 at dt = T / 250
+    # (sse_erk_steps) with save_callback every 5 steps, then the three analyses
+    from sse import io
+    from sse.analysis import (EntropyConservationAnalysis, ErrorAnalysis, PrimaryConservationAnalysis,
+                              analyze)
+    from sse.time_integration import DP8, solve
+    from sse.workloads import euler_3d_hex
+    law, sd, exact, form, ode, T, N_t = euler_3d_hex()
+    assert ode.p.describe().startswith("generic"), ode.p.describe()
+    path = io.save_project(law, sd, exact, form, ode.tspan, str(tmp_path / "res"))
+    sol = solve(ode, DP8(), dt=T / N_t, callback=io.SaveCallback(path, interval=N_t // 50))
+    err = analyze(ErrorAnalysis(ode.p, path), sol.u, exact, T)
+    gold = [0.18342164491797003, 0.1834216449179776, 0.18342164491796725, 0.18342164491796784,
+            0.2751324673769553]
+    np.testing.assert_allclose(err, gold, rtol=0, atol=1e-10)
+    steps = io.load_time_steps(path)
+    assert len(steps) == 51
+    cons = analyze(PrimaryConservationAnalysis(ode.p, path), steps)
+    np.testing.assert_allclose(cons.E[-1] - cons.E[0], 0.0, atol=1e-10)
+    ent = analyze(EntropyConservationAnalysis(ode.p, path), steps)
+    assert np.abs(ent.dEdt[:, 0]).max() < 1e-10
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_device_ec_flux_matches_oracle_flux(d):
+    # the kernels' two-point flux (hoisted logarithms, one refined reciprocal instead of a
+    # divide) against the oracle's Ranocha flux (the reference's formulas), contracted with
+    # a random vector, pair by pair: distinct states, near-equal ones (the series branch of
+    # the logarithmic mean) and equal ones
+    import ctypes as C
+    from sse import abi
+    from sse.hip import check, load_library
+    lib = load_library()
+    rng = np.random.default_rng(31 + d)
+    n, gamma = 3000, 1.4
+    nc = d + 2
+
+    def states(m):
+        rho = rng.uniform(0.3, 3.0, m)
+        v = rng.uniform(-1.5, 1.5, (m, d))
+        p = rng.uniform(0.3, 3.0, m)
+        E = p / (gamma - 1.0) + 0.5 * rho * (v ** 2).sum(1)
+        return np.column_stack([rho, rho[:, None] * v, E])
+
+    uL = states(n)
+    uR = states(n)
+    uR[1000:2000] = uL[1000:2000] * (1.0 + 1e-5 * rng.standard_normal((1000, nc)))
+    uR[2000:2200] = uL[2000:2200]
+    a = rng.standard_normal((n, d))
+    c = np.zeros((n, nc))
+    dp = lambda x: np.ascontiguousarray(x).ctypes.data
+    uL, uR, a = (np.ascontiguousarray(x) for x in (uL, uR, a))
+    check(lib, None, lib.sse_ec_flux_batch(d, gamma, n, dp(uL), dp(uR), dp(a), c.ctypes.data, abi.SSE_HOST))
+    worst = 0.0
+    for t in range(n):
+        F = O.two_point(1, d, gamma, uL[t], uR[t])   # N_c x d
+        ref = F @ a[t]
+        err = np.abs(c[t] - ref).max() / np.abs(ref).max()
+        worst = max(worst, err)
+    assert worst <= 2e-14, worst
+
+
+def test_rk_steps_graph_replay_matches_eager_and_oracle(monkeypatch):
+    # sse_rk_steps replays one captured step; a new state pointer or a new dt recaptures
+    from sse.time_integration import rk_steps, to_device, to_host
+    ode = vortex_2d_config(M=8, p=4)[4]
+    dt = 1e-2
+    ua, ub = to_device(ode.u0), to_device(ode.u0)
+    rk_steps(ode.p, ua, dt, 12)
+    rk_steps(ode.p, ub, dt, 12)         # another state: recaptured
+    rk_steps(ode.p, ub, 0.5 * dt, 6)    # another step size: recaptured
+    rk_steps(ode.p, ua, 0.5 * dt, 6)
+    monkeypatch.setenv("SSE_RK_GRAPH", "0")
+    ue = to_device(ode.u0)
+    rk_steps(ode.p, ue, dt, 12)
+    rk_steps(ode.p, ue, 0.5 * dt, 6)
+    ref = O.rk_steps(ode.p.arrays, O.rk_steps(ode.p.arrays, ode.u0, dt, 12), 0.5 * dt, 6)
+    for x in (ua, ub):
+        assert _rel(to_host(x), to_host(ue)) <= 1e-13
+        assert _rel(to_host(x), ref) <= RK_TOL
+
+
+def test_erk_steps_match_oracle_and_host_path():
+    # DP8 through sse_erk_steps on a ModalTensor Tet (the line-structured kernels) and a
+    # Tri against the oracle's explicit RK, device and host memory
+    from sse.time_integration import DP8, erk_steps, to_device, to_host
+    alg = DP8()
+    for ode, dt in ((density_wave_3d_config(M=2, p=3, ic_strength=0.2)[4], 2e-3),
+                    (vortex_2d_config(M=4, p=4)[4], 1e-2)):
+        ref = O.erk_steps(ode.p.arrays, ode.u0, dt, 6, alg.A, alg.b)
+        ud = to_device(ode.u0)
+        erk_steps(ode.p, ud, dt, 6, alg)
+        assert _rel(to_host(ud), ref) <= RK_TOL, _rel(to_host(ud), ref)
+        import ctypes as C
+        from sse import abi
+        from sse.hip import check
+        uh = np.array(ode.u0, order="F", copy=True)
+        dp = C.POINTER(C.c_double)
+        lib, h = ode.p.lib, ode.p.handle
+        check(lib, h, lib.sse_erk_steps(h, uh.ctypes.data, 0.0, dt, 6, 12, alg.A.ctypes.data_as(dp),
+                                        alg.b.ctypes.data_as(dp), abi.SSE_HOST))
+        assert _rel(uh, ref) <= RK_TOL
+
+
 # ----------------------------------------------------------------------------- batched problems
 def test_batched_problems_match_individual_solves():
     """Problems of uneven size and degree advanced together (grouped launches,
