@@ -681,6 +681,18 @@ int build_struct(sse_handle* h, const sse_desc* D) {
     }
     int stride[3] = {1, 1, 1};
     for (int f = d - 2; f >= 0; --f) stride[f] = stride[f + 1] * N1;
+    // The kernels evaluate each line pair once, at the node that owns it, and give the
+    // partner -S_m[i, j]: S must be antisymmetric (it is 1/2 (W D - D' W) in the reference);
+    // a descriptor whose S is not is left to the general kernels.
+    double smax = 0.0;
+    for (size_t t = 0; t < (size_t)N_q * N_q * d; ++t) smax = std::max(smax, std::fabs(D->S[t]));
+    for (int m = 0; m < d; ++m)
+        for (int i = 0; i < N_q; ++i)
+            for (int j = 0; j < i; ++j) {
+                const double a = D->S[i + (size_t)N_q * (j + (size_t)N_q * m)];
+                const double b = D->S[j + (size_t)N_q * (i + (size_t)N_q * m)];
+                if (std::fabs(a + b) > 1e-12 * smax) return 0;
+            }
     // S on tensor lines, S_m[i, j] = 0 for m < f (family f); checked against every nonzero
     std::vector<double> SG((size_t)d * N1 * d * N_q, 0.0);
     std::vector<char> covered((size_t)N_q * N_q, 0);

@@ -335,19 +335,28 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             for (int e = 0; e < NC; ++e) sr[e * NQ + tl] = r[e];
         }
         __syncthreads();
-        // ---- phase F: dudt = V' r_q, weight-adjusted mass solve
+        // ---- phase F: dudt = V' r_q, weight-adjusted mass solve; the factors of V are
+        // copied into the (now dead) hoisted-state array when they fit there
         double* sy = sm + L::off_y;
         double* sZ = sm + L::off_z;
         double* sW = sm + L::off_W;
-        SF::Vt(tp, tt, sr, sy, sZ, sW);
-        SF::V(tp, tt, sy, sr, sZ, sW);
+        using VF = VFactors<ELEM, P>;
+        constexpr bool STAGE = VF::size <= NH * NQ;
+        TensorParams tq = tp;
+        if constexpr (STAGE) {
+            VF::stage(tp, sm + L::off_h, tt, TS);
+            tq = VF::in_lds(tp, sm + L::off_h);
+            __syncthreads();
+        }
+        SF::Vt(tq, tt, sr, sy, sZ, sW);
+        SF::V(tq, tt, sy, sr, sZ, sW);
         if (half == 0 && tl < NQ) {
             const double jw = Wi / Ji;
 #pragma unroll
             for (int e = 0; e < NC; ++e) sr[e * NQ + tl] *= jw;
         }
         __syncthreads();
-        SF::Vt(tp, tt, sr, sy, sZ, sW);
+        SF::Vt(tq, tt, sr, sy, sZ, sW);
         const size_t base = (size_t)k * NP * NC;
         if (!valid) {
         } else if (!rk.active) {

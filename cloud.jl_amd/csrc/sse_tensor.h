@@ -93,6 +93,33 @@ __device__ __forceinline__ int element_of_block(int b, int nb, int remap) {
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// ----------------------------------------------------------------------------- factors of V in LDS
+// The sum-factorisation factors A, B (and C on the Tet) of V are a few KB read by every
+// stage of every application; a kernel copies them to LDS once and hands SumFact a
+// TensorParams whose VA, VB, VC point there (LDS latency instead of cache latency in the
+// factor loops).
+template <int ELEM, int P>
+struct VFactors {
+    static constexpr int N1 = P + 1;
+    static constexpr int NA = N1 * (P + 1), NB = N1 * (P + 1) * (P + 1);
+    static constexpr int NCF = ELEM == ELEM_TET ? N1 * (P + 1) * (P + 1) * (P + 1) : 0;
+    static constexpr int size = NA + NB + NCF;
+    // all `nt` threads of the caller copy; a barrier must follow before use
+    __device__ static void stage(const TensorParams& tp, double* dst, int t, int nt) {
+        for (int i = t; i < NA; i += nt) dst[i] = tp.VA[i];
+        for (int i = t; i < NB; i += nt) dst[NA + i] = tp.VB[i];
+        if constexpr (NCF > 0)
+            for (int i = t; i < NCF; i += nt) dst[NA + NB + i] = tp.VC[i];
+    }
+    __device__ static TensorParams in_lds(const TensorParams& tp, const double* src) {
+        TensorParams r = tp;
+        r.VA = src;
+        r.VB = src + NA;
+        r.VC = NCF > 0 ? src + NA + NB : tp.VC;
+        return r;
+    }
+};
+
 // ----------------------------------------------------------------------------- sum factorisation
 // All arrays are component-major LDS arrays: x[e * stride + point]. NCV
 // components (conserved variables, or several elements or directions side by

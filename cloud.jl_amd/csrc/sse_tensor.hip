@@ -63,10 +63,15 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
     double* sW = sm + L::off_W;
     using Ph = Physics<D, LAW_EULER>;
 
+    // the factors of V, shared by the block's teams, after the teams' arrays
+    using VF = VFactors<ELEM, P>;
+    double* sfac = smem + T::EPB1 * L::size;
+    VF::stage(tp, sfac, threadIdx.x, TS * T::EPB1);
+    const TensorParams tq = VF::in_lds(tp, sfac);
     const double* uk = u + (size_t)kk * NP * NC;
     for (int t = tl; t < NP * NC; t += TS) su[t] = uk[t];
     __syncthreads();
-    SF::V(tp, tl, su, sa, sZ, sW);  // u_q = V u
+    SF::V(tq, tl, su, sa, sZ, sW);  // u_q = V u
     const double* Jk = p.Jq + (size_t)kk * NQ;
     bool ok = true;
     for (int i = tl; i < NQ; i += TS) {  // w_q = WJ c2e(u_q)
@@ -79,16 +84,16 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
         for (int e = 0; e < NC; ++e) sa[e * NQ + i] = wb[e] * wj;
     }
     __syncthreads();
-    SF::Vt(tp, tl, sa, sw, sZ, sW);  // w = V' (WJ w_q)
-    SF::V(tp, tl, sw, sa, sZ, sW);   // WADG: J^{-1} W (V w)
+    SF::Vt(tq, tl, sa, sw, sZ, sW);  // w = V' (WJ w_q)
+    SF::V(tq, tl, sw, sa, sZ, sW);   // WADG: J^{-1} W (V w)
     for (int i = tl; i < NQ; i += TS) {
         const double jw = p.W[i] / Jk[i];
 #pragma unroll
         for (int e = 0; e < NC; ++e) sa[e * NQ + i] *= jw;
     }
     __syncthreads();
-    SF::Vt(tp, tl, sa, sw, sZ, sW);  // w = V' (...)
-    SF::V(tp, tl, sw, sa, sZ, sW);   // w_q = V w
+    SF::Vt(tq, tl, sa, sw, sZ, sW);  // w = V' (...)
+    SF::V(tq, tl, sw, sa, sZ, sW);   // w_q = V w
     double* uqk = p.uq + (size_t)kk * NQ * NC;
     double* ufk = p.uf + (size_t)kk * NF * NC;
     for (int i = tl; i < NQ; i += TS) {  // u_q = e2c(w_q)
@@ -105,12 +110,24 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
         double ub[NC], wb[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) wb[e] = 0.0;
-        for (int w = 0; w < tp.wr; ++w) {
-            const int i = tp.r_idx[w * NF + j];
-            if (i < 0) break;
-            const double rv = tp.r_val[w * NF + j];
+        // the row's entries four at a time, their loads issued together (rows are
+        // padded with index -1; a group whose first entry is padding ends the row)
+        for (int w0 = 0; w0 < tp.wr; w0 += 4) {
+            int ii[4];
+            double rv[4];
 #pragma unroll
-            for (int e = 0; e < NC; ++e) wb[e] += rv * sa[e * NQ + i];
+            for (int q = 0; q < 4; ++q) {
+                const bool in = w0 + q < tp.wr;
+                ii[q] = in ? tp.r_idx[(w0 + q) * NF + j] : -1;
+                rv[q] = in ? tp.r_val[(w0 + q) * NF + j] : 0.0;
+            }
+            if (ii[0] < 0) break;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ii[q] < 0) continue;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) wb[e] += rv[q] * sa[e * NQ + ii[q]];
+            }
         }
         ok &= e2c_fast<D>(p.law, wb, ub);
         if (valid) {
@@ -371,7 +388,8 @@ template <int ELEM, int P, int VAR>
 TensorConfig tensor_config_of() {
     using Sh = Shape<ELEM, P>;
     using T = Tune<ELEM, P, VAR>;
-    return TensorConfig{T::TS, T::EPB, T::R, (int)(sizeof(double) * K1Layout<ELEM, P>::size * T::EPB1),
+    return TensorConfig{T::TS, T::EPB, T::R,
+                        (int)(sizeof(double) * (K1Layout<ELEM, P>::size * T::EPB1 + VFactors<ELEM, P>::size)),
                         (int)(sizeof(double) * K2Layout<ELEM, P, VAR>::size * T::EPB), Sh::NQ, Sh::NF, Sh::NP};
 }
 

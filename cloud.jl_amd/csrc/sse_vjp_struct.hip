@@ -101,6 +101,11 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
     double* sx = sy + NC * NP;       // NC x NQ; also the sum-factorisation temporary Z
     double* sZ = sx;                 // (dead before V's last stage, and after V''s first)
     double* sW = sx + NC * NQ;       // NC x NW
+    // the factors of V in LDS after the work arrays
+    using VF = VFactors<ELEM, P>;
+    double* sfac = sW + NC * (Sh::NW > 0 ? Sh::NW : 1);
+    VF::stage(tp, sfac, tl, TS);
+    const TensorParams tq = VF::in_lds(tp, sfac);
     const size_t kb = (size_t)k * NP * NC;
     for (int t = tl; t < NP * NC; t += TS) {
         double y = in.a == 0.0 ? 0.0 : in.a * ybar[kb + t];
@@ -109,7 +114,7 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
         sy[t] = y;
     }
     __syncthreads();
-    SF::V(tp, tl, sy, sx, sZ, sW);
+    SF::V(tq, tl, sy, sx, sZ, sW);
     const double* Jk = p.Jq + (size_t)k * NQ;
     for (int i = tl; i < NQ; i += TS) {
         const double jw = p.W[i] / Jk[i];
@@ -117,8 +122,8 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
         for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
     }
     __syncthreads();
-    SF::Vt(tp, tl, sx, sy, sZ, sW);
-    SF::V(tp, tl, sy, sx, sZ, sW);
+    SF::Vt(tq, tl, sx, sy, sZ, sW);
+    SF::V(tq, tl, sy, sx, sZ, sW);
     double* rk = rbar + (size_t)k * NQ * NC;
     for (int t = tl; t < NQ * NC; t += TS) rk[t] = sx[t];
     for (int j = tl; j < NF; j += TS) {
@@ -451,14 +456,18 @@ k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __r
     double* su = sm + L::off_u;
     double* su0 = sm + L::off_u0;
     double* sw = sm + L::off_w;
+    using VF = VFactors<ELEM, P>;
+    double* sfac = sm + L::size;  // the factors of V in LDS after the work arrays
+    VF::stage(tp, sfac, tt, TS);
+    const TensorParams tq = VF::in_lds(tp, sfac);
     const double* uk = u + (size_t)k * NP * NC;
     const double* wk = wbar + (size_t)k * NQ * NC;
     for (int t = tt; t < NP * NC; t += TS) su[t] = uk[t];
     for (int t = tt; t < NQ * NC; t += TS) sw[t] = wk[t];
     __syncthreads();
-    SF::V(tp, tt, su, su0, sZ, sW);   // V u
-    SF::Vt(tp, tt, sw, sy, sZ, sW);   // V' (w_q' cotangent)
-    SF::V(tp, tt, sy, sx, sZ, sW);    // M^{-1} = V' (W/J V .)
+    SF::V(tq, tt, su, su0, sZ, sW);   // V u
+    SF::Vt(tq, tt, sw, sy, sZ, sW);   // V' (w_q' cotangent)
+    SF::V(tq, tt, sy, sx, sZ, sW);    // M^{-1} = V' (W/J V .)
     const double* Jk = p.Jq + (size_t)k * NQ;
     for (int i = tt; i < NQ; i += TS) {
         const double jw = p.W[i] / Jk[i];
@@ -466,8 +475,8 @@ k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __r
         for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
     }
     __syncthreads();
-    SF::Vt(tp, tt, sx, sy, sZ, sW);
-    SF::V(tp, tt, sy, sx, sZ, sW);    // V (M^{-1} V' w_q'bar), then times WJ and through c2e
+    SF::Vt(tq, tt, sx, sy, sZ, sW);
+    SF::V(tq, tt, sy, sx, sZ, sW);    // V (M^{-1} V' w_q'bar), then times WJ and through c2e
     for (int i = tt; i < NQ; i += TS) {
         const double wj = p.W[i] * Jk[i];
         double u0[NC], wb[NC], ub[NC];
@@ -482,7 +491,7 @@ k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __r
         for (int e = 0; e < NC; ++e) sw[e * NQ + i] = ub[e];
     }
     __syncthreads();
-    SF::Vt(tp, tt, sw, sy, sZ, sW);
+    SF::Vt(tq, tt, sw, sy, sZ, sW);
     double* outk = ubar + (size_t)k * NP * NC;
     for (int t = tt; t < NP * NC; t += TS) outk[t] = (out.accumulate ? outk[t] : 0.0) + out.scale * sy[t];
 }
@@ -491,8 +500,9 @@ k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __r
 template <int ELEM, int P>
 size_t vjp_tensor_lds(int which) {
     using Sh = Shape<ELEM, P>;
-    if (which == 1) return sizeof(double) * Sh::NC * (Sh::NP + Sh::NQ + (Sh::NW > 0 ? Sh::NW : 1));
-    if (which == 3) return sizeof(double) * VjpProjLayout<ELEM, P>::size;
+    if (which == 1)
+        return sizeof(double) * (Sh::NC * (Sh::NP + Sh::NQ + (Sh::NW > 0 ? Sh::NW : 1)) + VFactors<ELEM, P>::size);
+    if (which == 3) return sizeof(double) * (VjpProjLayout<ELEM, P>::size + VFactors<ELEM, P>::size);
     return sizeof(double) * VjpLayout<ELEM, P>::size;
 }
 

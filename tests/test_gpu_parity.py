@@ -420,6 +420,24 @@ def test_device_ec_flux_matches_oracle_flux(d):
     assert worst <= 2e-14, worst
 
 
+def test_non_antisymmetric_s_leaves_the_pair_once_kernels():
+    # the line-structured kernels give a pair's partner -S_m[i, j]; a descriptor whose S
+    # is not antisymmetric runs on kernels that, like the reference's flux_difference!
+    # (src/Solvers/flux_differencing_form.jl:37-75), read only the entries with i < j
+    ode = density_wave_3d_config(M=2, p=3, ic_strength=0.2)[4]
+    S = ode.p.arrays.S                      # S[m, j, i] = S_m[i, j]
+    d, n = S.shape[0], S.shape[1]
+    low = np.tril(np.ones((n, n), dtype=bool), -1)     # i > j, stored at S[m, j, i]
+    for m in range(d):
+        Sm = S[m].T.copy()                  # S_m[i, j]
+        Sm[low] *= 1.0 + 1e-3
+        S[m] = Sm.T
+    assert not ode.p.describe().startswith("struct"), ode.p.describe()
+    got = _gpu_residual(ode)
+    ref = O.residual(ode.p.arrays, ode.u0)
+    assert _rel(got, ref) <= RES_TOL, _rel(got, ref)
+
+
 def test_rk_steps_graph_replay_matches_eager_and_oracle(monkeypatch):
     # sse_rk_steps replays one captured step; a new state pointer or a new dt recaptures
     from sse.time_integration import rk_steps, to_device, to_host
