@@ -45,6 +45,15 @@ def build_workload(M: int, p: int):
     return density_wave_3d_config(M=M, p=p, factor=0.05, ic_strength=0.5)
 
 
+def _solver(args, law, sd, form, ode):
+    """The workload's solver; with --deterministic, one whose sums run in a fixed order."""
+    if not getattr(args, "deterministic", False):
+        return ode.p
+    from sse.solvers import MI355X, Solver
+    ode.p.close()
+    return Solver(law, sd, form, parallelism=MI355X(deterministic=True))
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -169,7 +178,7 @@ def bench_adjoint(args):
     M, p, nsteps, every = args.adj_mesh, 3, args.adj_steps, 20
     t0 = time.perf_counter()
     law, sd, ic, form, ode = build_workload(M, p)
-    solver = ode.p
+    solver = _solver(args, law, sd, form, ode)
     rng = np.random.default_rng(20261202)
     u0 = np.asfortranarray(ode.u0 + 1e-3 * rng.standard_normal(ode.u0.shape))
     dt = 0.2 * (2.0 / M) / ((2 * p + 1) * 2.9)
@@ -312,7 +321,7 @@ def bench_multi(args):
     out = {}
     for name, make, ref_s, where in cases:
         law, sd, ic, form, ode = make()
-        solver = ode.p
+        solver = _solver(args, law, sd, form, ode)
         N_p, N_c, N_e = solver.size()
         dof = N_p * N_c * N_e
         u = to_device(ode.u0)
@@ -555,6 +564,8 @@ def main():
                          "tet_p3_M16_ec")
     ap.add_argument("--adj-mesh", type=int, default=12)
     ap.add_argument("--autotune", action="store_true", help="pick the residual's kernel configuration by timing")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="residual, adjoint and multi lines: fixed-order sums (repeated calls bitwise equal)")
     ap.add_argument("--ad-mesh", type=int, default=24)
     ap.add_argument("--ad-members", type=int, default=64, help="diffusivities in the sweep (0: none)")
     ap.add_argument("--ad-steps", type=int, default=10)
@@ -595,7 +606,7 @@ def main():
 
     t0 = time.perf_counter()
     law, sd, ic, form, ode = build_workload(args.mesh, args.p)
-    solver = ode.p
+    solver = _solver(args, law, sd, form, ode)
     N_p, N_c, N_e = solver.size()
     dof = N_p * N_c * N_e
     log(f"prep: {time.perf_counter() - t0:.1f} s, N_e={N_e}, N_p={N_p}, N_c={N_c}, DOF={dof}")
@@ -660,7 +671,7 @@ def main():
                                "ModalTensor p=%d, %d^3 hexes x 6 tets, one residual per step" % (args.p, args.mesh),
                    "N_e": N_e, "p": args.p, "N_p": N_p, "N_q": sd.reference_approximation.N_q,
                    "N_f": sd.reference_approximation.N_f, "N_c": N_c, "DOF": dof,
-                   "kernel_ms": kms, "kernels": solver.describe()},
+                   "kernel_ms": kms, "kernels": solver.describe(), "deterministic": bool(args.deterministic)},
         "roofline": {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": dom, "flops_per_element": wk.flops,

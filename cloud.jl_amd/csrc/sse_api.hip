@@ -747,9 +747,28 @@ int build_struct(sse_handle* h, const sse_desc* D) {
     }
     S.ch = ch;
     S.n_items = n_items;
+    // deterministic mode: the rows of C (each volume node evaluates its own) and the first
+    // chunk of each facet node (chunks were made in facet-node order)
+    std::vector<int> ci_first(N_f + 1, 0);
+    for (int j = 0; j < N_f; ++j) ci_first[j + 1] = ci_first[j] + (int)((ccols[j].size() + ch - 1) / ch);
+    std::vector<std::vector<std::pair<int, double>>> crows(N_q);
+    for (int j = 0; j < N_f; ++j)
+        for (const auto& e : ccols[j]) crows[e.first].push_back({j, e.second});
+    int rw = 1;
+    for (int i = 0; i < N_q; ++i) rw = std::max(rw, (int)crows[i].size());
+    std::vector<int> cr_j((size_t)rw * N_q, -1);
+    std::vector<double> cr_v((size_t)rw * N_q, 0.0);
+    for (int i = 0; i < N_q; ++i)
+        for (size_t w = 0; w < crows[i].size(); ++w) {
+            cr_j[w * N_q + i] = crows[i][w].first;
+            cr_v[w * N_q + i] = crows[i][w].second;
+        }
+    S.rw = rw;
+    S.det = D->deterministic ? 1 : 0;
     int rc;
     if ((rc = upload(h, SG, &S.SG)) || (rc = upload(h, ci_i, &S.ci_i)) || (rc = upload(h, ci_v, &S.ci_v)) ||
-        (rc = upload(h, ci_j, &S.ci_j)))
+        (rc = upload(h, ci_j, &S.ci_j)) || (rc = upload(h, ci_first, &S.ci_first)) || (rc = upload(h, cr_j, &S.cr_j)) ||
+        (rc = upload(h, cr_v, &S.cr_v)))
         return rc;
     double* ext = nullptr;
     if ((rc = alloc(h, (size_t)N_f * D->N_c * N_e, &ext))) return rc;
@@ -977,6 +996,26 @@ int create_handle(const sse_desc* D, int members, sse_handle** out) {
         }
         if (!ij.empty() && ((rc = upload(h, ij, &P.item_ij)) || (rc = upload(h, val, &P.item_val))))
             return cleanup_fail(rc);
+        // deterministic mode: each node's items, in item order (volume nodes, then facet nodes)
+        P.det = D->deterministic ? 1 : 0;
+        if (P.det) {
+            std::vector<std::vector<int>> own((size_t)N_q + N_f);
+            for (int c = 0; c < (int)ij.size(); ++c) {
+                const int i = ij[c] & 0xffff, j = ij[c] >> 16;
+                own[i].push_back(c << 1);
+                if (c < P.n_vol_items)
+                    own[j].push_back(c << 1 | 1);
+                else
+                    own[(size_t)N_q + j].push_back(c << 1);
+            }
+            std::vector<int> optr((size_t)N_q + N_f + 1, 0), oitem;
+            for (size_t n = 0; n < own.size(); ++n) {
+                optr[n + 1] = optr[n] + (int)own[n].size();
+                oitem.insert(oitem.end(), own[n].begin(), own[n].end());
+            }
+            if ((rc = upload(h, optr, &P.own_ptr)) || (!oitem.empty() && (rc = upload(h, oitem, &P.own_item))))
+                return cleanup_fail(rc);
+        }
     }
     if (D->C_kind != SSE_C_NONE) {
         Rows Cr = rows_of(D->C, N_q, N_f), Cc = cols_of(D->C, N_q, N_f);
@@ -1054,8 +1093,8 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
     if (h->use_struct) {
         const sse::TensorParams& T = h->TP;
         const sse::StructTables& S = h->ST;
-        snprintf(buf, len, "struct elem=%d P=%d variant=%d blocks=%d items=%d ch=%d wrt=%d", T.elem, T.P, T.variant,
-                 h->struct_blocks, S.n_items, S.ch, T.wrt);
+        snprintf(buf, len, "struct elem=%d P=%d variant=%d blocks=%d items=%d ch=%d wrt=%d%s", T.elem, T.P, T.variant,
+                 h->struct_blocks, S.n_items, S.ch, T.wrt, S.det ? " deterministic" : "");
     } else if (h->use_tensor) {
         const sse::TensorParams& T = h->TP;
         snprintf(buf, len, "tensor elem=%d P=%d variant=%d pairs=%d vol_pairs=%d passes=%d pass_size=%d wv=%d wf=%d wrt=%d wr=%d",
@@ -1068,7 +1107,8 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
         snprintf(buf, len, "standard d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d mapping=%s%s", P.d, P.N_p, P.N_q, P.N_f,
                  P.N_c, P.N_e, P.skew ? "skew" : "standard", P.second_order ? " br1" : "");
     } else {
-        snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d", P.d, P.N_p, P.N_q, P.N_f, P.N_c, P.N_e);
+        snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d%s", P.d, P.N_p, P.N_q, P.N_f, P.N_c,
+                 P.N_e, P.det ? " deterministic" : "");
     }
     return 0;
 }

@@ -174,7 +174,11 @@ __device__ __forceinline__ double half_nJq(const double* s_L, int N_q, const dou
     return 0.5 * s;
 }
 
-template <int D, int LAW>
+// DET: the deterministic variant. Each node sums its own terms in the order of its
+// owner list (a volume node: its pairs, from either end, and its row of C; a facet
+// node: its interface flux and its column of C), so every pair is evaluated twice and
+// no sum depends on the order of atomics.
+template <int D, int LAW, bool DET>
 __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, double* __restrict__ dudt, RKStage rk) {
     using Ph = Physics<D, LAW>;
     constexpr int NC = Ph::NC;
@@ -194,59 +198,56 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
     for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) s_uq[t] = uqk[t];
     for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) s_uf[t] = ufk[t];
     for (int t = threadIdx.x; t < N_q * D * D; t += blockDim.x) s_L[t] = Lk[t];
-    for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) s_rq[t] = 0.0;
-    for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) s_ff[t] = 0.0;
+    if constexpr (!DET) {
+        for (int t = threadIdx.x; t < N_q * NC; t += blockDim.x) s_rq[t] = 0.0;
+        for (int t = threadIdx.x; t < N_f * NC; t += blockDim.x) s_ff[t] = 0.0;
+    }
     __syncthreads();
     const double* nJfk = p.nJf + (size_t)k * D * N_f;
     const double* Jfk = p.Jf + (size_t)k * N_f;
 
-    // Flux phase over a flat list of work items, spread evenly over the workgroup:
-    //   items 0..N_f-1   the interface flux of facet node j;
-    //   volume pairs     i < j with S_m[i,j] != 0 for some m (the reference's scatter
-    //                    over the upper triangle, flux_differencing_form.jl:37-75):
-    //                    sv = sum_m S_m[i,j] sum_n (Lambda_i + Lambda_j)[m,n] F_n(u_i,u_j)
-    //                    is subtracted from r_q[i] and added to r_q[j] (S antisymmetric,
-    //                    F symmetric);
-    //   facet pairs      C_ij != 0: c_ij nJ(i,j).F(u_i, u_f_j) is subtracted from both
-    //                    r_q[i] and f_f[j] (flux_differencing_form.jl:91-168).
-    // Each flux is evaluated once; the sums go through LDS atomic adds, so their
-    // order (and the last bits of the result) may vary between calls.
-    const int n_items = N_f + p.n_items;
-    for (int t = threadIdx.x; t < n_items; t += blockDim.x) {
-        double F[D * NC];
-        if (t < N_f) {
-            const int j = t;
-            double uin[NC], uout[NC], nrm[D];
-            const double jf = Jfk[j];
-            const int g = p.mapP[j + (size_t)N_f * k];
-            const int kn = g / N_f, jn = g - kn * N_f;
-            const double* ufn = p.uf + (size_t)kn * N_f * NC;
+    // The terms of the flux phase:
+    //   the interface flux of facet node j, B_j J_f (F(u_f, u_f^+) n + LF), enters f_f[j];
+    //   a volume pair i < j with S_m[i,j] != 0 for some m (the reference's scatter over
+    //   the upper triangle, flux_differencing_form.jl:37-75):
+    //   sv = sum_m S_m[i,j] sum_n (Lambda_i + Lambda_j)[m,n] F_n(u_i,u_j) is subtracted
+    //   from r_q[i] and added to r_q[j] (S antisymmetric, F symmetric);
+    //   a facet pair C_ij != 0: c_ij nJ(i,j).F(u_i, u_f_j) is subtracted from both r_q[i]
+    //   and f_f[j] (flux_differencing_form.jl:91-168).
+    auto interface_term = [&](int j, double* out) {
+        double uin[NC], uout[NC], nrm[D], F[D * NC];
+        const double jf = Jfk[j];
+        const int g = p.mapP[j + (size_t)N_f * k];
+        const int kn = g / N_f, jn = g - kn * N_f;
+        const double* ufn = p.uf + (size_t)kn * N_f * NC;
 #pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                uin[e] = s_uf[j + N_f * e];
-                uout[e] = ufn[jn + N_f * e];
-            }
-#pragma unroll
-            for (int m = 0; m < D; ++m) nrm[m] = nJfk[m + D * j] / jf;
-            Ph::two_point(p.law, uin, uout, F);
-            double a = 0.0;
-            const bool lf = (p.law.inviscid_flux == 1);
-            if (lf) a = p.law.half_lambda * Ph::wave_speed(p.law, uin, uout, nrm);
-            const double bj = p.B[j] * jf;
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                double fnv = 0.0;
-#pragma unroll
-                for (int m = 0; m < D; ++m) fnv += F[m * NC + e] * nrm[m];
-                if (lf) fnv += a * (uin[e] - uout[e]);
-                atomicAdd(&s_ff[j + N_f * e], fnv * bj);
-            }
-            continue;
+        for (int e = 0; e < NC; ++e) {
+            uin[e] = s_uf[j + N_f * e];
+            uout[e] = ufn[jn + N_f * e];
         }
-        const int c = t - N_f;
+#pragma unroll
+        for (int m = 0; m < D; ++m) nrm[m] = nJfk[m + D * j] / jf;
+        Ph::two_point(p.law, uin, uout, F);
+        double a = 0.0;
+        const bool lf = (p.law.inviscid_flux == 1);
+        if (lf) a = p.law.half_lambda * Ph::wave_speed(p.law, uin, uout, nrm);
+        const double bj = p.B[j] * jf;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            double fnv = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; ++m) fnv += F[m * NC + e] * nrm[m];
+            if (lf) fnv += a * (uin[e] - uout[e]);
+            out[e] = fnv * bj;
+        }
+    };
+    // item c (volume pairs first, then facet pairs): its nodes and its term (sv of a
+    // volume pair, c_ij nJ.F of a facet pair)
+    auto item_term = [&](int c, int& i, int& j, double* out) {
         const int ij = p.item_ij[c];
-        const int i = ij & 0xffff, j = ij >> 16;
-        double ui[NC], uj[NC];
+        i = ij & 0xffff;
+        j = ij >> 16;
+        double ui[NC], uj[NC], F[D * NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) ui[e] = s_uq[i + N_q * e];
         if (c < p.n_vol_items) {
@@ -267,8 +268,7 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
                 double sv = 0.0;
 #pragma unroll
                 for (int n = 0; n < D; ++n) sv += an[n] * F[n * NC + e];
-                atomicAdd(&s_rq[i + N_q * e], -sv);
-                atomicAdd(&s_rq[j + N_q * e], sv);
+                out[e] = sv;
             }
         } else {
             const double cij = p.item_val[(size_t)c * D];
@@ -284,9 +284,63 @@ __global__ void __launch_bounds__(256) k_time_derivative_generic(DevParams p, do
                 double fnv = 0.0;
 #pragma unroll
                 for (int m = 0; m < D; ++m) fnv += nJ[m] * F[m * NC + e];
-                fnv *= cij;
-                atomicAdd(&s_rq[i + N_q * e], -fnv);
-                atomicAdd(&s_ff[j + N_f * e], -fnv);
+                out[e] = fnv * cij;
+            }
+        }
+    };
+    if constexpr (!DET) {
+        // one flat list of work items (interface fluxes, then the items), spread evenly
+        // over the workgroup; each term is evaluated once and added to both ends with
+        // LDS atomic adds, so the order of the sums (and the last bits of the result)
+        // may vary between calls
+        const int n_items = N_f + p.n_items;
+        for (int t = threadIdx.x; t < n_items; t += blockDim.x) {
+            double v[NC];
+            if (t < N_f) {
+                interface_term(t, v);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) atomicAdd(&s_ff[t + N_f * e], v[e]);
+                continue;
+            }
+            const int c = t - N_f;
+            int i, j;
+            item_term(c, i, j, v);
+            if (c < p.n_vol_items) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    atomicAdd(&s_rq[i + N_q * e], -v[e]);
+                    atomicAdd(&s_rq[j + N_q * e], v[e]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    atomicAdd(&s_rq[i + N_q * e], -v[e]);
+                    atomicAdd(&s_ff[j + N_f * e], -v[e]);
+                }
+            }
+        }
+    } else {
+        // node nd < N_q is volume node nd, else facet node nd - N_q; own_item[w] = c << 1 | end,
+        // end = 1 when the node is the second node j of a volume pair
+        for (int nd = threadIdx.x; nd < N_q + N_f; nd += blockDim.x) {
+            double acc[NC], v[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) acc[e] = 0.0;
+            if (nd >= N_q) interface_term(nd - N_q, acc);
+            for (int w = p.own_ptr[nd]; w < p.own_ptr[nd + 1]; ++w) {
+                const int c = p.own_item[w] >> 1;
+                int i, j;
+                item_term(c, i, j, v);
+                const bool plus = c < p.n_vol_items && (p.own_item[w] & 1);
+#pragma unroll
+                for (int e = 0; e < NC; ++e) acc[e] += plus ? v[e] : -v[e];
+            }
+            if (nd < N_q) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s_rq[nd + N_q * e] = acc[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) s_ff[nd - N_q + N_f * e] = acc[e];
             }
         }
     }
@@ -365,8 +419,12 @@ hipError_t launch_generic(const DevParams& p, const double* u, double* dudt, con
         if (err != hipSuccess) return err;
     }
     if (which & 2) {
-        hipLaunchKernelGGL((k_time_derivative_generic<D, LAW>), grid, block, lds_time_derivative_generic(p), s, p,
-                           dudt, rk);
+        if (p.det)
+            hipLaunchKernelGGL((k_time_derivative_generic<D, LAW, true>), grid, block, lds_time_derivative_generic(p),
+                               s, p, dudt, rk);
+        else
+            hipLaunchKernelGGL((k_time_derivative_generic<D, LAW, false>), grid, block, lds_time_derivative_generic(p),
+                               s, p, dudt, rk);
         return hipGetLastError();
     }
     return hipSuccess;

@@ -38,6 +38,10 @@ struct DevParams {
     // item_ij = i | j << 16
     int n_items, n_vol_items;
     const int* item_ij; const double* item_val;
+    // deterministic mode (det = 1): the items of each node, volume nodes then facet nodes,
+    // own_item[w] = c << 1 | (1 if the node is the second node of volume pair c)
+    int det;
+    const int* own_ptr; const int* own_item;
     // StandardForm: union sparsity of D_1..D_d by rows (node i -> (node j, D_m[i,j] for m < d))
     // and of D_1^T..D_d^T (node i -> (node j, D_m[j,i] for m < d))
     const int* D_ptr; const int* D_idx; const double* D_val;

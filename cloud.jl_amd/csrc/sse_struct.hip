@@ -69,7 +69,12 @@ struct StructLayout {
 };
 
 
-template <int ELEM, int P>
+// DET: the deterministic variant. The partner shares of the volume pairs are added in a
+// fixed order (the pairs of one offset at a time, one half-team after the other: the
+// targets of one offset are distinct), each volume node evaluates its own row of C, and
+// the column sums of C go through per-chunk slots summed in chunk order. The C pairs are
+// then evaluated twice (row and column), and no sum depends on the order of atomics.
+template <int ELEM, int P, bool DET>
 __global__ void __launch_bounds__((StructTune<ELEM, P>::TS * StructTune<ELEM, P>::EPB), (StructTune<ELEM, P>::WAVES))
 k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ dudt, RKStage rk, int n_teams) {
     using Sh = Shape<ELEM, P>;
@@ -139,7 +144,10 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         double* srow = sst;            // NC x NQ: C-row sums of the volume nodes
         double* sfac = sst + NC * NQ;  // NC x NF: C-column sums of the facet nodes
         double* sfn = sm + L::off_fn;
-        for (int t = tt; t < NC * (NQ + NF); t += TS) sst[t] = 0.0;
+        // the row accumulators start at zero; each facet node's column accumulator starts at
+        // minus its interface flux (written below by the node's thread), so that phase D reads
+        // f_f - (column sums of C) from one place and the flux needs no register meanwhile
+        for (int t = tt; t < NC * NQ; t += TS) sst[t] = 0.0;
         if (tt < D * Sh::NFAC) sfn[tt] = p.fn[tt];
         double hi[NH], Li[D * D], r[NC], ff[NC], Ji = 1.0;
 #pragma unroll
@@ -176,6 +184,10 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             } else {
 #pragma unroll
                 for (int e = 0; e < NC; ++e) ff[e] = c[e] * bj;
+            }
+            if constexpr (!DET) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) sfac[e * NF + tl] = -ff[e];
             }
         }
         __syncthreads();
@@ -239,15 +251,93 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                     atomicAdd(&srow[e * NQ + j], -c[e]);
                 }
             };
-            double sa[D], sb[D];
-            if (q0 < q1) load_s(q0, sa);
+            if constexpr (!DET) {
+                double sa[D], sb[D];
+                if (q0 < q1) load_s(q0, sa);
 #pragma unroll 1
-            for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
-                if (q + 1 < q1) load_s(q + 1, sb);
-                eval(q, sa);
-                if (q + 1 < q1) {
-                    if (q + 2 < q1) load_s(q + 2, sa);
-                    eval(q + 1, sb);
+                for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
+                    if (q + 1 < q1) load_s(q + 1, sb);
+                    eval(q, sa);
+                    if (q + 1 < q1) {
+                        if (q + 2 < q1) load_s(q + 2, sa);
+                        eval(q + 1, sb);
+                    }
+                }
+            }
+        }
+        if constexpr (DET) {
+            // the same pairs, every thread through QH trips: at each trip the half-teams
+            // add their partner shares one after the other (plain read-add-write: within a
+            // half-team, the partners of one offset are distinct nodes)
+#pragma unroll 1
+            for (int t = 0; t < QH; ++t) {
+                const int q = half * QH + t;
+                const bool has = tl < NQ && q < NPART && q < (half + 1) * QH;
+                double c[NC];
+                int jd = 0;
+                bool act = false;
+                if (has) {
+                    const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
+                    const int f = q / PER_F, dlt = q - f * PER_F + 1;
+                    const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
+                    const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
+                    act = !(EVEN && 2 * dlt == N1 && 2 * af >= N1);
+                    const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+                    jd = tl + (b - af) * stride;
+                    if (act) {
+                        double hj[NH], av[D];
+#pragma unroll
+                        for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + jd];
+#pragma unroll
+                        for (int n = 0; n < D; ++n) av[n] = 0.0;
+#pragma unroll
+                        for (int m = 0; m < D; ++m) {
+                            if (m < f) continue;
+                            const double sv = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+#pragma unroll
+                            for (int n = 0; n < D; ++n) av[n] += sv * (Li[m + D * n] + sL[(m + D * n) * NQ + jd]);
+                        }
+                        ec_dot<D>(p.law, hi, hj, av, c);
+#pragma unroll
+                        for (int e = 0; e < NC; ++e) r[e] -= c[e];
+                    }
+                }
+#pragma unroll 1
+                for (int h = 0; h < SPLIT; ++h) {
+                    if (has && act && half == h) {
+#pragma unroll
+                        for (int e = 0; e < NC; ++e) srow[e * NQ + jd] -= c[e];
+                    }
+                    __syncthreads();
+                }
+            }
+            // the partner shares into the node's row; the accumulator area is then free
+            if (half == 0 && tl < NQ) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) r[e] -= srow[e * NQ + tl];
+            }
+            __syncthreads();
+            // the node's own row of C (entries split between the half-teams)
+            if (tl < NQ) {
+#pragma unroll 1
+                for (int w = half; w < st.rw; w += SPLIT) {
+                    const int j = st.cr_j[w * NQ + tl];
+                    if (j < 0) break;
+                    const double cv = st.cr_v[w * NQ + tl];
+                    const int fc = j / NPF;
+                    double hj[NH], av[D], c[NC];
+#pragma unroll
+                    for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+#pragma unroll
+                    for (int n = 0; n < D; ++n) {
+                        double s = snJ[n * NF + j];
+#pragma unroll
+                        for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + tl] * sfn[l + D * fc];
+                        av[n] = 0.5 * cv * s;
+                    }
+                    ec_dot<D>(p.law, hi, hj, av, c);
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) r[e] -= c[e];
                 }
             }
         }
@@ -259,12 +349,8 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             // the chunk (facet node, N1 volume nodes and C entries) requested at once
             const int j = st.ci_j[item];
             int ci[N1];
-            double cv[N1];
 #pragma unroll
-            for (int w = 0; w < N1; ++w) {
-                ci[w] = st.ci_i[w * MAXITEM + item];
-                cv[w] = st.ci_v[w * MAXITEM + item];
-            }
+            for (int w = 0; w < N1; ++w) ci[w] = st.ci_i[w * MAXITEM + item];
             const int fc = j / NPF;
             double hj[NH], fnl[D], nj[D], part[NC];
 #pragma unroll
@@ -279,6 +365,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             for (int w = 0; w < N1; ++w) {
                 const int i = ci[w];
                 if (i < 0) continue;
+                const double cvw = st.ci_v[w * MAXITEM + item];
                 double hv[NH], av[D], c[NC];
 #pragma unroll
                 for (int v = 0; v < NH; ++v) hv[v] = sh[v * NQ + i];
@@ -287,17 +374,22 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                     double s = nj[n];
 #pragma unroll
                     for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * fnl[l];
-                    av[n] = 0.5 * cv[w] * s;
+                    av[n] = 0.5 * cvw * s;
                 }
                 ec_dot<D>(p.law, hv, hj, av, c);
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
                     part[e] += c[e];
-                    atomicAdd(&srow[e * NQ + i], c[e]);
+                    if constexpr (!DET) atomicAdd(&srow[e * NQ + i], c[e]);
                 }
             }
+            if constexpr (DET) {
 #pragma unroll
-            for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
+                for (int e = 0; e < NC; ++e) sst[e * MAXITEM + item] = part[e];  // the chunk's slot
+            } else {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
+            }
         }
         __syncthreads();
         // ---- phase D: the C sums into r (volume nodes) and f_f (facet nodes); f_f to LDS
@@ -307,13 +399,26 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
 #pragma unroll
             for (int e = 0; e < NC; ++e) srb[e * NQ + tl] = r[e];
         }
-        if (half == 0 && tl < NQ) {  // C-row sums, read before r_q may overwrite them
+        if constexpr (!DET) {
+            if (half == 0 && tl < NQ) {  // C-row sums, read before r_q may overwrite them
 #pragma unroll
-            for (int e = 0; e < NC; ++e) r[e] -= srow[e * NQ + tl];
-        }
-        if (half == 0 && tl < NF) {
+                for (int e = 0; e < NC; ++e) r[e] -= srow[e * NQ + tl];
+            }
+            if (half == 0 && tl < NF) {
 #pragma unroll
-            for (int e = 0; e < NC; ++e) sff[e * NF + tl] = ff[e] - sfac[e * NF + tl];
+                for (int e = 0; e < NC; ++e) sff[e * NF + tl] = -sfac[e * NF + tl];
+            }
+        } else if (half == 0 && tl < NF) {  // the node's chunk slots, in chunk order
+            double col[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) col[e] = 0.0;
+            const int c0 = st.ci_first[tl], c1 = st.ci_first[tl + 1];
+            for (int c = c0; c < c1; ++c) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) col[e] += sst[e * MAXITEM + c];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sff[e * NF + tl] = ff[e] - col[e];
         }
         __syncthreads();
         // ---- phase E: lift r_q -= R' f_f
@@ -383,21 +488,28 @@ StructConfig struct_config_of() {
                         StructLayout<ELEM, P>::MAXITEM};
 }
 
-template <int ELEM, int P>
-hipError_t launch_struct_t(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
+template <int ELEM, int P, bool DET>
+hipError_t launch_struct_v(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
                            hipStream_t s, int n_blocks) {
     using T = StructTune<ELEM, P>;
     const StructConfig c = struct_config_of<ELEM, P>();
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_time_derivative_struct<ELEM, P>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_time_derivative_struct<ELEM, P, DET>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_time_derivative_struct<ELEM, P>), dim3(n_blocks), dim3(T::TS * T::EPB), c.lds, s, tp, st,
-                       dudt, rk, n_blocks * T::EPB);
+    hipLaunchKernelGGL((k_time_derivative_struct<ELEM, P, DET>), dim3(n_blocks), dim3(T::TS * T::EPB), c.lds, s, tp,
+                       st, dudt, rk, n_blocks * T::EPB);
     return hipGetLastError();
+}
+
+template <int ELEM, int P>
+hipError_t launch_struct_t(const TensorParams& tp, const StructTables& st, double* dudt, const RKStage& rk,
+                           hipStream_t s, int n_blocks) {
+    return st.det ? launch_struct_v<ELEM, P, true>(tp, st, dudt, rk, s, n_blocks)
+                  : launch_struct_v<ELEM, P, false>(tp, st, dudt, rk, s, n_blocks);
 }
 
 #define SSE_STRUCT_CASES                                                                       \
@@ -408,10 +520,11 @@ template <int ELEM, int P>
 int struct_blocks_per_cu() {
     using T = StructTune<ELEM, P>;
     const StructConfig c = struct_config_of<ELEM, P>();
-    (void)hipFuncSetAttribute((const void*)k_time_derivative_struct<ELEM, P>,
+    (void)hipFuncSetAttribute((const void*)k_time_derivative_struct<ELEM, P, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_time_derivative_struct<ELEM, P>, T::TS * T::EPB, c.lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_time_derivative_struct<ELEM, P, false>, T::TS * T::EPB,
+                                                     c.lds) !=
         hipSuccess)
         return 0;
     return nb;

@@ -70,6 +70,13 @@ struct StructTables {
     const int* ci_j;     // MAXITEM: facet node of each chunk (-1 = no chunk)
     int ch, n_items;
     const double* uf_ext;  // exterior facet states [k][e][j]
+    // deterministic mode: the rows of C (RW x NQ, facet node and entry, -1 pad) and the
+    // first chunk of each facet node (NF + 1 prefix offsets into the chunks)
+    const int* cr_j;
+    const double* cr_v;
+    const int* ci_first;
+    int rw;
+    int det;  // 1: every sum in a fixed order (no atomic accumulation)
 };
 
 // q index of (b1, b2), b1 + b2 <= P, enumerated with b1 outer: also the mode

@@ -49,8 +49,10 @@ struct VjpLayout {
     static constexpr int off_nJ = off_hf + NH * Sh::NF;      // D x NF
     static constexpr int off_rb = off_nJ + D * Sh::NF;       // NC x NQ rbar
     static constexpr int off_fb = off_rb + NC * Sh::NQ;      // NC x NF fbar
-    static constexpr int off_st = off_fb + NC * Sh::NF;      // NC x (NQ + NF) C-row and C-column sums
-    static constexpr int off_ub = off_st + NC * (Sh::NQ + Sh::NF);  // NC x NQ split partial rows, then w_q' cotangent
+    static constexpr int off_st = off_fb + NC * Sh::NF;      // NC x (NQ + NF) C-row and C-column sums,
+                                                              // or NC x MAXITEM chunk slots (deterministic)
+    static constexpr int ST = NC * (Sh::NQ + Sh::NF) > NC * MAXITEM ? NC * (Sh::NQ + Sh::NF) : NC * MAXITEM;
+    static constexpr int off_ub = off_st + ST;               // NC x NQ split partial rows, then w_q' cotangent
     static constexpr int off_wf = off_ub + NC * Sh::NQ;      // NC x NF w_f cotangent
     static constexpr int off_red = off_wf + NC * Sh::NF;     // TS reduction buffer
     static constexpr int off_fn = off_red + T::TS;           // D x NFAC reference face normals
@@ -152,7 +154,10 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
 // V2: the two-point terms and the entropy variables, one element per workgroup. It
 // leaves the cotangent of the projected volume states w_q' in the element's slice of
 // rbar (read into LDS at its start) for V3.
-template <int ELEM, int P>
+// DET: every sum in a fixed order, as the deterministic time derivative does: partner
+// shares of the volume pairs added one offset and one half-team at a time, each volume node
+// pulling back its own row of C, and the facet columns of C through per-chunk slots.
+template <int ELEM, int P, bool DET>
 __global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
 k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, const double* __restrict__ fbar,
                 const double* __restrict__ fbar_ext, double* __restrict__ lam_partial, VjpOut out) {
@@ -313,15 +318,98 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
 #pragma unroll
             for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
         };
-        double sa[D], sb[D];
-        if (q0 < q1) load_s(q0, sa);
+        if constexpr (!DET) {
+            double sa[D], sb[D];
+            if (q0 < q1) load_s(q0, sa);
 #pragma unroll 1
-        for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
-            if (q + 1 < q1) load_s(q + 1, sb);
-            eval(q, sa);
-            if (q + 1 < q1) {
-                if (q + 2 < q1) load_s(q + 2, sa);
-                eval(q + 1, sb);
+            for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
+                if (q + 1 < q1) load_s(q + 1, sb);
+                eval(q, sa);
+                if (q + 1 < q1) {
+                    if (q + 2 < q1) load_s(q + 2, sa);
+                    eval(q + 1, sb);
+                }
+            }
+        }
+    }
+    if constexpr (DET) {
+        // the same pairs, every thread through QH trips; the half-teams add their partner
+        // shares one after the other (the partners of one offset are distinct nodes)
+#pragma unroll 1
+        for (int t = 0; t < QH; ++t) {
+            const int q = half * QH + t;
+            const bool has = tl < NQ && q < NPART && q < (half + 1) * QH;
+            double ubj[NC];
+            int jd = 0;
+            bool act = false;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ubj[e] = 0.0;
+            if (has) {
+                const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
+                const int f = q / PER_F, dlt = q - f * PER_F + 1;
+                const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
+                const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
+                act = !(EVEN && 2 * dlt == N1 && 2 * af >= N1);
+                const int b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+                jd = tl + (b - af) * stride;
+                if (act) {
+                    double hj[NH], av[D], gb[NC];
+#pragma unroll
+                    for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + jd];
+#pragma unroll
+                    for (int n = 0; n < D; ++n) av[n] = 0.0;
+#pragma unroll
+                    for (int m = 0; m < D; ++m) {
+                        if (m < f) continue;
+                        const double sv = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+#pragma unroll
+                        for (int n = 0; n < D; ++n)
+                            av[n] += sv * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + jd]);
+                    }
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) gb[e] = srb[e * NQ + jd] - srb[e * NQ + tl];
+                    A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+                }
+            }
+#pragma unroll 1
+            for (int h = 0; h < SPLIT; ++h) {
+                if (has && act && half == h) {
+#pragma unroll
+                    for (int e = 0; e < NC; ++e) srow[e * NQ + jd] += ubj[e];
+                }
+                __syncthreads();
+            }
+        }
+        // the partner shares into the node's row; the accumulator area is then free
+        if (half == 0 && tl < NQ) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ubq[e] += srow[e * NQ + tl];
+        }
+        __syncthreads();
+        // the node's own row of C (entries split between the half-teams)
+        if (tl < NQ) {
+#pragma unroll 1
+            for (int w = half; w < st.rw; w += SPLIT) {
+                const int j = st.cr_j[w * NQ + tl];
+                if (j < 0) break;
+                const double cv = st.cr_v[w * NQ + tl];
+                const int fc = j / NPF;
+                double hj[NH], av[D], gb[NC], dummy[NC];
+#pragma unroll
+                for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
+#pragma unroll
+                for (int n = 0; n < D; ++n) {
+                    double s = snJ[n * NF + j];
+#pragma unroll
+                    for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + tl] * sfn[l + D * fc];
+                    av[n] = 0.5 * cv * s;
+                }
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    gb[e] = -(srb[e * NQ + tl] + sfb[e * NF + j]);
+                    dummy[e] = 0.0;
+                }
+                A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, dummy);
             }
         }
     }
@@ -367,11 +455,18 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
                 uv[e] = 0.0;
             }
             A::ec_dot_adj_h2(p.law, hv, hj, av, gb, uv, part);
+            if constexpr (!DET) {
 #pragma unroll
-            for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + i], uv[e]);
+                for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + i], uv[e]);
+            }
         }
+        if constexpr (DET) {
 #pragma unroll
-        for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
+            for (int e = 0; e < NC; ++e) srow[e * MAXITEM + item] = part[e];  // the chunk's slot
+        } else {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
+        }
     }
     if (SPLIT > 1 && half == 1 && tl < NQ) {
 #pragma unroll
@@ -384,13 +479,27 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
 #pragma unroll
             for (int e = 0; e < NC; ++e) ubq[e] += sub[e * NQ + tl];
         }
+        if constexpr (!DET) {
 #pragma unroll
-        for (int e = 0; e < NC; ++e) ubq[e] += srow[e * NQ + tl];
+            for (int e = 0; e < NC; ++e) ubq[e] += srow[e * NQ + tl];
+        }
     }
     if (half == 0 && tl < NF) {
         double ubf[NC], hf[NH];
+        if constexpr (DET) {  // the node's chunk slots, in chunk order
 #pragma unroll
-        for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl] + sfac[e * NF + tl];
+            for (int e = 0; e < NC; ++e) ubf[e] = 0.0;
+            const int c0 = st.ci_first[tl], c1 = st.ci_first[tl + 1];
+            for (int c = c0; c < c1; ++c) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) ubf[e] += srow[e * MAXITEM + c];
+            }
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ubf[e] += swf[e * NF + tl];
+        } else {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl] + sfac[e * NF + tl];
+        }
 #pragma unroll
         for (int v = 0; v < NH; ++v) hf[v] = shf[v * NF + tl];
         double wb[NC];
@@ -517,8 +626,11 @@ hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, c
         hipError_t e = hipFuncSetAttribute((const void*)k_vjp_v1_tensor<ELEM, P>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_vjp_v2_tensor<ELEM, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)l2);
+            e = hipFuncSetAttribute((const void*)k_vjp_v2_tensor<ELEM, P, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_vjp_v2_tensor<ELEM, P, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_vjp_v3_tensor<ELEM, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)l3);
@@ -529,8 +641,12 @@ hipError_t launch_vjp_tensor_t(const TensorParams& tp, const StructTables& st, c
     hipLaunchKernelGGL((k_vjp_v1_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l1, s, tp, ybar, rbar, fbar, fbar_ext, in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l2, s, tp, st, rbar, fbar, fbar_ext,
-                       lam_partial, VjpOut{scale, accumulate});
+    if (st.det)
+        hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P, true>), dim3(nb), dim3(T::TS), l2, s, tp, st, rbar, fbar,
+                           fbar_ext, lam_partial, VjpOut{scale, accumulate});
+    else
+        hipLaunchKernelGGL((k_vjp_v2_tensor<ELEM, P, false>), dim3(nb), dim3(T::TS), l2, s, tp, st, rbar, fbar,
+                           fbar_ext, lam_partial, VjpOut{scale, accumulate});
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_vjp_v3_tensor<ELEM, P>), dim3(nb), dim3(T::TS), l3, s, tp, u, rbar, ubar,

@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-SSE_ABI_VERSION = 2
+SSE_ABI_VERSION = 3
 
 SSE_OK = 0
 SSE_ERR_INVALID_ARGUMENT = 1
@@ -43,7 +43,7 @@ class SseDesc(C.Structure):
         ("C", _dp), ("Minv", _dp), ("D", _dp),
         ("V_A", _dp), ("V_B", _dp), ("V_C", _dp), ("V_p", C.c_int32), ("V_M", C.c_int32 * 3),
         ("J_q", _dp), ("Lambda_q", _dp), ("nJf", _dp), ("J_f", _dp), ("nJq", _dp),
-        ("face_normals", _dp), ("mapP", _lp),
+        ("face_normals", _dp), ("mapP", _lp), ("deterministic", C.c_int32),
     ]
 
 

@@ -64,7 +64,14 @@ class PhysicalOperator:
 
 
 class MI355X:
-    """Parallelism: the residual runs as HIP kernels on one MI355X (gfx950)."""
+    """Parallelism: the residual runs as HIP kernels on one MI355X (gfx950).
+
+    ``deterministic=True`` makes every sum of the line-structured kernels (ModalTensor
+    Tri/Tet Euler: time derivative and VJP) run in a fixed order, so repeated residuals
+    are bitwise equal; the default accumulates some sums with LDS atomics (faster)."""
+
+    def __init__(self, deterministic: bool = False):
+        self.deterministic = bool(deterministic)
 
 
 @dataclass
@@ -249,6 +256,7 @@ class Solver:
         self.parallelism = parallelism or MI355X()
         self.arrays = DescArrays(conservation_law, spatial_discretization, form, self.mass_solver,
                                  self.strategy)
+        self.arrays.desc.deterministic = int(getattr(self.parallelism, "deterministic", False))
         self._handle = None
         self._lib = None
 

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define SSE_ABI_VERSION 2
+#define SSE_ABI_VERSION 3
 
 typedef enum {
     SSE_OK = 0,
@@ -135,6 +135,13 @@ typedef struct sse_desc {
     const double* nJq;       /* d x N_fac x N_q x N_e; NULL = derived from Lambda_q */
     const double* face_normals; /* d x N_fac reference normals nrstJ of each face */
     const int64_t* mapP;     /* N_f x N_e, 1-based linear index into (N_f, N_e) */
+
+    /* 1: every sum of the line-structured kernels (ModalTensor Tri/Tet Euler: the time
+     * derivative and the VJP) in a fixed order, so that repeated calls are bitwise equal; the
+     * default (0) accumulates the partner shares and the facet-correction sums with LDS
+     * atomics (faster, equal to the last bits only). No counterpart in the reference,
+     * whose threaded loops have a fixed order. */
+    int32_t deterministic;
 } sse_desc;
 
 typedef struct sse_handle sse_handle;

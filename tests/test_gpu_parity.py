@@ -438,6 +438,86 @@ def test_non_antisymmetric_s_leaves_the_pair_once_kernels():
     assert _rel(got, ref) <= RES_TOL, _rel(got, ref)
 
 
+def _modal_multi():
+    from sse.reference_approximation import ModalMulti
+    return ModalMulti
+
+
+def _euler_3d_hex(**kw):
+    from sse.workloads import euler_3d_hex
+    return euler_3d_hex(**kw)
+
+
+def _deterministic(make):
+    from sse.solvers import MI355X, Solver
+    law, sd, ic, form, ode = make()[:5]
+    solver = Solver(law, sd, form, parallelism=MI355X(deterministic=True))
+    assert solver.describe().endswith("deterministic"), solver.describe()
+    return law, sd, ode, solver
+
+
+DET_CASES = {
+    "tet_p4": lambda: density_wave_3d_config(M=2, p=4),
+    "tet_p3_even_lines": lambda: density_wave_3d_config(M=2, p=3, ic_strength=0.2),
+    "tri_p4": lambda: vortex_2d_config(M=4, p=4),
+    "tet_p5": lambda: density_wave_3d_config(M=1, p=5, ic_strength=0.2),
+    # the general kernel (owner lists in place of its LDS atomics)
+    "tet_p3_multi": lambda: density_wave_3d_config(M=2, p=3, ic_strength=0.2, approx=_modal_multi()),
+    "euler1d_gauss_p5": euler_1d_gauss,
+    "hex_p3": lambda: _euler_3d_hex(M=2, p=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DET_CASES))
+def test_deterministic_residuals_are_bitwise_equal_and_match_oracle(name):
+    # the deterministic mode of the line-structured kernels: repeated residuals are equal
+    # bit for bit (the default mode differs in the last bits), and equal the oracle
+    torch = _gpu()
+    from sse import semi_discrete_residual
+    from sse.time_integration import to_device, to_host
+    law, sd, ode, solver = _deterministic(DET_CASES[name])
+    u = to_device(_perturbed(ode, amp=1e-3))
+    outs = []
+    for _ in range(4):
+        d = torch.empty_like(u)
+        semi_discrete_residual(d, u, solver)
+        outs.append(d)
+    torch.cuda.synchronize()
+    for d in outs[1:]:
+        assert torch.equal(d, outs[0])
+    ref = O.residual(solver.arrays, to_host(u))
+    assert _rel(to_host(outs[0]), ref) <= RES_TOL, _rel(to_host(outs[0]), ref)
+
+
+@pytest.mark.parametrize("name", ["tet_p4", "tet_p3_even_lines", "tri_p4", "tet_p3_multi"])
+def test_deterministic_vjp_is_bitwise_repeatable_and_passes_the_dot_product_test(name):
+    from sse.adjoint import residual_vjp
+    law, sd, ode, solver = _deterministic(DET_CASES[name])
+    u = _perturbed(ode, amp=1e-3)
+    rng = np.random.default_rng(41)
+    ybar = np.asfortranarray(rng.standard_normal(u.shape))
+    du = np.asfortranarray(rng.standard_normal(u.shape))
+    ub1, lam1 = residual_vjp(solver, u, ybar)
+    ub2, lam2 = residual_vjp(solver, u, ybar)
+    assert np.array_equal(ub1, ub2) and lam1 == lam2
+    dlam = 0.7
+    r, dr = O.residual_jvp(solver.arrays, u, du, dlam)
+    lhs = float(np.sum(ybar * dr))
+    rhs = float(np.sum(ub1 * du)) + lam1 * dlam
+    assert abs(lhs - rhs) <= 1e-11 * float(np.sum(np.abs(ybar * dr))), (lhs, rhs)
+
+
+def test_deterministic_rk_adjoint_is_bitwise_repeatable():
+    # with every sum ordered, the checkpointed adjoint's recomputed stage states equal the
+    # forward sweep's, and two gradients agree bit for bit
+    from sse.adjoint import rk_adjoint
+    law, sd, ode, solver = _deterministic(DET_CASES["tet_p3_even_lines"])
+    u0 = _perturbed(ode, amp=1e-3)
+    J1, g1, l1 = rk_adjoint(solver, u0, 2e-3, 8, checkpoint_every=3)
+    J2, g2, l2 = rk_adjoint(solver, u0, 2e-3, 8, checkpoint_every=3)
+    assert J1 == J2 and l1 == l2 and np.array_equal(g1, g2)
+
+
 def test_rk_steps_graph_replay_matches_eager_and_oracle(monkeypatch):
     # sse_rk_steps replays one captured step; a new state pointer or a new dt recaptures
     from sse.time_integration import rk_steps, to_device, to_host
