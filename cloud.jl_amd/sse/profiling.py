@@ -12,12 +12,14 @@ Two summaries, both plain text or JSON so they can be kept beside the code:
   whatever fraction of the memory channels the counter samples.
 
 * ``counters``: every counter of one or more ``rocprofv3 --pmc`` runs, mean per
-  dispatch, for each kernel slot.
+  dispatch, for each kernel slot; with ``--elements`` and ``--trace``, the derived
+  figures below the table (instructions per element, the VALU issue floor against
+  the kernel's mean duration, the share of wave time spent waiting).
 
 Usage::
 
     python -m sse.profiling kernel-stats <trace_dir>
-    python -m sse.profiling counters <pmc_dir> [<pmc_dir> ...]
+    python -m sse.profiling counters <pmc_dir> [<pmc_dir> ...] [--elements N --trace <trace_dir>]
     python -m sse.profiling traffic <fetch_dir> <write_dir> --copy-mib 256 \
         --workload NAME [--json profiles/hbm_traffic.json]
 """
@@ -133,6 +135,76 @@ def format_counters(data) -> str:
     return "\n".join(lines)
 
 
+# MI355X: 256 CUs of 4 SIMDs, 2.4 GHz peak engine clock. A wave64 VALU instruction
+# occupies its SIMD's 16 lanes for 4 cycles (FP64 at full rate); FP64 transcendental
+# instructions (v_rcp_f64, v_sqrt_f64, ...) run at a quarter of that rate.
+CUS, SIMDS_PER_CU, CLOCK_GHZ = 256, 4, 2.4
+
+
+def derived(data, elements: int, mean_us=None):
+    """{slot: {figure: value}} from the counter means of `data` (see ``all_counters``):
+    instructions per element, the FP64 lane flops the VALU executed, the VALU issue
+    floor (every SIMD issuing a VALU instruction every 4 cycles, 16 for FP64
+    transcendentals) and its share of the mean duration, and the shares of wave time
+    spent issuing VALU instructions and waiting."""
+    out = {}
+    for slot, c in data.items():
+        if slot == "copy_probe":
+            continue
+        g = c.get
+        d = {}
+        if g("SQ_WAVES"):
+            d["waves"] = g("SQ_WAVES")
+        if g("SQ_INSTS_VALU") is not None:
+            d["valu_insts_per_element"] = g("SQ_INSTS_VALU") / elements
+            if g("SQ_WAVES"):
+                d["valu_insts_per_wave"] = g("SQ_INSTS_VALU") / g("SQ_WAVES")
+        for k in ("FMA_F64", "ADD_F64", "MUL_F64", "TRANS_F64", "INT32"):
+            if g(f"SQ_INSTS_VALU_{k}") is not None:
+                d[f"valu_{k.lower()}_per_element"] = g(f"SQ_INSTS_VALU_{k}") / elements
+        if all(g(f"SQ_INSTS_VALU_{k}") is not None for k in ("FMA_F64", "ADD_F64", "MUL_F64")):
+            d["fp64_lane_flops_per_element"] = 64 * (2 * g("SQ_INSTS_VALU_FMA_F64") + g("SQ_INSTS_VALU_ADD_F64")
+                                                     + g("SQ_INSTS_VALU_MUL_F64")) / elements
+        for k in ("SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR"):
+            if g(k) is not None:
+                d[k[3:].lower() + "_per_element"] = g(k) / elements
+        if g("SQ_INSTS_VALU") is not None:
+            cycles = 4 * (g("SQ_INSTS_VALU") + 3 * (g("SQ_INSTS_VALU_TRANS_F64") or 0.0))
+            floor_us = cycles / (CUS * SIMDS_PER_CU) / (CLOCK_GHZ * 1e3)
+            d["valu_issue_floor_us"] = floor_us
+            if mean_us and slot in mean_us:
+                d["mean_us"] = mean_us[slot]
+                d["valu_issue_floor_frac"] = floor_us / mean_us[slot]
+        if g("SQ_WAVE_CYCLES"):
+            if g("SQ_WAIT_ANY") is not None:
+                d["wave_time_waiting"] = g("SQ_WAIT_ANY") / g("SQ_WAVE_CYCLES")
+            if g("SQ_ACTIVE_INST_VALU") is not None:
+                d["wave_time_issuing_valu"] = g("SQ_ACTIVE_INST_VALU") / g("SQ_WAVE_CYCLES")
+        if g("SQ_INSTS_LDS") and g("SQ_LDS_BANK_CONFLICT") is not None:
+            d["lds_bank_conflict_cycles_per_lds_inst"] = g("SQ_LDS_BANK_CONFLICT") / g("SQ_INSTS_LDS")
+        out[slot] = d
+    return out
+
+
+def slot_means(trace_dir: str):
+    """{slot: mean duration in us} from a kernel-trace run (the slowest kernel of each slot)."""
+    out = {}
+    for k, st in kernel_stats(trace_dir).items():
+        s = slot_of(k)
+        if s is not None:
+            out[s] = max(out.get(s, 0.0), st["mean_us"])
+    return out
+
+
+def format_derived(d) -> str:
+    slots = [s for s in ("nodal_values", "time_derivative") if s in d]
+    names = sorted({k for s in slots for k in d[s]})
+    lines = [f"{'derived':40s}" + "".join(f"{s:>18s}" for s in slots)]
+    for k in names:
+        lines.append(f"{k:40s}" + "".join(f"{d[s].get(k, float('nan')):18.6g}" for s in slots))
+    return "\n".join(lines)
+
+
 def traffic(fetch_dir: str, write_dir: str, copy_mib: int):
     """Calibrated HBM bytes per dispatch for each kernel slot."""
     fetch = counter_means(fetch_dir, "FETCH_SIZE")
@@ -173,6 +245,8 @@ def main(argv=None):
     a.add_argument("dir")
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
+    c.add_argument("--elements", type=int, default=0, help="elements per dispatch (derived figures)")
+    c.add_argument("--trace", default=None, help="kernel-trace run of the same command (mean durations)")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -184,7 +258,11 @@ def main(argv=None):
         print(format_kernel_stats(kernel_stats(args.dir)))
         return
     if args.cmd == "counters":
-        print(format_counters(all_counters(args.dirs)))
+        data = all_counters(args.dirs)
+        print(format_counters(data))
+        if args.elements:
+            print()
+            print(format_derived(derived(data, args.elements, slot_means(args.trace) if args.trace else None)))
         return
     t = traffic(args.fetch_dir, args.write_dir, args.copy_mib)
     print(json.dumps(t, indent=1))
