@@ -69,6 +69,11 @@ struct StructLayout {
 };
 
 
+template <int V>
+struct IntConst {
+    static constexpr int value = V;
+};
+
 // DET: the deterministic variant. The partner shares of the volume pairs are added in a
 // fixed order (the pairs of one offset at a time, one half-team after the other: the
 // targets of one offset are distinct), each volume node evaluates its own row of C, and
@@ -106,10 +111,13 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
     const double Wi = tl < NQ ? p.W[tl] : 0.0;
     const double Bj = tl < NF ? p.B[tl] : 0.0;
 
-    // ---- prefetch registers
+    // ---- prefetch registers. With two threads per node, the first half-team loads and
+    // hoists the volume nodes and the second the facet nodes (the interface flux is the
+    // longer chain of the two)
+    constexpr int FH = SPLIT - 1;
     double pu[NC], pL[D * D], pJ, pf[NC], px[NC], pn[D], pJf;
     auto prefetch = [&](int k) {
-        if (tl < NQ) {
+        if (half == 0 && tl < NQ) {
             const double* uqk = p.uq + (size_t)k * NQ * NC;
             const double* Lk = p.Lam + (size_t)k * NQ * D * D;
 #pragma unroll
@@ -118,7 +126,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             for (int mn = 0; mn < D * D; ++mn) pL[mn] = Lk[mn * NQ + tl];
             pJ = p.Jq[(size_t)k * NQ + tl];
         }
-        if (tl < NF) {
+        if (FH == 0 && tl < NF) {
             const double* ufk = p.uf + (size_t)k * NF * NC;
             const double* uxk = st.uf_ext + (size_t)k * NF * NC;
 #pragma unroll
@@ -139,7 +147,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
     {
         const int k = min(kb + team, last);
         const bool valid = kb + team < p.N_e;
-        if (half == 0) prefetch(k);
+        prefetch(k);
         // ---- phase A: hoist, interface flux; clear the facet-correction accumulators
         double* srow = sst;            // NC x NQ: C-row sums of the volume nodes
         double* sfac = sst + NC * NQ;  // NC x NF: C-column sums of the facet nodes
@@ -163,7 +171,19 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             }
             Ji = pJ;
         }
-        if (half == 0 && tl < NF) {
+        if (half == FH && tl < NF) {
+            if constexpr (FH > 0) {  // the facet loads of the second half-team
+                const double* ufk = p.uf + (size_t)k * NF * NC;
+                const double* uxk = st.uf_ext + (size_t)k * NF * NC;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    pf[e] = ufk[e * NF + tl];
+                    px[e] = uxk[e * NF + tl];
+                }
+#pragma unroll
+                for (int m = 0; m < D; ++m) pn[m] = p.nJf[m + D * (tl + (size_t)NF * k)];
+                pJf = p.Jf[tl + (size_t)NF * k];
+            }
             double hin[NH], hout[NH], nrm[D], c[NC];
             hoist<D>(p.law, pf, hin);
             hoist<D>(p.law, px, hout);
@@ -202,7 +222,6 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         if (tl < NQ) {
             // tensor indices of node tl (last index fastest) and the family strides
             const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
-            const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
             // pair q of this node: family f, partner index b on the line, partner node j;
             // `act` is false for the offset N1/2 of an even line taken by the other node
             auto pair_of = [&](int q, int& f, int& b, int& j, bool& act) {
@@ -252,16 +271,24 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 }
             };
             if constexpr (!DET) {
-                double sa[D], sb[D];
-                if (q0 < q1) load_s(q0, sa);
-#pragma unroll 1
-                for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
-                    if (q + 1 < q1) load_s(q + 1, sb);
-                    eval(q, sa);
-                    if (q + 1 < q1) {
-                        if (q + 2 < q1) load_s(q + 2, sa);
-                        eval(q + 1, sb);
+                // the half-team's pairs with compile-time pair indices (half is uniform over
+                // a wavefront): the family of each pair, and with it the S_m that vanish, is
+                // then known, and the straight-line code lets the next pair's loads overlap
+                auto sweep = [&](auto hc) {
+                    constexpr int H = decltype(hc)::value;
+                    constexpr int Q0 = H * QH, Q1 = (H + 1) * QH < NPART ? (H + 1) * QH : NPART;
+                    double sv[2][D];  // the S values of a pair are requested one pair ahead
+                    load_s(Q0, sv[0]);
+#pragma unroll
+                    for (int q = Q0; q < Q1; ++q) {
+                        if (q + 1 < Q1) load_s(q + 1, sv[(q + 1 - Q0) & 1]);
+                        eval(q, sv[(q - Q0) & 1]);
                     }
+                };
+                if (half == 0) {
+                    sweep(IntConst<0>{});
+                } else if constexpr (SPLIT > 1) {
+                    sweep(IntConst<1>{});
                 }
             }
         }
@@ -391,6 +418,23 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
             }
         }
+        // the lift's rows of R' (all MAXR entries at once, padding as a zero weight on node 0)
+        // and this thread's share of the factors of V, requested here so that their latency
+        // passes during phases D and E
+        int rj[MAXR];
+        double rv[MAXR];
+        if (half == 0 && tl < NQ) {
+#pragma unroll
+            for (int w = 0; w < MAXR; ++w) {
+                rj[w] = w < tp.wrt ? tp.rt_idx[w * NQ + tl] : -1;
+                rv[w] = w < tp.wrt ? tp.rt_val[w * NQ + tl] : 0.0;
+            }
+        }
+        using VF = VFactors<ELEM, P>;
+        constexpr bool STAGE = VF::size <= NH * NQ;
+        constexpr int NVF = STAGE ? (VF::size + TS - 1) / TS : 1;
+        double vf[NVF];
+        if constexpr (STAGE) VF::fetch(tp, vf, tt, TS);
         __syncthreads();
         // ---- phase D: the C sums into r (volume nodes) and f_f (facet nodes); f_f to LDS
         double* sff = sm + L::off_ff;
@@ -408,7 +452,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
 #pragma unroll
                 for (int e = 0; e < NC; ++e) sff[e * NF + tl] = -sfac[e * NF + tl];
             }
-        } else if (half == 0 && tl < NF) {  // the node's chunk slots, in chunk order
+        } else if (half == FH && tl < NF) {  // the node's chunk slots, in chunk order
             double col[NC];
 #pragma unroll
             for (int e = 0; e < NC; ++e) col[e] = 0.0;
@@ -430,11 +474,9 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             }
 #pragma unroll
             for (int w = 0; w < MAXR; ++w) {
-                const int jj = w < tp.wrt ? tp.rt_idx[w * NQ + tl] : -1;
-                if (jj < 0) continue;
-                const double rv = tp.rt_val[w * NQ + tl];
+                if (rj[w] < 0) continue;
 #pragma unroll
-                for (int e = 0; e < NC; ++e) r[e] -= rv * sff[e * NF + jj];
+                for (int e = 0; e < NC; ++e) r[e] -= rv[w] * sff[e * NF + rj[w]];
             }
 #pragma unroll
             for (int e = 0; e < NC; ++e) sr[e * NQ + tl] = r[e];
@@ -445,11 +487,9 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
         double* sy = sm + L::off_y;
         double* sZ = sm + L::off_z;
         double* sW = sm + L::off_W;
-        using VF = VFactors<ELEM, P>;
-        constexpr bool STAGE = VF::size <= NH * NQ;
         TensorParams tq = tp;
         if constexpr (STAGE) {
-            VF::stage(tp, sm + L::off_h, tt, TS);
+            VF::put(vf, sm + L::off_h, tt, TS);
             tq = VF::in_lds(tp, sm + L::off_h);
             __syncthreads();
         }

@@ -118,6 +118,20 @@ struct VFactors {
         if constexpr (NCF > 0)
             for (int i = t; i < NCF; i += nt) dst[NA + NB + i] = tp.VC[i];
     }
+    // the same copy in two steps: thread t's entries t, t + nt, ... (of the concatenated
+    // factors) into registers, then into LDS; v holds ceil(size / nt) values
+    __device__ static void fetch(const TensorParams& tp, double* v, int t, int nt) {
+#pragma unroll
+        for (int r = 0; r * nt < size; ++r) {
+            const int i = t + r * nt;
+            v[r] = i < NA ? tp.VA[i] : i < NA + NB ? tp.VB[i - NA] : i < size ? tp.VC[i - NA - NB] : 0.0;
+        }
+    }
+    __device__ static void put(const double* v, double* dst, int t, int nt) {
+#pragma unroll
+        for (int r = 0; r * nt < size; ++r)
+            if (t + r * nt < size) dst[t + r * nt] = v[r];
+    }
     __device__ static TensorParams in_lds(const TensorParams& tp, const double* src) {
         TensorParams r = tp;
         r.VA = src;
