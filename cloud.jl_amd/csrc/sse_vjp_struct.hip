@@ -154,6 +154,11 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
 // V2: the two-point terms and the entropy variables, one element per workgroup. It
 // leaves the cotangent of the projected volume states w_q' in the element's slice of
 // rbar (read into LDS at its start) for V3.
+template <int V>
+struct VjpHalf {
+    static constexpr int value = V;
+};
+
 // DET: every sum in a fixed order, as the deterministic time derivative does: partner
 // shares of the volume pairs added one offset and one half-team at a time, each volume node
 // pulling back its own row of C, and the facet columns of C through per-chunk slots.
@@ -213,7 +218,10 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
 #pragma unroll
         for (int mn = 0; mn < D * D; ++mn) sL[mn * NQ + tl] = Lk[mn * NQ + tl];
     }
-    if (half == 0 && tl < NF) {
+    // the facet nodes' interface adjoints (the longer chain) on the last half-team, so
+    // that with two threads per node they run beside the volume hoisting
+    if (half == SPLIT - 1 && tl < NF) {
+        const int g = p.mapP[tl + (size_t)NF * k];  // first: the neighbour's loads depend on it
         const double* ufk = p.uf + (size_t)k * NF * NC;
         const double* uxk = st.uf_ext + (size_t)k * NF * NC;
         const double* fk = fbar + (size_t)k * NF * NC;
@@ -249,7 +257,6 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         if (lf) lam += 0.5 * A::lf_diss_adj(p.law, uf, ux, nrm, gb, true, ubf);  // halfλ = λ / 2
         // the neighbour's flux at the shared node, f*(u_out, u_in, n'), cotangent
         // B' J_f' fbar' (keep u_in, its exterior state)
-        const int g = p.mapP[tl + (size_t)NF * k];
         const int kn = g / NF, jn = g - kn * NF;
         const double jfn = p.Jf[jn + (size_t)NF * kn];
 #pragma unroll
@@ -271,7 +278,6 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     }
     if (tl < NQ) {  // the node's metric and rbar are read from LDS (fewer live registers)
         const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
-        const int q0 = half * QH, q1 = (half + 1) * QH < NPART ? (half + 1) * QH : NPART;
         auto pair_of = [&](int q, int& f, int& b, int& j, bool& act) {
             f = q / PER_F;
             const int dlt = q - f * PER_F + 1;
@@ -319,15 +325,36 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
             for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
         };
         if constexpr (!DET) {
-            double sa[D], sb[D];
-            if (q0 < q1) load_s(q0, sa);
+            // with two threads per node, the half-team's pairs with compile-time pair indices
+            // (half is uniform over a wavefront), the S values of each pair requested one
+            // pair ahead
+            auto sweep = [&](auto hc) {
+                constexpr int H = decltype(hc)::value;
+                constexpr int Q0 = H * QH, Q1 = (H + 1) * QH < NPART ? (H + 1) * QH : NPART;
+                double sv[2][D];
+                load_s(Q0, sv[0]);
+#pragma unroll
+                for (int q = Q0; q < Q1; ++q) {
+                    if (q + 1 < Q1) load_s(q + 1, sv[(q + 1 - Q0) & 1]);
+                    eval(q, sv[(q - Q0) & 1]);
+                }
+            };
+            if constexpr (SPLIT > 1) {
+                if (half == 0)
+                    sweep(VjpHalf<0>{});
+                else
+                    sweep(VjpHalf<1>{});
+            } else {  // one thread per node: a runtime loop keeps the register count down
+                double sa[D], sb[D];
+                if (NPART > 0) load_s(0, sa);
 #pragma unroll 1
-            for (int q = q0; q < q1; q += 2) {  // two pairs per trip: the S values alternate buffers
-                if (q + 1 < q1) load_s(q + 1, sb);
-                eval(q, sa);
-                if (q + 1 < q1) {
-                    if (q + 2 < q1) load_s(q + 2, sa);
-                    eval(q + 1, sb);
+                for (int q = 0; q < NPART; q += 2) {  // two pairs per trip: the S values alternate buffers
+                    if (q + 1 < NPART) load_s(q + 1, sb);
+                    eval(q, sa);
+                    if (q + 1 < NPART) {
+                        if (q + 2 < NPART) load_s(q + 2, sa);
+                        eval(q + 1, sb);
+                    }
                 }
             }
         }
