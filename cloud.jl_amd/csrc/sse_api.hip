@@ -27,6 +27,7 @@ namespace sse {
 hipError_t launch_generic_dispatch(const DevParams& p, const double* u, double* dudt, const RKStage& rk,
                                    hipStream_t s, int which);
 size_t lds_nodal_values_generic(const DevParams& p);
+size_t lds_nodal_values_mfma(const DevParams& p);
 size_t lds_time_derivative_generic(const DevParams& p);
 hipError_t launch_standard(const DevParams& p, double* dudt, const RKStage& rk, hipStream_t s, int which);
 size_t lds_standard(const DevParams& p);
@@ -914,6 +915,24 @@ int create_handle(const sse_desc* D, int members, sse_handle** out) {
         (rc = upload(h, Rt.ptr, &P.Rt_ptr)) || (rc = upload(h, Rt.idx, &P.Rt_idx)) ||
         (rc = upload(h, Rt.val, &P.Rt_val)))
         return cleanup_fail(rc);
+    // the matrix-core nodal values of the general path (Euler with a modal V) when the
+    // element data of 16 elements fit the LDS of a compute unit, for meshes of at least
+    // 4,096 elements (one workgroup of 16 per CU and more; on fewer elements the vector
+    // kernel, one element per workgroup, is faster). SSE_NODAL_MFMA=0 or 1 overrides.
+    if (P.project && P.has_V && D->law == SSE_EULER && D->d >= 2 && sse::lds_nodal_values_mfma(P) <= 160 * 1024) {
+        const char* env = getenv("SSE_NODAL_MFMA");
+        const bool on = env && env[0] ? env[0] != '0' : N_e >= 4096;
+        if (on) {
+            std::vector<double> Rd((size_t)N_f * N_q, 0.0);
+            if (D->R_selection) {
+                for (int j = 0; j < N_f; ++j) Rd[j + (size_t)N_f * (D->R_selection[j] - 1)] = 1.0;
+            } else {
+                std::copy(D->R, D->R + (size_t)N_f * N_q, Rd.begin());
+            }
+            if ((rc = upload(h, Rd, &P.Rd))) return cleanup_fail(rc);
+            P.mfma = 1;
+        }
+    }
     // union sparsity by rows of d stacked N_q x N_q operators A_m (column-major,
     // A[i + N_q (j + N_q m)]), optionally transposed, with or without the diagonal
     auto union_rows = [&](const double* A, bool transpose, bool skip_diag, const int** p_ptr, const int** p_idx,
@@ -1107,8 +1126,8 @@ int sse_describe(const sse_handle* h, char* buf, int32_t len) {
         snprintf(buf, len, "standard d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d mapping=%s%s", P.d, P.N_p, P.N_q, P.N_f,
                  P.N_c, P.N_e, P.skew ? "skew" : "standard", P.second_order ? " br1" : "");
     } else {
-        snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d%s", P.d, P.N_p, P.N_q, P.N_f, P.N_c,
-                 P.N_e, P.det ? " deterministic" : "");
+        snprintf(buf, len, "generic d=%d N_p=%d N_q=%d N_f=%d N_c=%d N_e=%d%s%s", P.d, P.N_p, P.N_q, P.N_f, P.N_c,
+                 P.N_e, P.mfma ? " nodal_values=mfma" : "", P.det ? " deterministic" : "");
     }
     return 0;
 }

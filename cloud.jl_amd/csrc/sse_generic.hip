@@ -409,11 +409,16 @@ size_t lds_time_derivative_generic(const DevParams& p) {
     return sizeof(double) * ((size_t)(3 * p.N_q + 2 * p.N_f + p.N_p) * p.N_c + (size_t)p.N_q * p.d * p.d);
 }
 
+hipError_t launch_nodal_values_mfma(const DevParams& p, const double* u, hipStream_t s);
+
 template <int D, int LAW>
 hipError_t launch_generic(const DevParams& p, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                           int which /* bit 0: phase 1, bit 1: phase 2 */) {
     dim3 grid(p.N_e), block(256);
-    if (which & 1) {
+    if ((which & 1) && p.mfma) {
+        hipError_t err = launch_nodal_values_mfma(p, u, s);
+        if (err != hipSuccess) return err;
+    } else if (which & 1) {
         hipLaunchKernelGGL((k_nodal_values_generic<D, LAW>), grid, block, lds_nodal_values_generic(p), s, p, u);
         hipError_t err = hipGetLastError();
         if (err != hipSuccess) return err;

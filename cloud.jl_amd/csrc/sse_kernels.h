@@ -38,6 +38,10 @@ struct DevParams {
     // item_ij = i | j << 16
     int n_items, n_vol_items;
     const int* item_ij; const double* item_val;
+    // dense R (N_f x N_q column-major) and the switch of the matrix-core nodal values
+    // (Euler, entropy projection, modal V: k_nodal_values_mfma)
+    const double* Rd;
+    int mfma;
     // deterministic mode (det = 1): the items of each node, volume nodes then facet nodes,
     // own_item[w] = c << 1 | (1 if the node is the second node of volume pair c)
     int det;

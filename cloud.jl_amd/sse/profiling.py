@@ -83,6 +83,17 @@ def kernel_stats(directory: str):
     return out
 
 
+def kernel_stats_by_grid(directory: str):
+    """Per (kernel, grid size in threads) {calls, mean_us, min_us, max_us} from a kernel-trace
+    run: one line per problem size when a run times several."""
+    out = defaultdict(list)
+    for r in _csv_rows(directory, "kernel_trace.csv"):
+        key = f"{short_name(r['Kernel_Name'])} [grid {int(r['Grid_Size_X'])}]"
+        out[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+    return {k: {"calls": len(v), "mean_us": statistics.fmean(v), "min_us": min(v), "max_us": max(v),
+                "total_us": sum(v)} for k, v in out.items()}
+
+
 def format_kernel_stats(stats) -> str:
     lines = [f"{'kernel':60s} {'calls':>6s} {'mean us':>10s} {'min us':>10s} {'max us':>10s}"]
     for k, s in sorted(stats.items(), key=lambda kv: -kv[1]["total_us"]):
@@ -243,6 +254,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("kernel-stats")
     a.add_argument("dir")
+    a.add_argument("--by-grid", action="store_true", help="one line per (kernel, grid size)")
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
     c.add_argument("--elements", type=int, default=0, help="elements per dispatch (derived figures)")
@@ -255,7 +267,7 @@ def main(argv=None):
     b.add_argument("--json", default=None, help="merge the result into this JSON file")
     args = ap.parse_args(argv)
     if args.cmd == "kernel-stats":
-        print(format_kernel_stats(kernel_stats(args.dir)))
+        print(format_kernel_stats(kernel_stats_by_grid(args.dir) if args.by_grid else kernel_stats(args.dir)))
         return
     if args.cmd == "counters":
         data = all_counters(args.dirs)

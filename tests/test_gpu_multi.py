@@ -115,3 +115,36 @@ def test_multi_repeated_residuals_agree_to_rounding():
     first = _gpu_residual(ode, u)
     for _ in range(3):
         assert _rel(_gpu_residual(ode, u), first) <= 1e-14
+
+
+MFMA_CASES = ["wave3d_modal_multi_p3_ec", "wave3d_modal_multi_p3_lf", "wave3d_modal_multi_p4_lf",
+              "vortex2d_modal_multi_p4_lf", "vortex2d_modal_multi_p2_ec"]
+
+
+@pytest.mark.parametrize("name", MFMA_CASES)
+def test_matrix_core_nodal_values_match_oracle(name, monkeypatch):
+    # the nodal values on the matrix cores (16 elements per workgroup, MFMA tiles for V,
+    # V' and R), switched on for these small meshes, whose last workgroup is partly empty
+    monkeypatch.setenv("SSE_NODAL_MFMA", "1")
+    ode = CASES[name]()
+    assert "nodal_values=mfma" in ode.p.describe(), ode.p.describe()
+    u = _perturbed(ode)
+    got = _gpu_residual(ode, u)
+    ref = O.residual(ode.p.arrays, u)
+    assert _rel(got, ref) <= RES_TOL, (name, _rel(got, ref))
+
+
+def test_matrix_core_nodal_values_are_chosen_on_a_large_mesh_and_match_oracle():
+    # 10^3 hexes x 6 tets = 6,000 elements: the matrix-core kernel by default; 20 RK steps
+    ode = density_wave_3d_config(M=10, p=3, approx=ModalMulti, ic_strength=0.2)[4]
+    assert "nodal_values=mfma" in ode.p.describe(), ode.p.describe()
+    u = _perturbed(ode)
+    got = _gpu_residual(ode, u)
+    ref = O.residual(ode.p.arrays, u)
+    assert _rel(got, ref) <= RES_TOL, _rel(got, ref)
+    from sse.time_integration import rk_steps, to_device, to_host
+    dt = 1e-3
+    ud = to_device(ode.u0)
+    rk_steps(ode.p, ud, dt, 20)
+    ref = O.rk_steps(ode.p.arrays, ode.u0, dt, 20)
+    assert _rel(to_host(ud), ref) <= RK_TOL, _rel(to_host(ud), ref)
