@@ -1732,6 +1732,11 @@ struct sse_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the captured RK step runs each group's five stages on a stream of its own (a branch
+    // of the graph, forked from and joined to `stream`), so that groups overlap
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     std::string err;
 };
 
@@ -2027,13 +2032,50 @@ int sse_batch_rk_steps(sse_batch* b, double* const* u, const double* dt, int32_t
     }
     if (use_graph && step < nsteps) {
         if (!b->exec) {
+            // one branch per group (SSE_BATCH_BRANCHES=0: all groups in sequence on one stream)
+            const char* env = getenv("SSE_BATCH_BRANCHES");
+            const bool branches = !(env && env[0] == '0') && b->groups.size() > 1;
+            if (branches && b->branch.empty()) {
+                b->branch.assign(b->groups.size(), nullptr);
+                b->join.assign(b->groups.size(), nullptr);
+                if ((rc = hip_ok(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event create"))) return rc;
+                for (size_t g = 0; g < b->groups.size(); ++g)
+                    if ((rc = hip_ok(hipStreamCreate(&b->branch[g]), "stream create")) ||
+                        (rc = hip_ok(hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming), "event create")))
+                        return rc;
+            }
             if ((rc = hip_ok(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
                 return rc;
-            int r = one_step();
+            int r = 0;
+            hipError_t e = hipSuccess;
+            if (branches) {
+                e = hipEventRecord(b->fork, b->stream);
+                for (size_t g = 0; g < b->groups.size() && e == hipSuccess; ++g) {
+                    e = hipStreamWaitEvent(b->branch[g], b->fork, 0);
+                    sse_set_stream(b->groups[g], b->branch[g]);
+                }
+                for (size_t g = 0; g < b->groups.size() && e == hipSuccess && !r; ++g)
+                    for (int s = 0; s < 5 && !r; ++s) {
+                        if (b->chunk[g]) {
+                            r = ensemble_stage(b, g, s, false);
+                            continue;
+                        }
+                        RKStage rk{1, CK_A[s], CK_B[s], 0.0, b->tmp[g], b->state[g], b->dt_e[g]};
+                        r = launch(b->groups[g], b->state[g], nullptr, rk, 3);
+                        if (r) r = bfail(b, r, b->groups[g]->err);
+                    }
+                for (size_t g = 0; g < b->groups.size(); ++g) {
+                    if (e == hipSuccess) e = hipEventRecord(b->join[g], b->branch[g]);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[g], 0);
+                    sse_set_stream(b->groups[g], b->stream);
+                }
+            } else {
+                r = one_step();
+            }
             hipGraph_t graph = nullptr;
-            hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            const hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
             if (r) return r;
-            if ((rc = hip_ok(e, "end capture"))) return rc;
+            if ((rc = hip_ok(e, "branch capture")) || (rc = hip_ok(e2, "end capture"))) return rc;
             b->graph = graph;
             if ((rc = hip_ok(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0), "graph instantiate")))
                 return rc;
@@ -2160,6 +2202,11 @@ void sse_batch_destroy(sse_batch* b) {
     for (auto* v : {&b->b_dev, &b->t_state, &b->t_tmp, &b->t_dt, &b->t_b})
         for (double* p : *v)
             if (p) (void)hipFree(p);
+    for (hipStream_t st : b->branch)
+        if (st) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : b->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

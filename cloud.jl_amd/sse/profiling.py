@@ -11,6 +11,8 @@ Two summaries, both plain text or JSON so they can be kept beside the code:
   size (``bench.py --copy-probe-mib``), so that the figure is in real bytes
   whatever fraction of the memory channels the counter samples.
 
+* ``busy``: how much of the window spanned by the large dispatches the GPU spends
+  running kernels (the gaps between the dispatches of a graph replay);
 * ``counters``: every counter of one or more ``rocprofv3 --pmc`` runs, mean per
   dispatch, for each kernel slot; with ``--elements`` and ``--trace``, the derived
   figures below the table (instructions per element, the VALU issue floor against
@@ -20,6 +22,7 @@ Usage::
 
     python -m sse.profiling kernel-stats <trace_dir>
     python -m sse.profiling counters <pmc_dir> [<pmc_dir> ...] [--elements N --trace <trace_dir>]
+    python -m sse.profiling busy <trace_dir> --min-grid THREADS
     python -m sse.profiling traffic <fetch_dir> <write_dir> --copy-mib 256 \
         --workload NAME [--json profiles/hbm_traffic.json]
 """
@@ -92,6 +95,35 @@ def kernel_stats_by_grid(directory: str):
         out[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
     return {k: {"calls": len(v), "mean_us": statistics.fmean(v), "min_us": min(v), "max_us": max(v),
                 "total_us": sum(v)} for k, v in out.items()}
+
+
+def busy(directory: str, min_grid: int):
+    """GPU occupancy of the window spanned by the dispatches whose grid has at least
+    `min_grid` threads: the union of all kernels' busy intervals in that window against its
+    length, the number of dispatches in it, and the gaps between consecutive dispatches."""
+    rows = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), int(r["Grid_Size_X"]), r["Kernel_Name"])
+            for r in _csv_rows(directory, "kernel_trace.csv")]
+    big = [r for r in rows if r[2] >= min_grid]
+    if not big:
+        raise RuntimeError("no dispatch with that grid size")
+    t0, t1 = min(r[0] for r in big), max(r[1] for r in big)
+    win = sorted((a, b) for a, b, _, _ in rows if a >= t0 and b <= t1)
+    busy_ns, cur_a, cur_b = 0, None, None
+    gaps = []
+    for a, b in win:
+        if cur_b is None:
+            cur_a, cur_b = a, b
+        elif a > cur_b:
+            busy_ns += cur_b - cur_a
+            gaps.append(a - cur_b)
+            cur_a, cur_b = a, b
+        else:
+            cur_b = max(cur_b, b)
+    busy_ns += cur_b - cur_a
+    span = t1 - t0
+    return {"window_us": span * 1e-3, "busy_us": busy_ns * 1e-3, "busy_frac": busy_ns / span,
+            "dispatches": len(win), "gaps": len(gaps), "mean_gap_us": (sum(gaps) / len(gaps) * 1e-3) if gaps else 0.0,
+            "kernel_sum_us": sum(b - a for a, b in win) * 1e-3}
 
 
 def format_kernel_stats(stats) -> str:
@@ -255,6 +287,9 @@ def main(argv=None):
     a = sub.add_parser("kernel-stats")
     a.add_argument("dir")
     a.add_argument("--by-grid", action="store_true", help="one line per (kernel, grid size)")
+    bz = sub.add_parser("busy")
+    bz.add_argument("dir")
+    bz.add_argument("--min-grid", type=int, required=True, help="threads; dispatches at least this large span the window")
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
     c.add_argument("--elements", type=int, default=0, help="elements per dispatch (derived figures)")
@@ -268,6 +303,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "kernel-stats":
         print(format_kernel_stats(kernel_stats_by_grid(args.dir) if args.by_grid else kernel_stats(args.dir)))
+        return
+    if args.cmd == "busy":
+        print(json.dumps(busy(args.dir, args.min_grid), indent=1))
         return
     if args.cmd == "counters":
         data = all_counters(args.dirs)
