@@ -67,14 +67,17 @@ def _csv_rows(directory: str, suffix: str):
     return rows
 
 
-def kernel_stats(directory: str):
-    """Per-kernel {calls, mean_us, min_us, max_us} from a kernel-trace run."""
+def kernel_stats(directory: str, last: int = 0):
+    """Per-kernel {calls, mean_us, min_us, max_us} from a kernel-trace run; with `last`, over
+    each kernel's last `last` dispatches only (in launch order)."""
     trace = _csv_rows(directory, "kernel_trace.csv")
     out = {}
     if trace:
         durs = defaultdict(list)
-        for r in trace:
+        for r in sorted(trace, key=lambda r: int(r["Start_Timestamp"])):
             durs[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+        if last > 0:
+            durs = {k: v[-last:] for k, v in durs.items()}
         for k, v in durs.items():
             out[k] = {"calls": len(v), "mean_us": statistics.fmean(v), "min_us": min(v),
                       "max_us": max(v), "total_us": sum(v)}
@@ -287,6 +290,7 @@ def main(argv=None):
     a = sub.add_parser("kernel-stats")
     a.add_argument("dir")
     a.add_argument("--by-grid", action="store_true", help="one line per (kernel, grid size)")
+    a.add_argument("--last", type=int, default=0, help="each kernel's last N dispatches only")
     bz = sub.add_parser("busy")
     bz.add_argument("dir")
     bz.add_argument("--min-grid", type=int, required=True, help="threads; dispatches at least this large span the window")
@@ -302,7 +306,7 @@ def main(argv=None):
     b.add_argument("--json", default=None, help="merge the result into this JSON file")
     args = ap.parse_args(argv)
     if args.cmd == "kernel-stats":
-        print(format_kernel_stats(kernel_stats_by_grid(args.dir) if args.by_grid else kernel_stats(args.dir)))
+        print(format_kernel_stats(kernel_stats_by_grid(args.dir) if args.by_grid else kernel_stats(args.dir, args.last)))
         return
     if args.cmd == "busy":
         print(json.dumps(busy(args.dir, args.min_grid), indent=1))
