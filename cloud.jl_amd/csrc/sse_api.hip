@@ -1496,6 +1496,18 @@ int sse_set_stream(sse_handle* h, void* stream) {
     return 0;
 }
 
+int sse_set_diffusion(sse_handle* h, double b) {
+    if (!h) return SSE_ERR_INVALID_ARGUMENT;
+    if (!h->P.second_order) return fail(h, SSE_ERR_UNSUPPORTED, "sse_set_diffusion: not an advection-diffusion law");
+    h->P.b = b;
+    h->TP.base.b = b;
+    if (h->rk_exec) (void)hipGraphExecDestroy(h->rk_exec);  // it holds the old b in its kernel arguments
+    if (h->rk_graph) (void)hipGraphDestroy(h->rk_graph);
+    h->rk_exec = nullptr;
+    h->rk_graph = nullptr;
+    return 0;
+}
+
 int sse_synchronize(sse_handle* h) {
     if (!h) return SSE_ERR_INVALID_ARGUMENT;
     SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));

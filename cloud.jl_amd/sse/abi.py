@@ -49,7 +49,7 @@ class SseDesc(C.Structure):
 
 # Names of every entry point declared in include/sse_hip.h.
 EXPORTED_SYMBOLS = (
-    "sse_create", "sse_residual", "sse_rk_steps", "sse_profile_residual", "sse_set_stream",
+    "sse_create", "sse_residual", "sse_rk_steps", "sse_profile_residual", "sse_set_stream", "sse_set_diffusion",
     "sse_synchronize", "sse_last_error", "sse_destroy", "sse_abi_version", "sse_describe",
     "sse_batch_create", "sse_batch_rk_steps", "sse_batch_describe", "sse_batch_last_error",
     "sse_batch_destroy", "sse_functional", "sse_residual_vjp", "sse_rk_adjoint", "sse_batch_rk_tangent",

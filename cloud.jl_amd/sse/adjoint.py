@@ -87,15 +87,16 @@ def rk_adjoint(solver: Solver, u0, dt: float, nsteps: int, checkpoint_every: int
 
 
 def energy_adjoint_helpers(solver: Solver):
-    """The three solvers ``energy_adjoint_b`` runs, the same for every diffusion
-    coefficient on this mesh (reuse them across a sweep): the velocity a and the
-    velocity reversed, both at b = 0, and a = 0 at b = 1 (the viscous operator dR/db)."""
+    """The solvers ``energy_adjoint_b`` shares across a sweep of diffusion coefficients on
+    this mesh: the law (a, b) and the velocity reversed (-a, b), whose handles it sets to
+    each member's b (``sse_set_diffusion``, no new copy of the geometry), and the viscous
+    operator dR/db (velocity 0 at b = 1; R is affine in b)."""
     from .conservation_laws import LinearAdvectionDiffusionEquation
     law = solver.conservation_law
     if not isinstance(law, LinearAdvectionDiffusionEquation):
         raise NotImplementedError("the velocity-reversal adjoint is for linear advection-diffusion")
-    return (solver.with_law(LinearAdvectionDiffusionEquation(tuple(law.a), 0.0)),
-            solver.with_law(LinearAdvectionDiffusionEquation(tuple(-x for x in law.a), 0.0)),
+    return (solver.with_law(LinearAdvectionDiffusionEquation(tuple(law.a), law.b)),
+            solver.with_law(LinearAdvectionDiffusionEquation(tuple(-x for x in law.a), law.b)),
             solver.with_law(LinearAdvectionDiffusionEquation(tuple(0.0 for _ in law.a), 1.0)))
 
 
@@ -111,12 +112,14 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
     u~ = M^{-1} ubar, k~ = M^{-1} kbar the reverse of a stage (k = A_s k + dt R(u_s);
     u = u + B_s k) is
         k~ += B_s u~;  u~ += dt R_{-a}(k~);  dJ/db += dt <k~, dR/db(u_s)>_M;  k~ *= A_s
-    with R = A(a, 0) + b dR/db, R_{-a} = A(-a, 0) + b dR/db and dR/db(u) = R(u; a = 0,
-    b = 1) (R is affine in b), evaluated by the three solvers of
-    ``energy_adjoint_helpers`` (pass them to reuse them across members; `solver` only
-    supplies a and b), and <x, y>_M from ``sse_functional``. u~ starts at u_N.
-    Every operator application is a forward residual on the GPU; step states are
-    kept and each step's stage states are recomputed once. `u0` is a device tensor
+    with R_{-a} the residual with the velocity reversed and the same b, and dR/db(u) =
+    R(u; a = 0, b = 1) (R is affine in b), from the three solvers of
+    ``energy_adjoint_helpers`` (pass them to share them across members; `solver` only
+    supplies b, set on the first two with ``sse_set_diffusion``). <x, y>_M is
+    ``sse_functional``. u~ starts at u_N. Every operator application is a forward residual
+    on the GPU: one per stage in the forward sweep (the library's RK steps), two in the
+    recomputation (R(u_s) and dR/db(u_s)), one in the reverse stage. Step states are kept
+    and each step's stage states are recomputed once. `u0` is a device tensor
     (N_e, 1, N_p).
 
     Each solver's handle has its own stream, ordered against torch's default stream
@@ -126,9 +129,12 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
 
     from .analysis import EnergyConservationAnalysis
     from .time_integration import CarpenterKennedy2N54 as CK
+    from .hip import check
     own = helpers is None
-    fwd0, rev0, visc = energy_adjoint_helpers(solver) if own else helpers
+    fwd, rev, visc = energy_adjoint_helpers(solver) if own else helpers
     b = float(solver.conservation_law.b)
+    for h_ in (fwd, rev):
+        check(h_.lib, h_.handle, h_.lib.sse_set_diffusion(h_.handle, b))
     energy = EnergyConservationAnalysis(visc)  # the mass matrix is the same for every law on the mesh
     r = torch.empty_like(u0)
 
@@ -136,18 +142,19 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
         """One step; with `keep`, the viscous terms dR/db(u_s) of the stage states."""
         k = torch.zeros_like(u)
         for s in range(5):
-            dv = keep[s] if keep is not None else r
-            semi_discrete_residual(dv, u, visc)
-            k.mul_(CK.A[s]).add_(dv, alpha=dt * b)
-            semi_discrete_residual(r, u, fwd0)
-            k.add_(r, alpha=dt)
+            if keep is not None:
+                semi_discrete_residual(keep[s], u, visc)
+            semi_discrete_residual(r, u, fwd)
+            k.mul_(CK.A[s]).add_(r, alpha=dt)
             u.add_(k, alpha=CK.B[s])
 
     try:
+        # the forward sweep on the library's RK steps (stage update fused into the residual)
+        from .time_integration import rk_steps
         states = [u0.clone()]
         u = u0.clone()
         for _ in range(nsteps):
-            step(u)
+            rk_steps(fwd, u, dt, 1)
             states.append(u.clone())
         J = float(energy.evaluate_conservation(u)[0])
         ut = u.clone()
@@ -159,14 +166,12 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
             kt.zero_()
             for s in range(4, -1, -1):
                 kt.add_(ut, alpha=CK.B[s])
-                semi_discrete_residual(r, kt, rev0)
+                semi_discrete_residual(r, kt, rev)
                 ut.add_(r, alpha=dt)
-                semi_discrete_residual(r, kt, visc)
-                ut.add_(r, alpha=dt * b)
                 dJdb += dt * float(energy.evaluate_conservation_residual(kt, dvis[s])[0])
                 kt.mul_(CK.A[s])
         return J, dJdb, ut
     finally:
         if own:
-            for h in (fwd0, rev0, visc):
-                h.close()
+            for h_ in (fwd, rev, visc):
+                h_.close()

@@ -50,6 +50,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sse_profile_residual.restype = C.c_int
     lib.sse_set_stream.argtypes = [vp, vp]
     lib.sse_set_stream.restype = C.c_int
+    lib.sse_set_diffusion.argtypes = [vp, C.c_double]
+    lib.sse_set_diffusion.restype = C.c_int
     lib.sse_synchronize.argtypes = [vp]
     lib.sse_synchronize.restype = C.c_int
     lib.sse_last_error.argtypes = [vp]

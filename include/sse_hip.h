@@ -261,6 +261,13 @@ int sse_functional_noalloc(sse_handle* h, const double* u, const double* dudt, i
 /* Uses the given hipStream_t (0 = the handle's own stream) for later calls. */
 int sse_set_stream(sse_handle* h, void* hip_stream);
 
+/* Sets the diffusion coefficient b of an advection-diffusion handle for later calls
+ * (SSE_ERR_UNSUPPORTED for other laws). No counterpart in the reference, whose laws are
+ * values (a new Solver per coefficient, Solvers.jl:429-452); it is for callers that sweep
+ * b on one mesh (the adjoint in b), so that the operators and geometric factors are not
+ * copied to the device again for every coefficient. A captured RK step is re-captured. */
+int sse_set_diffusion(sse_handle* h, double b);
+
 /* Blocks until work queued on the handle's stream has finished. */
 int sse_synchronize(sse_handle* h);
 

@@ -800,6 +800,51 @@ def test_rk_adjoint_matches_oracle_forward_mode(every):
     assert abs(dJ_fwd - dJ_adj) <= 1e-10 * (abs(dJ_fwd) + 1e-3), (dJ_fwd, dJ_adj)
 
 
+def test_set_diffusion_equals_a_new_handle_and_recaptures_the_rk_step():
+    # sse_set_diffusion on a handle gives the residual and the (graph-captured) RK steps of
+    # a handle built with that coefficient
+    import torch
+    from sse import semi_discrete_residual
+    from sse.hip import check
+    from sse.time_integration import rk_steps, to_device, to_host
+    solvers, u0s = _advdiff_ensemble(3, (4e-3, 7e-3))
+    s0, s1 = solvers
+    u = to_device(u0s[0])
+    x0 = u.clone()
+    rk_steps(s0, x0, 1e-3, 2)  # captures the step with b = 4e-3
+    check(s0.lib, s0.handle, s0.lib.sse_set_diffusion(s0.handle, 7e-3))
+    d0, d1 = torch.empty_like(u), torch.empty_like(u)
+    semi_discrete_residual(d0, u, s0)
+    semi_discrete_residual(d1, u, s1)
+    torch.cuda.synchronize()
+    assert _rel(to_host(d0), to_host(d1)) <= 1e-14
+    x0, x1 = u.clone(), u.clone()
+    rk_steps(s0, x0, 1e-3, 3)
+    rk_steps(s1, x1, 1e-3, 3)
+    assert _rel(to_host(x0), to_host(x1)) <= 1e-14
+
+
+def test_energy_adjoint_in_b_with_shared_helpers_matches_oracle():
+    # helpers built for b = 4e-3 serve a member with b = 7e-3
+    from oracle import functionals as F
+    from sse.adjoint import energy_adjoint_b, energy_adjoint_helpers
+    from sse.time_integration import to_device
+    solvers, u0s = _advdiff_ensemble(3, (4e-3, 7e-3))
+    helpers = energy_adjoint_helpers(solvers[0])
+    try:
+        s, u0 = solvers[1], u0s[1]
+        dt, n = 1e-3, 4
+        J, dJdb, g = energy_adjoint_b(s, to_device(u0), dt, n, helpers)
+        uN, duN = O.rk_jvp(s.arrays, u0, np.zeros_like(u0), dt, n, db=1.0)
+        sd = s.spatial_discretization
+        assert abs(J - 0.5 * F.energy_rate(sd, uN, uN)[0]) <= 1e-12 * abs(J)
+        want = F.energy_rate(sd, uN, duN)[0]
+        assert abs(dJdb - want) <= 1e-10 * abs(want), (dJdb, want)
+    finally:
+        for h_ in helpers:
+            h_.close()
+
+
 def test_energy_adjoint_in_b_matches_oracle_forward_mode():
     # the discrete adjoint of the BR1 solve (run on the forward kernels with the
     # velocity reversed) against the oracle's forward-mode duals: dJ/db for
