@@ -30,11 +30,12 @@ namespace {
 // team size and split of a node's partner list (as the forward kernel)
 template <int ELEM, int P>
 struct VjpTune;
-template <int P> struct VjpTune<ELEM_TRI, P> { static constexpr int TS = 64, SPLIT = 1; };
-template <> struct VjpTune<ELEM_TET, 2> { static constexpr int TS = 64, SPLIT = 1; };
-template <> struct VjpTune<ELEM_TET, 3> { static constexpr int TS = 128, SPLIT = 2; };
-template <> struct VjpTune<ELEM_TET, 4> { static constexpr int TS = 256, SPLIT = 2; };
-template <> struct VjpTune<ELEM_TET, 5> { static constexpr int TS = 256, SPLIT = 1; };
+// WAVES: waves per SIMD V2's register allocation must allow (what its LDS allows)
+template <int P> struct VjpTune<ELEM_TRI, P> { static constexpr int TS = 64, SPLIT = 1, WAVES = 3; };
+template <> struct VjpTune<ELEM_TET, 2> { static constexpr int TS = 64, SPLIT = 1, WAVES = 3; };
+template <> struct VjpTune<ELEM_TET, 3> { static constexpr int TS = 128, SPLIT = 2, WAVES = 3; };
+template <> struct VjpTune<ELEM_TET, 4> { static constexpr int TS = 256, SPLIT = 2, WAVES = 2; };
+template <> struct VjpTune<ELEM_TET, 5> { static constexpr int TS = 256, SPLIT = 1, WAVES = 2; };
 
 template <int ELEM, int P>
 struct VjpLayout {
@@ -43,20 +44,24 @@ struct VjpLayout {
     static constexpr int NC = Sh::NC, D = Sh::D, NH = EulerAdj<Sh::D>::NH;
     static constexpr int ITEMS = ELEM == ELEM_TET ? 3 * Sh::N1 * Sh::N1 + Sh::N1 * Sh::N1 * Sh::N1 : 3 * Sh::N1;
     static constexpr int MAXITEM = (ITEMS + T::TS - 1) / T::TS * T::TS;
+    // the facet normals nJ and fbar at the facet nodes are read from global memory where
+    // they are used, and two arrays live in space that is dead by then, so that the LDS
+    // of one element allows WAVES waves per SIMD
     static constexpr int off_h = 0;                          // NH x NQ hoisted volume states
     static constexpr int off_L = off_h + NH * Sh::NQ;        // D*D x NQ
     static constexpr int off_hf = off_L + D * D * Sh::NQ;    // NH x NF hoisted facet states
-    static constexpr int off_nJ = off_hf + NH * Sh::NF;      // D x NF
-    static constexpr int off_rb = off_nJ + D * Sh::NF;       // NC x NQ rbar
-    static constexpr int off_fb = off_rb + NC * Sh::NQ;      // NC x NF fbar
-    static constexpr int off_st = off_fb + NC * Sh::NF;      // NC x (NQ + NF) C-row and C-column sums,
+    static constexpr int off_rb = off_hf + NH * Sh::NF;      // NC x NQ rbar
+    static constexpr int off_st = off_rb + NC * Sh::NQ;      // NC x (NQ + NF) C-row and C-column sums,
                                                               // or NC x MAXITEM chunk slots (deterministic)
     static constexpr int ST = NC * (Sh::NQ + Sh::NF) > NC * MAXITEM ? NC * (Sh::NQ + Sh::NF) : NC * MAXITEM;
-    static constexpr int off_ub = off_st + ST;               // NC x NQ split partial rows, then w_q' cotangent
-    static constexpr int off_wf = off_ub + NC * Sh::NQ;      // NC x NF w_f cotangent
-    static constexpr int off_red = off_wf + NC * Sh::NF;     // TS reduction buffer
-    static constexpr int off_fn = off_red + T::TS;           // D x NFAC reference face normals
+    static constexpr int off_wf = off_st + ST;               // NC x NF w_f cotangent
+    static constexpr int off_fn = off_wf + NC * Sh::NF;      // D x NFAC reference face normals
     static constexpr int size = off_fn + D * Sh::NFAC;
+    // after the facet-correction pairs: the split partial rows, then the w_q' cotangent, in
+    // the metric's array; at the end the TS reduction buffer in the C sums' array
+    static constexpr int off_ub = off_L;
+    static constexpr int off_red = off_st;
+    static_assert(NC <= D * D && T::TS <= ST, "aliases fit");
 };
 
 // LDS of the reverse projection (phase V3)
@@ -163,7 +168,7 @@ struct VjpHalf {
 // shares of the volume pairs added one offset and one half-team at a time, each volume node
 // pulling back its own row of C, and the facet columns of C through per-chunk slots.
 template <int ELEM, int P, bool DET>
-__global__ void __launch_bounds__((VjpTune<ELEM, P>::TS))
+__global__ void __launch_bounds__((VjpTune<ELEM, P>::TS), (VjpTune<ELEM, P>::WAVES))
 k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, const double* __restrict__ fbar,
                 const double* __restrict__ fbar_ext, double* __restrict__ lam_partial, VjpOut out) {
     using Sh = Shape<ELEM, P>;
@@ -185,9 +190,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     double* sh = sm + L::off_h;
     double* sL = sm + L::off_L;
     double* shf = sm + L::off_hf;
-    double* snJ = sm + L::off_nJ;
     double* srb = sm + L::off_rb;
-    double* sfb = sm + L::off_fb;
     double* srow = sm + L::off_st;       // NC x NQ: facet-correction sums of the volume nodes
     double* sfac = srow + NC * NQ;       // NC x NF: facet-correction sums of the facet nodes
     double* sub = sm + L::off_ub;
@@ -226,15 +229,14 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         const double* uxk = st.uf_ext + (size_t)k * NF * NC;
         const double* fk = fbar + (size_t)k * NF * NC;
         const double* fxk = fbar_ext + (size_t)k * NF * NC;
-        double uf[NC], ux[NC], ubf[NC], hin[NH], hout[NH], fo[NC], nrm[D], gb[NC];
+        double uf[NC], ux[NC], ubf[NC], hin[NH], hout[NH], fo[NC], fown[NC], nrm[D], gb[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) ubf[e] = 0.0;
 #pragma unroll
         for (int e = 0; e < NC; ++e) {
             uf[e] = ufk[e * NF + tl];
             ux[e] = uxk[e * NF + tl];
-            const double f = fk[e * NF + tl];
-            sfb[e * NF + tl] = f;
+            fown[e] = fk[e * NF + tl];
             fo[e] = fxk[e * NF + tl];
         }
         A::hoist_full(p.law, uf, hin);
@@ -244,15 +246,13 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         const double jf = p.Jf[tl + (size_t)NF * k];
 #pragma unroll
         for (int m = 0; m < D; ++m) {
-            const double nj = p.nJf[m + D * (tl + (size_t)NF * k)];
-            snJ[m * NF + tl] = nj;
-            nrm[m] = nj / jf;
+            nrm[m] = p.nJf[m + D * (tl + (size_t)NF * k)] / jf;
         }
         const bool lf = (p.law.inviscid_flux == 1);
         // this node's flux f*(u_in, u_out, n), cotangent B J_f fbar (keep u_in)
         const double bj = p.B[tl] * jf;
 #pragma unroll
-        for (int e = 0; e < NC; ++e) gb[e] = bj * sfb[e * NF + tl];
+        for (int e = 0; e < NC; ++e) gb[e] = bj * fown[e];
         A::ec_dot_adj_h(p.law, hin, hout, nrm, gb, true, ubf);
         if (lf) lam += 0.5 * A::lf_diss_adj(p.law, uf, ux, nrm, gb, true, ubf);  // halfλ = λ / 2
         // the neighbour's flux at the shared node, f*(u_out, u_in, n'), cotangent
@@ -426,14 +426,14 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
                 for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
 #pragma unroll
                 for (int n = 0; n < D; ++n) {
-                    double s = snJ[n * NF + j];
+                    double s = p.nJf[n + D * (j + (size_t)NF * k)];
 #pragma unroll
                     for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + tl] * sfn[l + D * fc];
                     av[n] = 0.5 * cv * s;
                 }
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
-                    gb[e] = -(srb[e * NQ + tl] + sfb[e * NF + j]);
+                    gb[e] = -(srb[e * NQ + tl] + fbar[(size_t)k * NF * NC + e * NF + j]);
                     dummy[e] = 0.0;
                 }
                 A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, dummy);
@@ -454,9 +454,14 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
             ci[w] = st.ci_i[w * MAXITEM + item];
             cv[w] = st.ci_v[w * MAXITEM + item];
         }
-        double hj[NH], part[NC], fnl[D];
+        double hj[NH], part[NC], fnl[D], njv[D], fbj[NC];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) part[e] = 0.0;
+        for (int e = 0; e < NC; ++e) {
+            part[e] = 0.0;
+            fbj[e] = fbar[(size_t)k * NF * NC + e * NF + j];
+        }
+#pragma unroll
+        for (int n = 0; n < D; ++n) njv[n] = p.nJf[n + D * (j + (size_t)NF * k)];
 #pragma unroll
         for (int v = 0; v < NH; ++v) hj[v] = shf[v * NF + j];
         const int fc = j / NPF;
@@ -471,14 +476,14 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
             for (int v = 0; v < NH; ++v) hv[v] = sh[v * NQ + i];
 #pragma unroll
             for (int n = 0; n < D; ++n) {
-                double s = snJ[n * NF + j];
+                double s = njv[n];
 #pragma unroll
                 for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * fnl[l];
                 av[n] = 0.5 * cv[w] * s;
             }
 #pragma unroll
             for (int e = 0; e < NC; ++e) {
-                gb[e] = -(srb[e * NQ + i] + sfb[e * NF + j]);
+                gb[e] = -(srb[e * NQ + i] + fbj[e]);
                 uv[e] = 0.0;
             }
             A::ec_dot_adj_h2(p.law, hv, hj, av, gb, uv, part);
@@ -495,9 +500,24 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
             for (int e = 0; e < NC; ++e) atomicAdd(&sfac[e * NF + j], part[e]);
         }
     }
-    if (SPLIT > 1 && half == 1 && tl < NQ) {
+    // the lift's rows of R' (all MAXR entries, padding as a zero weight on node 0),
+    // requested before the barriers so that their latency passes during phase D
+    constexpr int MAXR = TENSOR_MAXR;
+    int rj[MAXR];
+    double rv[MAXR];
+    if (half == 0 && tl < NQ) {
 #pragma unroll
-        for (int e = 0; e < NC; ++e) sub[e * NQ + tl] = ubq[e];
+        for (int w = 0; w < MAXR; ++w) {
+            rj[w] = w < tp.wrt ? tp.rt_idx[w * NQ + tl] : -1;
+            rv[w] = w < tp.wrt ? tp.rt_val[w * NQ + tl] : 0.0;
+        }
+    }
+    if constexpr (SPLIT > 1) {
+        __syncthreads();  // the pairs are done with the metric array, which takes the partial rows
+        if (half == 1 && tl < NQ) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sub[e * NQ + tl] = ubq[e];
+        }
     }
     __syncthreads();
     // ---- phase D: cotangents of the entropy variables, w = c2e(u): volume and facet
@@ -545,12 +565,11 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         for (int e = 0; e < NC; ++e) wb[e] = 0.0;
         A::e2c_adj_prim(p.law, hi[A::H_RHO], hi + A::H_V, hi[A::H_P], ubq, wb);
         // w_q' cotangent += R' (w_f cotangent)
-        for (int ww = 0; ww < tp.wrt; ++ww) {
-            const int jj = tp.rt_idx[ww * NQ + tl];
-            if (jj < 0) break;
-            const double rv = tp.rt_val[ww * NQ + tl];
 #pragma unroll
-            for (int e = 0; e < NC; ++e) wb[e] += rv * swf[e * NF + jj];
+        for (int w = 0; w < MAXR; ++w) {
+            if (rj[w] < 0) continue;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) wb[e] += rv[w] * swf[e * NF + rj[w]];
         }
 #pragma unroll
         for (int e = 0; e < NC; ++e) sub[e * NQ + tl] = wb[e];
