@@ -134,17 +134,28 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
     double* rk = rbar + (size_t)k * NQ * NC;
     for (int t = tl; t < NQ * NC; t += TS) rk[t] = sx[t];
     for (int j = tl; j < NF; j += TS) {
+        const int g = p.mapP[j + (size_t)NF * k];
         double acc[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) acc[e] = 0.0;
-        for (int w = 0; w < tp.wr; ++w) {
-            const int i = tp.r_idx[w * NF + j];
-            if (i < 0) break;
-            const double v = tp.r_val[w * NF + j];
+        // the row four entries at a time, their loads issued together (rows are padded
+        // with index -1 to a multiple of four; a group whose first entry is padding ends it)
+        for (int w0 = 0; w0 < tp.wr; w0 += 4) {
+            int ii[4];
+            double rv[4];
 #pragma unroll
-            for (int e = 0; e < NC; ++e) acc[e] += v * sx[e * NQ + i];
+            for (int q = 0; q < 4; ++q) {
+                ii[q] = tp.r_idx[(w0 + q) * NF + j];
+                rv[q] = tp.r_val[(w0 + q) * NF + j];
+            }
+            if (ii[0] < 0) break;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ii[q] < 0) continue;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) acc[e] += rv[q] * sx[e * NQ + ii[q]];
+            }
         }
-        const int g = p.mapP[j + (size_t)NF * k];
         const int kn = g / NF, jn = g - kn * NF;
         double* fk = fbar + (size_t)k * NF * NC;
         double* fx = fbar_ext + (size_t)kn * NF * NC;
