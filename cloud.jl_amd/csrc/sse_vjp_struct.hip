@@ -138,15 +138,16 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
         double acc[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) acc[e] = 0.0;
-        // the row four entries at a time, their loads issued together (rows are padded
-        // with index -1 to a multiple of four; a group whose first entry is padding ends it)
+        // the row four entries at a time, their loads issued together (entries past the
+        // table width count as padding, index -1; a group whose first entry is padding ends it)
         for (int w0 = 0; w0 < tp.wr; w0 += 4) {
             int ii[4];
             double rv[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                ii[q] = tp.r_idx[(w0 + q) * NF + j];
-                rv[q] = tp.r_val[(w0 + q) * NF + j];
+                const bool in = w0 + q < tp.wr;
+                ii[q] = in ? tp.r_idx[(w0 + q) * NF + j] : -1;
+                rv[q] = in ? tp.r_val[(w0 + q) * NF + j] : 0.0;
             }
             if (ii[0] < 0) break;
 #pragma unroll
