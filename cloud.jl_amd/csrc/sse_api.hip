@@ -101,6 +101,7 @@ struct sse_handle {
     double *vjp_rbar = nullptr, *vjp_fbar = nullptr, *vjp_lam = nullptr, *vjp_lamout = nullptr;  // VJP scratch
     double* vjp_fext = nullptr;  // fbar in the neighbour's exterior slot (tensor VJP kernels)
     bool vjp_tensor = false;
+    hipStream_t adj_stream = nullptr;  // sse_rk_adjoint: the segment recomputation, beside the reverse sweep
     double* noalloc_ws = nullptr;  // this device's address of the file-scope workspace `scratch`
     std::string noalloc_err;       // why that address is missing, if it is
     std::vector<void*> allocations;
@@ -1366,8 +1367,14 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         *p = static_cast<double*>(q);
         return 0;
     };
+    const hipStream_t ms = h->stream;  // the caller's stream: forward and reverse sweeps
+    hipEvent_t ev_fwd = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     auto release = [&]() {
-        (void)hipStreamSynchronize(h->stream);
+        h->stream = ms;
+        if (h->adj_stream) (void)hipStreamSynchronize(h->adj_stream);
+        (void)hipStreamSynchronize(ms);
+        for (hipEvent_t e : {ev_fwd, ev_rec[0], ev_rec[1], ev_rev[0], ev_rev[1]})
+            if (e) (void)hipEventDestroy(e);
         for (void* q : mine) (void)hipFree(q);
     };
     double *u = nullptr, *ktmp = nullptr, *ub = nullptr, *kb = nullptr, *vb = nullptr, *ckpt = nullptr,
@@ -1388,8 +1395,45 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         release();
         return rc;
     }
-    auto p1_of = [&](int idx) {
-        double* b = p1 + p1_stage * (size_t)idx;
+    // With a second set of segment buffers, segment s - 1 is recomputed on a second stream
+    // while the reverse sweep runs through segment s (SSE_ADJ_OVERLAP=0: one stream). Each
+    // buffer set is reused every other segment; events order a recomputation after the
+    // reverse sweep that last read its buffers, and a reverse sweep after its recomputation.
+    struct SegBufs {
+        double* stages;
+        double* p1;
+    };
+    SegBufs bufs[2] = {{stages, p1}, {nullptr, nullptr}};
+    double *u2 = nullptr, *ktmp2 = nullptr;
+    bool overlap = keep_p1 && nseg > 1;
+    if (const char* env = getenv("SSE_ADJ_OVERLAP")) overlap = overlap && env[0] != '0';
+    if (overlap) {  // optional: without the memory (or a stream), one stream does both
+        const size_t want[4] = {n * n_stages, p1_stage * n_stages, n, n};
+        double** into[4] = {&bufs[1].stages, &bufs[1].p1, &u2, &ktmp2};
+        for (int i = 0; i < 4 && overlap; ++i) {
+            void* q = nullptr;
+            if (hipMalloc(&q, want[i] * sizeof(double)) != hipSuccess) {
+                (void)hipGetLastError();  // clear the allocation failure
+                overlap = false;
+                break;
+            }
+            mine.push_back(q);
+            *into[i] = static_cast<double*>(q);
+        }
+        if (overlap && !h->adj_stream && hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            h->adj_stream = nullptr;
+            overlap = false;
+        }
+        for (hipEvent_t* e : {&ev_fwd, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
+            if (overlap && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                overlap = false;
+            }
+    }
+    auto buf_of = [&](int seg) -> const SegBufs& { return bufs[overlap ? (seg & 1) : 0]; };
+    auto p1_of = [&](const SegBufs& sb, int idx) {
+        double* b = sb.p1 + p1_stage * (size_t)idx;
         return Phase1Bufs{b, b + p1q, b + p1q + p1f};
     };
     auto hip_step = [&](hipError_t e, const char* what) {
@@ -1403,22 +1447,23 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         }                   \
     } while (0)
     ADJ_TRY(hip_step(hipMemcpyAsync(u, u0, n * sizeof(double),
-                                    where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream),
+                                    where == SSE_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ms),
                      "copy u0"));
-    // one stage of a segment that keeps what its VJP reads: the stage state and, on the
-    // fast path, the phase-1 outputs
-    auto stage_keep = [&](int step, int first, int s) -> int {
+    // one stage of a segment that keeps what its VJP reads in sb: the stage state and, on
+    // the fast path, the phase-1 outputs; us and kt are the RK state and register, and the
+    // launches go to h->stream
+    auto stage_keep = [&](const SegBufs& sb, double* us, double* kt, int step, int first, int s) -> int {
         const int idx = 5 * (step - first) + s;
-        double* keep = stages + n * (size_t)idx;
+        double* keep = sb.stages + n * (size_t)idx;
         if (keep_p1) {  // the stage kernel also stores the stage state
-            RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr, keep};
-            return launch_struct_into(h, u, rk, p1_of(idx));
+            RKStage rk{1, CK_A[s], CK_B[s], dt, kt, us, nullptr, keep};
+            return launch_struct_into(h, us, rk, p1_of(sb, idx));
         }
-        int r = hip_step(hipMemcpyAsync(keep, u, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
+        int r = hip_step(hipMemcpyAsync(keep, us, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
                          "stage state");
         if (r) return r;
-        RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
-        return launch(h, u, nullptr, rk, 3);
+        RKStage rk{1, CK_A[s], CK_B[s], dt, kt, us, nullptr};
+        return launch(h, us, nullptr, rk, 3);
     };
     // forward sweep with checkpoints at the start of each segment; the last segment
     // keeps its stages as it goes, so the reverse sweep does not recompute it
@@ -1426,34 +1471,54 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
     for (int step = 0; step < nsteps; ++step) {
         if (step % checkpoint_every == 0)
             ADJ_TRY(hip_step(hipMemcpyAsync(ckpt + n * (size_t)(step / checkpoint_every), u, n * sizeof(double),
-                                            hipMemcpyDeviceToDevice, h->stream), "checkpoint"));
+                                            hipMemcpyDeviceToDevice, ms), "checkpoint"));
         for (int s = 0; s < 5; ++s) {
             if (step >= last_first) {
-                ADJ_TRY(stage_keep(step, last_first, s));
+                ADJ_TRY(stage_keep(buf_of(nseg - 1), u, ktmp, step, last_first, s));
                 continue;
             }
             RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
             ADJ_TRY(launch(h, u, nullptr, rk, 3));
         }
     }
+    if (overlap) ADJ_TRY(hip_step(hipEventRecord(ev_fwd, ms), "event"));  // the checkpoints are written
+    // a segment's stages again from its checkpoint, on stream st with its own RK state
+    auto recompute = [&](int seg, hipStream_t st, double* us, double* kt) -> int {
+        const int first = seg * checkpoint_every, last = std::min(nsteps, first + checkpoint_every);
+        h->stream = st;
+        int r = hip_step(hipMemcpyAsync(us, ckpt + n * (size_t)seg, n * sizeof(double), hipMemcpyDeviceToDevice, st),
+                         "restore");
+        for (int step = first; step < last && !r; ++step)
+            for (int s = 0; s < 5 && !r; ++s) r = stage_keep(buf_of(seg), us, kt, step, first, s);
+        h->stream = ms;
+        return r;
+    };
     // objective and its gradient at the final state
     ADJ_TRY(hip_step(sse::launch_functionals(h->P, u, nullptr, SSE_ENTROPY, h->fpartial ? h->fpartial : ktmp,
-                                             vb, h->stream), "entropy"));
-    ADJ_TRY(hip_step(hipMemcpyAsync(J, vb, sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy J"));
-    ADJ_TRY(hip_step(hipStreamSynchronize(h->stream), "sync"));
-    ADJ_TRY(hip_step(sse::launch_entropy_grad(h->P, u, ub, h->stream), "entropy gradient"));
-    ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lamout, 0, sizeof(double), h->stream), "zero"));
-    if (fast) ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lam, 0, h->P.N_e * sizeof(double), h->stream), "zero"));
+                                             vb, ms), "entropy"));
+    ADJ_TRY(hip_step(hipMemcpyAsync(J, vb, sizeof(double), hipMemcpyDeviceToHost, ms), "copy J"));
+    ADJ_TRY(hip_step(hipStreamSynchronize(ms), "sync"));
+    ADJ_TRY(hip_step(sse::launch_entropy_grad(h->P, u, ub, ms), "entropy gradient"));
+    ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lamout, 0, sizeof(double), ms), "zero"));
+    if (fast) ADJ_TRY(hip_step(hipMemsetAsync(h->vjp_lam, 0, h->P.N_e * sizeof(double), ms), "zero"));
     // reverse sweep, one segment at a time
+    bool rev_recorded[2] = {false, false};
+    if (overlap) ADJ_TRY(hip_step(hipStreamWaitEvent(h->adj_stream, ev_fwd, 0), "wait"));
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int first = seg * checkpoint_every;
         const int last = std::min(nsteps, first + checkpoint_every);
-        if (seg != nseg - 1) {  // the last segment's stages are still there from the forward sweep
-            ADJ_TRY(hip_step(hipMemcpyAsync(u, ckpt + n * (size_t)seg, n * sizeof(double), hipMemcpyDeviceToDevice,
-                                            h->stream), "restore"));
-            for (int step = first; step < last; ++step)
-                for (int s = 0; s < 5; ++s) ADJ_TRY(stage_keep(step, first, s));
+        if (overlap) {
+            if (seg >= 1) {  // the previous segment, beside this one's reverse sweep
+                const int b = (seg - 1) & 1;
+                if (rev_recorded[b]) ADJ_TRY(hip_step(hipStreamWaitEvent(h->adj_stream, ev_rev[b], 0), "wait"));
+                ADJ_TRY(recompute(seg - 1, h->adj_stream, u2, ktmp2));
+                ADJ_TRY(hip_step(hipEventRecord(ev_rec[b], h->adj_stream), "event"));
+            }
+            if (seg != nseg - 1) ADJ_TRY(hip_step(hipStreamWaitEvent(ms, ev_rec[seg & 1], 0), "wait"));
+        } else if (seg != nseg - 1) {  // the last segment's stages are still there from the forward sweep
+            ADJ_TRY(recompute(seg, ms, u, ktmp));
         }
+        const SegBufs& sb = buf_of(seg);
         // adjoint of the 2N stages: kbar += B_s ubar; ubar += dt R'(u_s)' kbar; kbar *= A_s
         for (int step = last - 1; step >= first; --step) {
             if (fast) {
@@ -1461,21 +1526,25 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
                 // kernel), then ubar += dt R'(u_s)' kbar (written by its last kernel)
                 for (int s = 4; s >= 0; --s) {
                     const int idx = 5 * (step - first) + s;
-                    const double* us = stages + n * (size_t)idx;
-                    const Phase1Bufs b = keep_p1 ? p1_of(idx) : Phase1Bufs{nullptr, nullptr, nullptr};
+                    const double* us = sb.stages + n * (size_t)idx;
+                    const Phase1Bufs b = keep_p1 ? p1_of(sb, idx) : Phase1Bufs{nullptr, nullptr, nullptr};
                     ADJ_TRY(vjp_tensor_device(h, us, kb, ub, dt, 1, keep_p1 ? &b : nullptr, ub,
                                               s == 4 ? 0.0 : CK_A[s + 1], CK_B[s]));
                 }
                 continue;
             }
-            ADJ_TRY(hip_step(hipMemsetAsync(kb, 0, n * sizeof(double), h->stream), "zero"));
+            ADJ_TRY(hip_step(hipMemsetAsync(kb, 0, n * sizeof(double), ms), "zero"));
             for (int s = 4; s >= 0; --s) {
-                const double* us = stages + n * (size_t)(5 * (step - first) + s);
-                ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[s], ub, 1.0, kb, h->stream), "axpby"));
+                const double* us = sb.stages + n * (size_t)(5 * (step - first) + s);
+                ADJ_TRY(hip_step(sse::launch_axpby(n, CK_B[s], ub, 1.0, kb, ms), "axpby"));
                 ADJ_TRY(vjp_device(h, us, kb, vb, dt, 1));
-                ADJ_TRY(hip_step(sse::launch_axpby(n, dt, vb, 1.0, ub, h->stream), "axpby"));
-                if (s > 0) ADJ_TRY(hip_step(sse::launch_axpby(n, 0.0, kb, CK_A[s], kb, h->stream), "scale"));
+                ADJ_TRY(hip_step(sse::launch_axpby(n, dt, vb, 1.0, ub, ms), "axpby"));
+                if (s > 0) ADJ_TRY(hip_step(sse::launch_axpby(n, 0.0, kb, CK_A[s], kb, ms), "scale"));
             }
+        }
+        if (overlap) {  // this segment's buffers are free once its reverse sweep is done
+            ADJ_TRY(hip_step(hipEventRecord(ev_rev[seg & 1], ms), "event"));
+            rev_recorded[seg & 1] = true;
         }
     }
     if (fast) ADJ_TRY(hip_step(sse::launch_vjp_sum(h->vjp_lam, h->P.N_e, h->vjp_lamout, 1.0, 0, h->stream), "λ sum"));
@@ -1722,6 +1791,7 @@ void sse_destroy(sse_handle* h) {
     if (h->rk_graph) (void)hipGraphDestroy(h->rk_graph);
     for (void* p : h->allocations) (void)hipFree(p);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    if (h->adj_stream) (void)hipStreamDestroy(h->adj_stream);
     delete h;
 }
 

@@ -518,6 +518,20 @@ def test_deterministic_rk_adjoint_is_bitwise_repeatable():
     assert J1 == J2 and l1 == l2 and np.array_equal(g1, g2)
 
 
+@pytest.mark.parametrize("nsteps,every", [(8, 3), (9, 2), (4, 2)])
+def test_rk_adjoint_recompute_beside_reverse_sweep_equals_one_stream(monkeypatch, nsteps, every):
+    # segments recomputed on a second stream while the reverse sweep runs (two buffer sets,
+    # reused every other segment) give the gradient of one stream bit for bit; odd and
+    # even segment counts, and a count of two
+    from sse.adjoint import rk_adjoint
+    law, sd, ode, solver = _deterministic(DET_CASES["tet_p3_even_lines"])
+    u0 = _perturbed(ode, amp=1e-3)
+    J1, g1, l1 = rk_adjoint(solver, u0, 2e-3, nsteps, checkpoint_every=every)
+    monkeypatch.setenv("SSE_ADJ_OVERLAP", "0")
+    J0, g0, l0 = rk_adjoint(solver, u0, 2e-3, nsteps, checkpoint_every=every)
+    assert J1 == J0 and l1 == l0 and np.array_equal(g1, g0)
+
+
 def test_rk_steps_graph_replay_matches_eager_and_oracle(monkeypatch):
     # sse_rk_steps replays one captured step; a new state pointer or a new dt recaptures
     from sse.time_integration import rk_steps, to_device, to_host
