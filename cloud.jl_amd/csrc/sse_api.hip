@@ -1465,16 +1465,17 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         RKStage rk{1, CK_A[s], CK_B[s], dt, kt, us, nullptr};
         return launch(h, us, nullptr, rk, 3);
     };
-    // forward sweep with checkpoints at the start of each segment; the last segment
-    // keeps its stages as it goes, so the reverse sweep does not recompute it
-    const int last_first = (nseg - 1) * checkpoint_every;
+    // forward sweep with checkpoints at the start of each segment; the last segments (one
+    // per buffer set) keep their stages as they go, so the reverse sweep does not recompute them
+    const int kept_first = overlap ? std::max(nseg - 2, 0) : nseg - 1;
     for (int step = 0; step < nsteps; ++step) {
         if (step % checkpoint_every == 0)
             ADJ_TRY(hip_step(hipMemcpyAsync(ckpt + n * (size_t)(step / checkpoint_every), u, n * sizeof(double),
                                             hipMemcpyDeviceToDevice, ms), "checkpoint"));
         for (int s = 0; s < 5; ++s) {
-            if (step >= last_first) {
-                ADJ_TRY(stage_keep(buf_of(nseg - 1), u, ktmp, step, last_first, s));
+            const int seg = step / checkpoint_every;
+            if (seg >= kept_first) {
+                ADJ_TRY(stage_keep(buf_of(seg), u, ktmp, step, seg * checkpoint_every, s));
                 continue;
             }
             RKStage rk{1, CK_A[s], CK_B[s], dt, ktmp, u, nullptr};
@@ -1508,14 +1509,14 @@ int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, i
         const int first = seg * checkpoint_every;
         const int last = std::min(nsteps, first + checkpoint_every);
         if (overlap) {
-            if (seg >= 1) {  // the previous segment, beside this one's reverse sweep
+            if (seg - 1 >= 0 && seg - 1 < kept_first) {  // the previous segment, beside this one's reverse sweep
                 const int b = (seg - 1) & 1;
                 if (rev_recorded[b]) ADJ_TRY(hip_step(hipStreamWaitEvent(h->adj_stream, ev_rev[b], 0), "wait"));
                 ADJ_TRY(recompute(seg - 1, h->adj_stream, u2, ktmp2));
                 ADJ_TRY(hip_step(hipEventRecord(ev_rec[b], h->adj_stream), "event"));
             }
-            if (seg != nseg - 1) ADJ_TRY(hip_step(hipStreamWaitEvent(ms, ev_rec[seg & 1], 0), "wait"));
-        } else if (seg != nseg - 1) {  // the last segment's stages are still there from the forward sweep
+            if (seg < kept_first) ADJ_TRY(hip_step(hipStreamWaitEvent(ms, ev_rec[seg & 1], 0), "wait"));
+        } else if (seg < kept_first) {  // the kept segments' stages are still there from the forward sweep
             ADJ_TRY(recompute(seg, ms, u, ktmp));
         }
         const SegBufs& sb = buf_of(seg);
