@@ -32,7 +32,7 @@ size_t lds_time_derivative_generic(const DevParams& p);
 hipError_t launch_standard(const DevParams& p, double* dudt, const RKStage& rk, hipStream_t s, int which);
 size_t lds_standard(const DevParams& p);
 hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
-                              double* out, hipStream_t s, double* chol = nullptr, int chol_ready = 0);
+                              double* out, hipStream_t s, double* mass = nullptr, int mass_ready = 0);
 size_t lds_functionals(const DevParams& p, int which);
 hipError_t launch_vjp(const DevParams& p, const double* u, const double* ybar, double* ubar, double* rbar,
                       double* fbar, double* lam_partial, double* lam_out, double lam_scale, int accumulate,
@@ -96,8 +96,8 @@ struct sse_handle {
     double* fpartial = nullptr;  // functionals: per-element partial sums and the result
     double* fout = nullptr;
     double* ferr = nullptr;      // staging of u_exact for sse_l2_error with host arrays
-    double* fchol = nullptr;     // Cholesky factors of every element's M_k^{-1} (energy functionals)
-    bool fchol_ready = false;
+    double* fmass = nullptr;     // every element's M_k, kept by the first energy functional
+    bool fmass_ready = false;
     double *vjp_rbar = nullptr, *vjp_fbar = nullptr, *vjp_lam = nullptr, *vjp_lamout = nullptr;  // VJP scratch
     double* vjp_fext = nullptr;  // fbar in the neighbour's exterior slot (tensor VJP kernels)
     bool vjp_tensor = false;
@@ -1185,14 +1185,14 @@ int sse_functional(sse_handle* h, const double* u, const double* dudt, int32_t w
     }
     const int kind = which + (dudt ? 3 : 0);
     // the mass-matrix factors depend only on the handle's geometry: kept after the first energy call
-    if (which == SSE_ENERGY && h->P.has_V && !h->fchol &&
-        (rc = alloc(h, (size_t)h->P.N_p * h->P.N_p * h->P.N_e, &h->fchol)))
+    if (which == SSE_ENERGY && h->P.has_V && !h->fmass &&
+        (rc = alloc(h, (size_t)h->P.N_p * h->P.N_p * h->P.N_e, &h->fmass)))
         return rc;
-    double* chol = which == SSE_ENERGY ? h->fchol : nullptr;
-    hipError_t e = sse::launch_functionals(h->P, ud, vd, kind, h->fpartial, h->fout, h->stream, chol,
-                                           chol && h->fchol_ready ? 1 : 0);
+    double* mass = which == SSE_ENERGY ? h->fmass : nullptr;
+    hipError_t e = sse::launch_functionals(h->P, ud, vd, kind, h->fpartial, h->fout, h->stream, mass,
+                                           mass && h->fmass_ready ? 1 : 0);
     if (e != hipSuccess) return fail(h, SSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    if (chol) h->fchol_ready = true;  // stream order: later calls read the factors after this one wrote them
+    if (mass) h->fmass_ready = true;  // stream order: later calls read M_k after this one wrote it
     const int nout = which == SSE_ENTROPY ? 1 : h->P.N_c;
     SSE_HIP_TRY(h, hipMemcpyAsync(out, h->fout, nout * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     SSE_HIP_TRY(h, hipStreamSynchronize(h->stream));

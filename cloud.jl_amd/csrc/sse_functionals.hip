@@ -13,10 +13,10 @@
 //   volume     sum_k sum_i W_i J_k,i                                (for normalize)
 // M_k is the mass matrix of the handle's solver: (V' diag(W / J_k) V)^{-1} for the
 // weight-adjusted solver with M^{-1} = I (src/Solvers/mass_matrix.jl:150-153), W J_k
-// for the diagonal one. The energy kinds solve with the Cholesky factor of M_k^{-1};
-// the caller may keep the factors of every element (`chol`) so that later calls skip
-// forming and factoring M_k^{-1}. One workgroup per element writes its partial sums;
-// a second kernel adds them in a fixed order (deterministic, no atomics).
+// for the diagonal one. The energy kinds form M_k in LDS; the caller may keep every
+// element's M_k (`mass`) so that later calls read it instead. One workgroup per element
+// writes its partial sums; a second kernel adds them in a fixed order (deterministic, no
+// atomics).
 #include <hip/hip_runtime.h>
 
 #include "sse_kernels.h"
@@ -63,8 +63,8 @@ __device__ double entropy_of(const LawParams& P, const double* u) {
 template <int D, int LAW>
 __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* __restrict__ u,
                                                     const double* __restrict__ dudt, int which,
-                                                    double* __restrict__ partial, double* __restrict__ chol,
-                                                    int chol_ready) {
+                                                    double* __restrict__ partial, double* __restrict__ mass,
+                                                    int mass_ready) {
     using Ph = Physics<D, LAW>;
     constexpr int NC = Ph::NC;
     extern __shared__ double sm[];
@@ -75,8 +75,7 @@ __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* _
     double* s_v = s_u + Np * NC;     // Np x NC (dudt)
     double* s_uq = s_v + Np * NC;    // Nq x NC
     double* s_vq = s_uq + Nq * NC;   // Nq x NC
-    double* s_M = s_vq + Nq * NC;    // Np x Np (M_k^{-1}, then its Cholesky factor)
-    double* s_y = s_M + Np * Np;     // Np x NC
+    double* s_M = s_vq + Nq * NC;    // Np x Np (M_k^{-1}, then M_k), and 2 Np
     const double* Jk = p.Jq + (size_t)k * Nq;
     const size_t base = (size_t)k * Np * NC;
     const bool error = which == FUNC_L2_ERROR;
@@ -179,62 +178,57 @@ __global__ void __launch_bounds__(FT) k_functionals(DevParams p, const double* _
         }
         return;
     }
-    // energy: y = M_k x by solving (M_k^{-1}) y = x, x = u (energy) or dudt (rate)
-    double* ck = chol ? chol + (size_t)k * Np * Np : nullptr;
-    if (ck && chol_ready) {  // the factor of M_k^{-1} from an earlier call
-        for (int t = threadIdx.x; t < Np * Np; t += FT) s_M[t] = ck[t];
-        for (int t = threadIdx.x; t < Np * NC; t += FT) s_y[t] = rate ? s_v[t] : s_u[t];
+    // energy: u' M_k x, x = u (energy) or dudt (rate). M_k is formed as the inverse of
+    // M_k^{-1} (in LDS, by Gauss-Jordan elimination: M_k^{-1} is symmetric positive
+    // definite, so no pivoting) or read from `mass`, where an earlier call kept it
+    double* mk = mass ? mass + (size_t)k * Np * Np : nullptr;
+    const double* Mk = mk;
+    if (!(mk && mass_ready)) {
+        if (p.has_V) {
+            for (int t = threadIdx.x; t < Np * Np; t += FT) {  // M_k^{-1} = V' diag(W/J) V
+                const int a = t % Np, b = t / Np;
+                double s = 0.0;
+                for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * a] * (p.W[i] / Jk[i]) * p.V[i + (size_t)Nq * b];
+                s_M[t] = s;
+            }
+        } else {
+            for (int t = threadIdx.x; t < Np * Np; t += FT) {
+                const int a = t % Np, b = t / Np;
+                s_M[t] = a == b ? 1.0 / (p.W[a] * Jk[a]) : 0.0;
+            }
+        }
+        double* rowk = s_M + Np * Np;  // the pivot row and column of the step
+        double* colk = rowk + Np;
+        for (int kk = 0; kk < Np; ++kk) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < Np; t += FT) {
+                rowk[t] = s_M[kk + Np * t];
+                colk[t] = s_M[t + Np * kk];
+            }
+            __syncthreads();
+            const double ip = 1.0 / rowk[kk];
+            for (int i = threadIdx.x; i < Np; i += FT) {
+                const double ci = colk[i];
+                for (int j = 0; j < Np; ++j) {
+                    const double rkj = (j == kk ? 1.0 : rowk[j]) * ip;  // row kk after the step
+                    s_M[i + Np * j] = i == kk ? rkj : (j == kk ? 0.0 : s_M[i + Np * j]) - ci * rkj;
+                }
+            }
+        }
         __syncthreads();
-    } else {
-    if (p.has_V) {
-        for (int t = threadIdx.x; t < Np * Np; t += FT) {  // M_k^{-1} = V' diag(W/J) V
-            const int a = t % Np, b = t / Np;
-            double s = 0.0;
-            for (int i = 0; i < Nq; ++i) s += p.V[i + (size_t)Nq * a] * (p.W[i] / Jk[i]) * p.V[i + (size_t)Nq * b];
-            s_M[t] = s;
-        }
-    } else {
-        for (int t = threadIdx.x; t < Np * Np; t += FT) {
-            const int a = t % Np, b = t / Np;
-            s_M[t] = a == b ? 1.0 / (p.W[a] * Jk[a]) : 0.0;
-        }
+        if (mk)
+            for (int t = threadIdx.x; t < Np * Np; t += FT) mk[t] = s_M[t];
+        Mk = s_M;
     }
-    for (int t = threadIdx.x; t < Np * NC; t += FT) s_y[t] = rate ? s_v[t] : s_u[t];
-    __syncthreads();
-    // Cholesky (lower, in place, column by column)
-    for (int j = 0; j < Np; ++j) {
-        if (threadIdx.x == 0) s_M[j + Np * j] = sqrt(s_M[j + Np * j]);
-        __syncthreads();
-        const double djj = s_M[j + Np * j];
-        for (int a = j + 1 + threadIdx.x; a < Np; a += FT) s_M[a + Np * j] /= djj;
-        __syncthreads();
-        for (int t = threadIdx.x; t < (Np - j - 1) * (Np - j - 1); t += FT) {
-            const int a = j + 1 + t % (Np - j - 1), b = j + 1 + t / (Np - j - 1);
-            if (b <= a) s_M[a + Np * b] -= s_M[a + Np * j] * s_M[b + Np * j];
-        }
-        __syncthreads();
-    }
-    if (ck)
-        for (int t = threadIdx.x; t < Np * Np; t += FT) ck[t] = s_M[t];
-    }
-    // forward and back substitution, one variable per thread
-    for (int e = threadIdx.x; e < NC; e += FT) {
-        double* y = s_y + Np * e;
-        for (int a = 0; a < Np; ++a) {
-            double s = y[a];
-            for (int b = 0; b < a; ++b) s -= s_M[a + Np * b] * y[b];
-            y[a] = s / s_M[a + Np * a];
-        }
-        for (int a = Np - 1; a >= 0; --a) {
-            double s = y[a];
-            for (int b = a + 1; b < Np; ++b) s -= s_M[b + Np * a] * y[b];
-            y[a] = s / s_M[a + Np * a];
-        }
-    }
-    __syncthreads();
+    // u' M_k x: thread a takes row a of M_k (columns of consecutive words)
+    const double* x = rate ? s_v : s_u;
     for (int e = 0; e < NC; ++e) {
         double s = 0.0;
-        for (int a = threadIdx.x; a < Np; a += FT) s += s_u[a + Np * e] * s_y[a + Np * e];
+        for (int a = threadIdx.x; a < Np; a += FT) {
+            double y = 0.0;
+            for (int b = 0; b < Np; ++b) y += Mk[a + Np * b] * x[b + Np * e];
+            s += s_u[a + Np * e] * y;
+        }
         const double tot = block_sum(s, red);
         if (threadIdx.x == 0) out[e] = rate ? tot : 0.5 * tot;
     }
@@ -252,19 +246,22 @@ __global__ void __launch_bounds__(FT) k_sum_partials(const double* __restrict__ 
     }
 }
 
-// LDS of one functional kind: the mass-matrix factor and the solve buffer only for
+// LDS of one functional kind: the mass matrix and a pivot row and column only for
 // the energy kinds of a modal (weight-adjusted) solver
 size_t lds_functionals(const DevParams& p, int which) {
     const bool matrix = (which == 1 || which == 4) && p.has_V;
     return sizeof(double) * ((size_t)FT + 2 * (size_t)p.N_p * p.N_c + 2 * (size_t)p.N_q * p.N_c +
-                             (matrix ? (size_t)p.N_p * p.N_p + (size_t)p.N_p * p.N_c : 0));
+                             (matrix ? (size_t)p.N_p * p.N_p + 2 * (size_t)p.N_p : 0));
 }
 
 template <int D, int LAW>
 hipError_t launch_functionals_t(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
-                                double* out, hipStream_t s, double* chol, int chol_ready) {
-    hipLaunchKernelGGL((k_functionals<D, LAW>), dim3(p.N_e), dim3(FT), lds_functionals(p, which), s, p, u, dudt, which,
-                       partial, chol, chol_ready);
+                                double* out, hipStream_t s, double* mass, int mass_ready) {
+    // with M_k kept, the matrix is read from global memory: no LDS for it
+    const bool kept = mass && mass_ready && (which == 1 || which == 4) && p.has_V;
+    const size_t lds = lds_functionals(p, which) - (kept ? sizeof(double) * ((size_t)p.N_p * p.N_p + 2 * (size_t)p.N_p) : 0);
+    hipLaunchKernelGGL((k_functionals<D, LAW>), dim3(p.N_e), dim3(FT), lds, s, p, u, dudt, which,
+                       partial, mass, mass_ready);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FT), 0, s, partial, p.N_e,
@@ -273,15 +270,15 @@ hipError_t launch_functionals_t(const DevParams& p, const double* u, const doubl
 }
 
 hipError_t launch_functionals(const DevParams& p, const double* u, const double* dudt, int which, double* partial,
-                              double* out, hipStream_t s, double* chol, int chol_ready) {
+                              double* out, hipStream_t s, double* mass, int mass_ready) {
     const bool euler = (p.N_c != 1);
     switch (p.d) {
-        case 1: return euler ? launch_functionals_t<1, LAW_EULER>(p, u, dudt, which, partial, out, s, chol, chol_ready)
-                             : launch_functionals_t<1, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, chol, chol_ready);
-        case 2: return euler ? launch_functionals_t<2, LAW_EULER>(p, u, dudt, which, partial, out, s, chol, chol_ready)
-                             : launch_functionals_t<2, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, chol, chol_ready);
-        case 3: return euler ? launch_functionals_t<3, LAW_EULER>(p, u, dudt, which, partial, out, s, chol, chol_ready)
-                             : launch_functionals_t<3, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, chol, chol_ready);
+        case 1: return euler ? launch_functionals_t<1, LAW_EULER>(p, u, dudt, which, partial, out, s, mass, mass_ready)
+                             : launch_functionals_t<1, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, mass, mass_ready);
+        case 2: return euler ? launch_functionals_t<2, LAW_EULER>(p, u, dudt, which, partial, out, s, mass, mass_ready)
+                             : launch_functionals_t<2, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, mass, mass_ready);
+        case 3: return euler ? launch_functionals_t<3, LAW_EULER>(p, u, dudt, which, partial, out, s, mass, mass_ready)
+                             : launch_functionals_t<3, LAW_ADVECTION>(p, u, dudt, which, partial, out, s, mass, mass_ready);
         default: return hipErrorInvalidValue;
     }
 }
