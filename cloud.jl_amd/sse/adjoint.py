@@ -100,7 +100,7 @@ def energy_adjoint_helpers(solver: Solver):
             solver.with_law(LinearAdvectionDiffusionEquation(tuple(0.0 for _ in law.a), 1.0)))
 
 
-def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
+def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None, keep_bytes: int = 16 << 30):
     """Discrete adjoint of ``nsteps`` CK2N54 steps of the linear advection-diffusion
     residual R(u) = A(a, b) u (StandardForm, skew-symmetric mapping, LF, BR1) for
     J = 1/2 u_N' M u_N, M the solver's mass matrix. Returns (J, dJ/db, g) with
@@ -117,10 +117,12 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
     ``energy_adjoint_helpers`` (pass them to share them across members; `solver` only
     supplies b, set on the first two with ``sse_set_diffusion``). <x, y>_M is
     ``sse_functional``. u~ starts at u_N. Every operator application is a forward residual
-    on the GPU: one per stage in the forward sweep (the library's RK steps), two in the
-    recomputation (R(u_s) and dR/db(u_s)), one in the reverse stage. Step states are kept
-    and each step's stage states are recomputed once. `u0` is a device tensor
-    (N_e, 1, N_p).
+    on the GPU. When the 5 nsteps viscous terms dR/db(u_s) fit in `keep_bytes` of device
+    memory, the forward sweep runs stage by stage and keeps them (two residuals per stage:
+    R(u_s) and dR/db(u_s)), and the reverse sweep needs no recomputation; otherwise the
+    forward sweep runs on the library's RK steps (one residual per stage), keeps the step
+    states, and each step's stages are recomputed once (two residuals per stage). Either
+    way the reverse stage is one residual. `u0` is a device tensor (N_e, 1, N_p).
 
     Each solver's handle has its own stream, ordered against torch's default stream
     but not against the other handles' streams; the functionals synchronise the
@@ -149,20 +151,28 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None):
             u.add_(k, alpha=CK.B[s])
 
     try:
-        # the forward sweep on the library's RK steps (stage update fused into the residual)
-        from .time_integration import rk_steps
-        states = [u0.clone()]
+        keep_all = 5 * nsteps * u0.numel() * u0.element_size() <= keep_bytes
         u = u0.clone()
-        for _ in range(nsteps):
-            rk_steps(fwd, u, dt, 1)
-            states.append(u.clone())
+        if keep_all:  # stage by stage, keeping dR/db(u_s) of every stage
+            kept = torch.empty((nsteps, 5) + tuple(u0.shape), dtype=u0.dtype, device=u0.device)
+            for n in range(nsteps):
+                step(u, kept[n])
+        else:  # the library's RK steps (stage update fused into the residual), step states kept
+            from .time_integration import rk_steps
+            states = [u0.clone()]
+            for _ in range(nsteps):
+                rk_steps(fwd, u, dt, 1)
+                states.append(u.clone())
+            dvis = [torch.empty_like(u) for _ in range(5)]
         J = float(energy.evaluate_conservation(u)[0])
         ut = u.clone()
         kt = torch.empty_like(u)
-        dvis = [torch.empty_like(u) for _ in range(5)]
         dJdb = 0.0
         for n in range(nsteps - 1, -1, -1):
-            step(states[n].clone(), dvis)
+            if keep_all:
+                dvis = kept[n]
+            else:
+                step(states[n].clone(), dvis)
             kt.zero_()
             for s in range(4, -1, -1):
                 kt.add_(ut, alpha=CK.B[s])

@@ -859,17 +859,19 @@ def test_energy_adjoint_in_b_with_shared_helpers_matches_oracle():
             h_.close()
 
 
-def test_energy_adjoint_in_b_matches_oracle_forward_mode():
+@pytest.mark.parametrize("keep_bytes", [16 << 30, 0], ids=["kept", "recomputed"])
+def test_energy_adjoint_in_b_matches_oracle_forward_mode(keep_bytes):
     # the discrete adjoint of the BR1 solve (run on the forward kernels with the
     # velocity reversed) against the oracle's forward-mode duals: dJ/db for
-    # J = 1/2 u_N' M u_N, and the gradient state through a dot-product test
+    # J = 1/2 u_N' M u_N, and the gradient state through a dot-product test; with the
+    # stages' viscous terms kept from the forward sweep, and recomputed from step states
     from oracle import functionals as F
     from sse.adjoint import energy_adjoint_b
     from sse.time_integration import to_device, to_host
     solvers, u0s = _advdiff_ensemble(3, (4e-3,))
     s, u0 = solvers[0], u0s[0]
     dt, n = 1e-3, 6
-    J, dJdb, g = energy_adjoint_b(s, to_device(u0), dt, n)
+    J, dJdb, g = energy_adjoint_b(s, to_device(u0), dt, n, keep_bytes=keep_bytes)
     uN, duN = O.rk_jvp(s.arrays, u0, np.zeros_like(u0), dt, n, db=1.0)
     sd = s.spatial_discretization
     assert abs(J - 0.5 * F.energy_rate(sd, uN, uN)[0]) <= 1e-12 * abs(J)
