@@ -89,15 +89,20 @@ def rk_adjoint(solver: Solver, u0, dt: float, nsteps: int, checkpoint_every: int
 def energy_adjoint_helpers(solver: Solver):
     """The solvers ``energy_adjoint_b`` shares across a sweep of diffusion coefficients on
     this mesh: the law (a, b) and the velocity reversed (-a, b), whose handles it sets to
-    each member's b (``sse_set_diffusion``, no new copy of the geometry), and the viscous
-    operator dR/db (velocity 0 at b = 1; R is affine in b)."""
-    from .conservation_laws import LinearAdvectionDiffusionEquation
+    each member's b (``sse_set_diffusion``, no new copy of the geometry), the viscous
+    operator dR/db (velocity 0 at b = 1; R is affine in b), and the advection operator
+    alone (``LinearAdvectionEquation(a)``: R(u) = R_adv(u) + b dR/db(u), and R_adv needs
+    no auxiliary gradient)."""
+    from .conservation_laws import LinearAdvectionDiffusionEquation, LinearAdvectionEquation
     law = solver.conservation_law
     if not isinstance(law, LinearAdvectionDiffusionEquation):
         raise NotImplementedError("the velocity-reversal adjoint is for linear advection-diffusion")
+    adv = Solver(LinearAdvectionEquation(tuple(law.a)), solver.spatial_discretization, solver.form,
+                 solver.strategy, solver.mass_solver, solver.parallelism)
     return (solver.with_law(LinearAdvectionDiffusionEquation(tuple(law.a), law.b)),
             solver.with_law(LinearAdvectionDiffusionEquation(tuple(-x for x in law.a), law.b)),
-            solver.with_law(LinearAdvectionDiffusionEquation(tuple(0.0 for _ in law.a), 1.0)))
+            solver.with_law(LinearAdvectionDiffusionEquation(tuple(0.0 for _ in law.a), 1.0)),
+            adv)
 
 
 def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None, keep_bytes: int = 16 << 30):
@@ -113,16 +118,16 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None, k
     u = u + B_s k) is
         k~ += B_s u~;  u~ += dt R_{-a}(k~);  dJ/db += dt <k~, dR/db(u_s)>_M;  k~ *= A_s
     with R_{-a} the residual with the velocity reversed and the same b, and dR/db(u) =
-    R(u; a = 0, b = 1) (R is affine in b), from the three solvers of
+    R(u; a = 0, b = 1) (R is affine in b), from the solvers of
     ``energy_adjoint_helpers`` (pass them to share them across members; `solver` only
     supplies b, set on the first two with ``sse_set_diffusion``). <x, y>_M is
     ``sse_functional``. u~ starts at u_N. Every operator application is a forward residual
     on the GPU. When the 5 nsteps viscous terms dR/db(u_s) fit in `keep_bytes` of device
-    memory, the forward sweep runs stage by stage and keeps them (two residuals per stage:
-    R(u_s) and dR/db(u_s)), and the reverse sweep needs no recomputation; otherwise the
-    forward sweep runs on the library's RK steps (one residual per stage), keeps the step
-    states, and each step's stages are recomputed once (two residuals per stage). Either
-    way the reverse stage is one residual. `u0` is a device tensor (N_e, 1, N_p).
+    memory, the forward sweep runs stage by stage and keeps them (per stage dR/db(u_s) and
+    the advection residual R_adv(u_s), R = R_adv + b dR/db), and the reverse sweep needs no
+    recomputation; otherwise the forward sweep runs on the library's RK steps (one
+    residual per stage), keeps the step states, and each step's stages are recomputed
+    once in the same way. Either way the reverse stage is one residual. `u0` is a device tensor (N_e, 1, N_p).
 
     Each solver's handle has its own stream, ordered against torch's default stream
     but not against the other handles' streams; the functionals synchronise the
@@ -133,20 +138,21 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None, k
     from .time_integration import CarpenterKennedy2N54 as CK
     from .hip import check
     own = helpers is None
-    fwd, rev, visc = energy_adjoint_helpers(solver) if own else helpers
+    fwd, rev, visc, adv = energy_adjoint_helpers(solver) if own else helpers
     b = float(solver.conservation_law.b)
     for h_ in (fwd, rev):
         check(h_.lib, h_.handle, h_.lib.sse_set_diffusion(h_.handle, b))
     energy = EnergyConservationAnalysis(visc)  # the mass matrix is the same for every law on the mesh
     r = torch.empty_like(u0)
 
-    def step(u, keep=None):
-        """One step; with `keep`, the viscous terms dR/db(u_s) of the stage states."""
+    def step(u, keep):
+        """One step, keeping the viscous terms dR/db(u_s) of the stage states; the stage
+        residual is R_adv(u_s) + b dR/db(u_s)."""
         k = torch.zeros_like(u)
         for s in range(5):
-            if keep is not None:
-                semi_discrete_residual(keep[s], u, visc)
-            semi_discrete_residual(r, u, fwd)
+            semi_discrete_residual(keep[s], u, visc)
+            semi_discrete_residual(r, u, adv)
+            r.add_(keep[s], alpha=b)
             k.mul_(CK.A[s]).add_(r, alpha=dt)
             u.add_(k, alpha=CK.B[s])
 
@@ -183,5 +189,5 @@ def energy_adjoint_b(solver: Solver, u0, dt: float, nsteps: int, helpers=None, k
         return J, dJdb, ut
     finally:
         if own:
-            for h_ in (fwd, rev, visc):
+            for h_ in (fwd, rev, visc, adv):
                 h_.close()
