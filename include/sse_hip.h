@@ -211,8 +211,11 @@ int sse_residual_vjp(sse_handle* h, const double* u, const double* ybar, double*
  * objective J = entropy integral of the final state (SSE_ENTROPY; the only
  * objective so far): writes J, grad_u0 = dJ/du0 (N_p, N_c, N_e) and *grad_lambda =
  * dJ/dλ. The forward states are checkpointed every `checkpoint_every` steps and
- * each segment is recomputed once, keeping its stage states. No counterpart in
- * the reference. u0 and grad_u0 are device or host per `where`. Blocks. */
+ * each segment is recomputed once, keeping its stage states; with the tensor VJP
+ * kernels, the recomputation runs on a second stream of the library's beside the
+ * reverse sweep, and the forward sweep keeps its last two segments (environment
+ * SSE_ADJ_OVERLAP=0: one stream, the last segment kept). No counterpart in the
+ * reference. u0 and grad_u0 are device or host per `where`. Blocks. */
 int sse_rk_adjoint(sse_handle* h, const double* u0, double dt, int32_t nsteps, int32_t checkpoint_every,
                    int32_t objective, double* J, double* grad_u0, double* grad_lambda, int32_t where);
 
