@@ -44,6 +44,7 @@ struct TensorConfig {
     int TS, EPB, R, lds1, lds2, NQ, NF, NP;
 };
 bool tensor_config(int elem, int P, int var, TensorConfig* c);
+hipError_t launch_weight_adjusted_inverse(const DevParams& p, double* Mw, hipStream_t s);
 hipError_t launch_tensor(const TensorParams& tp, const double* u, double* dudt, const RKStage& rk, hipStream_t s,
                          int which);
 struct StructConfig {
@@ -1085,6 +1086,22 @@ int create_handle(const sse_desc* D, int members, sse_handle** out) {
     }
     if (kenv && !strcmp(kenv, "generic")) h->use_tensor = h->use_std_tensor = false;
     if (!h->use_tensor) h->use_struct = false;
+    // the tensor kernels' weight-adjusted mass solve with a kept matrix per element
+    // (N_p^2 doubles each; up to 16 GiB) in place of V, W/J and V' (SSE_MASS_MATRICES=0: off)
+    if (h->use_tensor && P.has_V && (size_t)N_p * N_p * N_e * sizeof(double) <= ((size_t)16 << 30)) {
+        const char* menv = getenv("SSE_MASS_MATRICES");
+        if (!(menv && menv[0] == '0')) {
+            double* Mw = nullptr;
+            if ((rc = alloc(h, (size_t)N_p * N_p * N_e, &Mw))) return cleanup_fail(rc);
+            hipError_t e = sse::launch_weight_adjusted_inverse(P, Mw, nullptr);
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+                h->err = std::string("weight-adjusted inverse: ") + hipGetErrorString(e);
+                return cleanup_fail(SSE_ERR_HIP);
+            }
+            h->TP.Mw = Mw;
+        }
+    }
     if (members > 1 && !h->use_std_tensor) {
         h->err = "an ensemble of problems needs the compiled StandardForm shapes of scalar laws";
         return cleanup_fail(SSE_ERR_UNSUPPORTED);

@@ -91,6 +91,9 @@ struct TensorParams {
     const double* VC;
     const double* D1;    // StandardForm scalar kernels: 1D derivative matrix of each collapsed
                          // direction, d x N1 x N1 row-major (D_m = sum_l diag(c_lm) D1_l along lines)
+    const double* Mw;    // weight-adjusted inverse mass matrix of every element, V' diag(W / J_k) V
+                         // (N_p x N_p column-major per element), or null: the kernels then apply
+                         // V, W / J_k and V' by sum factorisation
     int xcd_remap;       // 1: spread contiguous element ranges over the 8 XCDs
     double* uf_ext;      // when set, nodal_values also writes each facet state into the
                          // neighbour's exterior slot [k][e][j] (mapP is an involution)

@@ -493,15 +493,26 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
             tq = VF::in_lds(tp, sm + L::off_h);
             __syncthreads();
         }
-        SF::Vt(tq, tt, sr, sy, sZ, sW);
-        SF::V(tq, tt, sy, sr, sZ, sW);
-        if (half == 0 && tl < NQ) {
-            const double jw = Wi / Ji;
-#pragma unroll
-            for (int e = 0; e < NC; ++e) sr[e * NQ + tl] *= jw;
+        // the weight-adjusted inverse V' diag(W/J) V kept per element, requested before
+        // the transform and staged where r_q and the temporaries were
+        constexpr bool MW = NP * NP <= L::off_fn - L::off_hf;
+        MassPrefetch<NP, TS> pre;
+        if constexpr (MW) {
+            if (tp.Mw) mass_prefetch<NP, TS>(tp.Mw + (size_t)k * NP * NP, tt, pre);
         }
-        __syncthreads();
         SF::Vt(tq, tt, sr, sy, sZ, sW);
+        if (MW && tp.Mw) {
+            mass_apply_staged<NP, NC, TS>(pre, sm + L::off_hf, sy, tt);
+        } else {
+            SF::V(tq, tt, sy, sr, sZ, sW);
+            if (half == 0 && tl < NQ) {
+                const double jw = Wi / Ji;
+#pragma unroll
+                for (int e = 0; e < NC; ++e) sr[e * NQ + tl] *= jw;
+            }
+            __syncthreads();
+            SF::Vt(tq, tt, sr, sy, sZ, sW);
+        }
         const size_t base = (size_t)k * NP * NC;
         if (!valid) {
         } else if (!rk.active) {

@@ -355,4 +355,61 @@ struct SumFact<ELEM_TET, P, TS, NCV> {
     }
 };
 
+// The weight-adjusted mass solve with the element's kept inverse Mk = V' diag(W/J_k) V
+// (NP x NP column-major in global memory): mass_prefetch requests a team's share of Mk
+// into registers (issued before a transform, whose time hides the latency), then
+// mass_apply_staged writes it to LDS (sM, NP x NP, not aliasing x) and computes
+// x = Mk x in place for NC columns of NP modes in LDS: thread t of a team of TS takes
+// the items (e, j) = t, t + TS, ..., into registers, then writes them back. Ends with a
+// barrier.
+template <int NP, int TS>
+struct MassPrefetch {
+    static constexpr int MI = (NP * NP + TS - 1) / TS;
+    double v[MI];
+};
+
+template <int NP, int TS>
+__device__ __forceinline__ void mass_prefetch(const double* __restrict__ Mk, int t, MassPrefetch<NP, TS>& pre) {
+#pragma unroll
+    for (int r = 0; r < MassPrefetch<NP, TS>::MI; ++r) {
+        const int c = t + r * TS;
+        pre.v[r] = c < NP * NP ? Mk[c] : 0.0;
+    }
+}
+
+template <int NP, int NC, int TS>
+__device__ __forceinline__ void mass_apply_staged(const MassPrefetch<NP, TS>& pre, double* sM, double* x, int t) {
+#pragma unroll
+    for (int r = 0; r < MassPrefetch<NP, TS>::MI; ++r) {
+        const int c = t + r * TS;
+        if (c < NP * NP) sM[c] = pre.v[r];
+    }
+    __syncthreads();
+    constexpr int IT = (NP * NC + TS - 1) / TS;
+    double y[IT];
+#pragma unroll
+    for (int r = 0; r < IT; ++r) {
+        const int it = t + r * TS;
+        double s0 = 0.0, s1 = 0.0;
+        if (it < NP * NC) {
+            const int e = it / NP, j = it - e * NP;
+            const double* xe = x + e * NP;
+#pragma unroll 4
+            for (int i = 0; i + 1 < NP; i += 2) {
+                s0 += sM[j + NP * i] * xe[i];
+                s1 += sM[j + NP * (i + 1)] * xe[i + 1];
+            }
+            if constexpr (NP % 2 == 1) s0 += sM[j + NP * (NP - 1)] * xe[NP - 1];
+        }
+        y[r] = s0 + s1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < IT; ++r) {
+        const int it = t + r * TS;
+        if (it < NP * NC) x[it] = y[r];
+    }
+    __syncthreads();
+}
+
 }  // namespace sse

@@ -23,6 +23,35 @@
 
 namespace sse {
 
+// ----------------------------------------------------------------------------- set-up
+// The weight-adjusted inverse mass matrix of every element, V' diag(W / J_k) V
+// (mass_matrix.jl:150-153 with the orthonormal modal basis: M^{-1} = I), column-major
+// and symmetric: the upper triangle is summed, each entry written to both halves.
+__global__ void __launch_bounds__(256) k_weight_adjusted_inverse(int N_p, int N_q, const double* __restrict__ V,
+                                                                 const double* __restrict__ W,
+                                                                 const double* __restrict__ Jq, double* __restrict__ Mw) {
+    extern __shared__ double s_w[];  // N_q: W / J_k
+    const size_t k = blockIdx.x;
+    for (int i = threadIdx.x; i < N_q; i += blockDim.x) s_w[i] = W[i] / Jq[k * N_q + i];
+    __syncthreads();
+    double* Mk = Mw + k * N_p * N_p;
+    for (int t = threadIdx.x; t < N_p * N_p; t += blockDim.x) {
+        const int a = t % N_p, b = t / N_p;
+        if (a > b) continue;
+        double s = 0.0;
+        for (int i = 0; i < N_q; ++i) s += V[i + (size_t)N_q * a] * s_w[i] * V[i + (size_t)N_q * b];
+        Mk[a + (size_t)N_p * b] = s;
+        Mk[b + (size_t)N_p * a] = s;
+    }
+}
+
+hipError_t launch_weight_adjusted_inverse(const DevParams& p, double* Mw, hipStream_t s) {
+    if (!p.has_V || !p.V) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_weight_adjusted_inverse, dim3(p.N_e), dim3(256), sizeof(double) * p.N_q, s, p.N_p, p.N_q,
+                       p.V, p.W, p.Jq, Mw);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------- nodal_values
 // The sum-factorisation temporary Z shares the volume-node array: in V it is dead
 // before the last stage writes the output, in V' the input is dead after the first
@@ -84,15 +113,25 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
         for (int e = 0; e < NC; ++e) sa[e * NQ + i] = wb[e] * wj;
     }
     __syncthreads();
-    SF::Vt(tq, tl, sa, sw, sZ, sW);  // w = V' (WJ w_q)
-    SF::V(tq, tl, sw, sa, sZ, sW);   // WADG: J^{-1} W (V w)
-    for (int i = tl; i < NQ; i += TS) {
-        const double jw = p.W[i] / Jk[i];
-#pragma unroll
-        for (int e = 0; e < NC; ++e) sa[e * NQ + i] *= jw;
+    // the kept weight-adjusted inverse is staged where u_q and the temporary W were
+    constexpr bool MW = NP * NP <= L::off_u - L::off_a;
+    MassPrefetch<NP, TS> pre;
+    if constexpr (MW) {
+        if (tp.Mw) mass_prefetch<NP, TS>(tp.Mw + (size_t)kk * NP * NP, tl, pre);
     }
-    __syncthreads();
-    SF::Vt(tq, tl, sa, sw, sZ, sW);  // w = V' (...)
+    SF::Vt(tq, tl, sa, sw, sZ, sW);  // w = V' (WJ w_q)
+    if (MW && tp.Mw) {               // WADG: w = V' diag(W/J) V w with the kept matrix
+        mass_apply_staged<NP, NC, TS>(pre, sa, sw, tl);
+    } else {                         // WADG by sum factorisation: J^{-1} W (V w), then V'
+        SF::V(tq, tl, sw, sa, sZ, sW);
+        for (int i = tl; i < NQ; i += TS) {
+            const double jw = p.W[i] / Jk[i];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sa[e * NQ + i] *= jw;
+        }
+        __syncthreads();
+        SF::Vt(tq, tl, sa, sw, sZ, sW);
+    }
     SF::V(tq, tl, sw, sa, sZ, sW);   // w_q = V w
     double* uqk = p.uq + (size_t)kk * NQ * NC;
     double* ufk = p.uf + (size_t)kk * NF * NC;

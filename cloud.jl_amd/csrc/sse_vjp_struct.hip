@@ -113,6 +113,12 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
     double* sfac = sW + NC * (Sh::NW > 0 ? Sh::NW : 1);
     VF::stage(tp, sfac, tl, TS);
     const TensorParams tq = VF::in_lds(tp, sfac);
+    // the kept weight-adjusted inverse (requested with ybar), staged in x and W
+    constexpr bool MW = NP * NP <= NC * (NQ + Sh::NW);
+    MassPrefetch<NP, TS> pre;
+    if constexpr (MW) {
+        if (tp.Mw) mass_prefetch<NP, TS>(tp.Mw + (size_t)k * NP * NP, tl, pre);
+    }
     const size_t kb = (size_t)k * NP * NC;
     for (int t = tl; t < NP * NC; t += TS) {
         double y = in.a == 0.0 ? 0.0 : in.a * ybar[kb + t];
@@ -121,15 +127,19 @@ k_vjp_v1_tensor(TensorParams tp, const double* ybar, double* __restrict__ rbar, 
         sy[t] = y;
     }
     __syncthreads();
-    SF::V(tq, tl, sy, sx, sZ, sW);
-    const double* Jk = p.Jq + (size_t)k * NQ;
-    for (int i = tl; i < NQ; i += TS) {
-        const double jw = p.W[i] / Jk[i];
+    if (MW && tp.Mw) {  // M^{-1} ybar = V' diag(W/J) V ybar with the kept matrix
+        mass_apply_staged<NP, NC, TS>(pre, sx, sy, tl);
+    } else {
+        SF::V(tq, tl, sy, sx, sZ, sW);
+        const double* Jk = p.Jq + (size_t)k * NQ;
+        for (int i = tl; i < NQ; i += TS) {
+            const double jw = p.W[i] / Jk[i];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
+            for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
+        }
+        __syncthreads();
+        SF::Vt(tq, tl, sx, sy, sZ, sW);
     }
-    __syncthreads();
-    SF::Vt(tq, tl, sx, sy, sZ, sW);
     SF::V(tq, tl, sy, sx, sZ, sW);
     double* rk = rbar + (size_t)k * NQ * NC;
     for (int t = tl; t < NQ * NC; t += TS) rk[t] = sx[t];
@@ -627,6 +637,13 @@ k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __r
     double* sfac = sm + L::size;  // the factors of V in LDS after the work arrays
     VF::stage(tp, sfac, tt, TS);
     const TensorParams tq = VF::in_lds(tp, sfac);
+    // the kept weight-adjusted inverse (requested with u and the cotangent), staged in the
+    // temporaries Z and W
+    constexpr bool MW = NP * NP <= L::off_y - L::off_z;
+    MassPrefetch<NP, TS> pre;
+    if constexpr (MW) {
+        if (tp.Mw) mass_prefetch<NP, TS>(tp.Mw + (size_t)k * NP * NP, tt, pre);
+    }
     const double* uk = u + (size_t)k * NP * NC;
     const double* wk = wbar + (size_t)k * NQ * NC;
     for (int t = tt; t < NP * NC; t += TS) su[t] = uk[t];
@@ -634,15 +651,19 @@ k_vjp_v3_tensor(TensorParams tp, const double* __restrict__ u, const double* __r
     __syncthreads();
     SF::V(tq, tt, su, su0, sZ, sW);   // V u
     SF::Vt(tq, tt, sw, sy, sZ, sW);   // V' (w_q' cotangent)
-    SF::V(tq, tt, sy, sx, sZ, sW);    // M^{-1} = V' (W/J V .)
     const double* Jk = p.Jq + (size_t)k * NQ;
-    for (int i = tt; i < NQ; i += TS) {
-        const double jw = p.W[i] / Jk[i];
+    if (MW && tp.Mw) {                // M^{-1} = V' (W/J V .) with the kept matrix
+        mass_apply_staged<NP, NC, TS>(pre, sZ, sy, tt);
+    } else {
+        SF::V(tq, tt, sy, sx, sZ, sW);
+        for (int i = tt; i < NQ; i += TS) {
+            const double jw = p.W[i] / Jk[i];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
+            for (int e = 0; e < NC; ++e) sx[e * NQ + i] *= jw;
+        }
+        __syncthreads();
+        SF::Vt(tq, tt, sx, sy, sZ, sW);
     }
-    __syncthreads();
-    SF::Vt(tq, tt, sx, sy, sZ, sW);
     SF::V(tq, tt, sy, sx, sZ, sW);    // V (M^{-1} V' w_q'bar), then times WJ and through c2e
     for (int i = tt; i < NQ; i += TS) {
         const double wj = p.W[i] * Jk[i];
