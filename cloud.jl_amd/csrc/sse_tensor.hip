@@ -145,18 +145,16 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
             for (int e = 0; e < NC; ++e) uqk[e * NQ + i] = ub[e];
         }
     }
-    for (int j = tl; j < NF; j += TS) {  // u_f = e2c(R w_q)
-        double ub[NC], wb[NC];
-#pragma unroll
-        for (int e = 0; e < NC; ++e) wb[e] = 0.0;
-        // the row's entries four at a time, their loads issued together (rows are
-        // padded with index -1; a group whose first entry is padding ends the row)
-        for (int w0 = 0; w0 < tp.wr; w0 += 4) {
+    // u_f = e2c(R w_q). A row's entries are taken four at a time, their loads issued
+    // together (entries past the table width or the row's end are -1; a group whose first
+    // entry is -1 ends the row)
+    auto row_sum = [&](int j, int w_begin, int w_end, double* wb) {
+        for (int w0 = w_begin; w0 < w_end; w0 += 4) {
             int ii[4];
             double rv[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const bool in = w0 + q < tp.wr;
+                const bool in = w0 + q < w_end;
                 ii[q] = in ? tp.r_idx[(w0 + q) * NF + j] : -1;
                 rv[q] = in ? tp.r_val[(w0 + q) * NF + j] : 0.0;
             }
@@ -168,6 +166,9 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
                 for (int e = 0; e < NC; ++e) wb[e] += rv[q] * sa[e * NQ + ii[q]];
             }
         }
+    };
+    auto store_facet = [&](int j, const double* wb) {
+        double ub[NC];
         ok &= e2c_fast<D>(p.law, wb, ub);
         if (valid) {
 #pragma unroll
@@ -179,6 +180,42 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
 #pragma unroll
                 for (int e = 0; e < NC; ++e) uxn[e * NF + jn] = ub[e];
             }
+        }
+    };
+    // On a tet the last facet nodes (those of the face whose nodes each see a whole plane
+    // of volume nodes) have the long rows: the team's otherwise idle threads take the
+    // second half of the last NHELP rows and hand their partial sums over in LDS (the
+    // modal array, dead by now)
+    constexpr int NHELP = NF < TS ? TS - NF : 0;
+    constexpr bool HELP = ELEM == ELEM_TET && NHELP > 0 && NHELP <= NP;
+    if constexpr (HELP) {
+        const int wh = (tp.wr / 2 + 3) / 4 * 4;  // split point (a multiple of four)
+        const bool helper = tl >= NF;
+        const int j = helper ? tl - NHELP : tl, h = j - (NF - NHELP);
+        const bool split = h >= 0;
+        double wb[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) wb[e] = 0.0;
+        row_sum(j, helper ? wh : 0, (split && !helper) ? wh : tp.wr, wb);
+        if (helper) {
+#pragma unroll
+            for (int e = 0; e < NC; ++e) sw[e * NHELP + h] = wb[e];
+        }
+        __syncthreads();
+        if (!helper) {
+            if (split) {
+#pragma unroll
+                for (int e = 0; e < NC; ++e) wb[e] += sw[e * NHELP + h];
+            }
+            store_facet(j, wb);
+        }
+    } else {
+        for (int j = tl; j < NF; j += TS) {
+            double wb[NC];
+#pragma unroll
+            for (int e = 0; e < NC; ++e) wb[e] = 0.0;
+            row_sum(j, 0, tp.wr, wb);
+            store_facet(j, wb);
         }
     }
     if (valid && !ok) atomicOr(p.status, 1);
