@@ -625,8 +625,15 @@ def main():
     for _ in range(args.warmup):
         semi_discrete_residual(dudt, u, solver)
     solver.lib.sse_synchronize(solver.handle)
-    # per-kernel durations with HIP events on the library's stream, taken before the timed
-    # region (untimed; it also brings the clocks up from idle, as a longer warm-up would)
+    # untimed: residuals back to back for about 0.3 s, over which the GPU's clocks settle
+    # (in a kernel trace the durations fall by a quarter over the first ~40 residuals)
+    t_settle = time.perf_counter() + 0.3
+    while time.perf_counter() < t_settle:
+        for _ in range(20):
+            semi_discrete_residual(dudt, u, solver)
+        solver.lib.sse_synchronize(solver.handle)
+    # per-kernel durations with HIP events on the library's stream over residuals run back
+    # to back, taken before the timed region (untimed)
     import ctypes as C
     ms = (C.c_double * abi.SSE_NUM_KERNELS)()
     code = solver.lib.sse_profile_residual(solver.handle, u.data_ptr(), dudt.data_ptr(),

@@ -1778,25 +1778,37 @@ int sse_erk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps
 
 int sse_profile_residual(sse_handle* h, const double* u, double* dudt, int32_t nrep, double* ms_out) {
     if (!h || !u || !dudt || !ms_out || nrep <= 0) return fail(h, SSE_ERR_INVALID_ARGUMENT, "bad arguments");
-    hipEvent_t ev[3];
-    for (int i = 0; i < 3; ++i) SSE_HIP_TRY(h, hipEventCreate(&ev[i]));
-    double acc[SSE_NUM_KERNELS] = {0.0, 0.0};
-    RKStage rk{};
+    // the nrep residuals back to back, as a solve runs them, with an event after each
+    // launch; one wait at the end
+    std::vector<hipEvent_t> ev(2 * (size_t)nrep + 1, nullptr);
     int rc = 0;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            rc = fail(h, SSE_ERR_HIP, "hipEventCreate failed");
+            break;
+        }
+    RKStage rk{};
+    if (!rc && hipEventRecord(ev[0], h->stream) != hipSuccess) rc = fail(h, SSE_ERR_HIP, "hipEventRecord failed");
     for (int r = 0; r < nrep && !rc; ++r) {
-        SSE_HIP_TRY(h, hipEventRecord(ev[0], h->stream));
         if ((rc = launch(h, u, dudt, rk, 1))) break;
-        SSE_HIP_TRY(h, hipEventRecord(ev[1], h->stream));
-        if ((rc = launch(h, u, dudt, rk, 2))) break;
-        SSE_HIP_TRY(h, hipEventRecord(ev[2], h->stream));
-        SSE_HIP_TRY(h, hipEventSynchronize(ev[2]));
+        if (hipEventRecord(ev[2 * r + 1], h->stream) != hipSuccess) rc = fail(h, SSE_ERR_HIP, "hipEventRecord failed");
+        if (rc || (rc = launch(h, u, dudt, rk, 2))) break;
+        if (hipEventRecord(ev[2 * r + 2], h->stream) != hipSuccess) rc = fail(h, SSE_ERR_HIP, "hipEventRecord failed");
+    }
+    double acc[SSE_NUM_KERNELS] = {0.0, 0.0};
+    if (!rc && hipEventSynchronize(ev[2 * (size_t)nrep]) != hipSuccess) rc = fail(h, SSE_ERR_HIP, "hipEventSynchronize failed");
+    for (int r = 0; r < nrep && !rc; ++r) {
         float a = 0.f, b = 0.f;
-        SSE_HIP_TRY(h, hipEventElapsedTime(&a, ev[0], ev[1]));
-        SSE_HIP_TRY(h, hipEventElapsedTime(&b, ev[1], ev[2]));
+        if (hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]) != hipSuccess)
+            rc = fail(h, SSE_ERR_HIP, "hipEventElapsedTime failed");
         acc[0] += a;
         acc[1] += b;
     }
-    for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
+    (void)hipStreamSynchronize(h->stream);
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
     if (rc) return rc;
     for (int i = 0; i < SSE_NUM_KERNELS; ++i) ms_out[i] = acc[i] / nrep;
     return 0;

@@ -184,9 +184,9 @@ int sse_rk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps,
 int sse_erk_steps(sse_handle* h, double* u, double t0, double dt, int32_t nsteps, int32_t stages,
                   const double* A, const double* b, int32_t where);
 
-/* Runs the residual nrep times on device buffers and reports the mean duration
- * of each kernel slot in milliseconds (ms_out has SSE_NUM_KERNELS entries),
- * timed with HIP events on the handle's stream. No counterpart in the
+/* Runs the residual nrep times back to back on device buffers and reports the
+ * mean duration of each kernel slot in milliseconds (ms_out has SSE_NUM_KERNELS
+ * entries), timed with HIP events on the handle's stream. No counterpart in the
  * reference (which times with TimerOutputs, src/Solvers/Solvers.jl:474). */
 int sse_profile_residual(sse_handle* h, const double* u_dev, double* dudt_dev, int32_t nrep,
                          double* ms_out);
