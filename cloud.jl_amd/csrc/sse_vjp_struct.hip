@@ -205,6 +205,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     constexpr int DH = (N1 - 1) / 2, EVEN = N1 % 2 == 0 ? 1 : 0, PER_F = DH + EVEN;
     constexpr int NPART = D * PER_F, QH = (NPART + SPLIT - 1) / SPLIT;
     static_assert(NQ <= NT && NF <= NT, "one node per thread");
+    constexpr bool EARLY = !DET && SPLIT == 2 && NT == 64;
     extern __shared__ double sm[];
     const DevParams& p = tp.base;
     const int k = element_of_block(blockIdx.x, gridDim.x, tp.xcd_remap);
@@ -221,12 +222,84 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     double* sfn = sm + L::off_fn;
 
     // ---- phase A: loads, hoisting, interface flux adjoints; clear the C-pair sums
-    for (int t = tt; t < NC * (NQ + NF); t += TS) srow[t] = 0.0;
+    // EARLY: where each half-team is one wavefront, the first sweeps all the volume pairs
+    // right after its hoisting (a wavefront needs no workgroup barrier to see its own LDS
+    // writes) while the second computes the interface adjoints; the default mode only
+    if constexpr (EARLY) {
+        if (half == 0) {
+            for (int t = tl; t < NC * NQ; t += NT) srow[t] = 0.0;
+        } else {
+            for (int t = tl; t < NC * NF; t += NT) sfac[t] = 0.0;
+        }
+    } else {
+        for (int t = tt; t < NC * (NQ + NF); t += TS) srow[t] = 0.0;
+    }
     if (tt < D * Sh::NFAC) sfn[tt] = p.fn[tt];
     double hi[NH], ubq[NC];
     double lam = 0.0;
 #pragma unroll
     for (int e = 0; e < NC; ++e) ubq[e] = 0.0;
+    // the volume line pairs of node tl (phase B; in the early schedule run in phase A)
+    const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
+    auto pair_of = [&](int q, int& f, int& b, int& j, bool& act) {
+        f = q / PER_F;
+        const int dlt = q - f * PER_F + 1;
+        const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
+        const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
+        act = !(EVEN && 2 * dlt == N1 && 2 * af >= N1);
+        b = af + dlt < N1 ? af + dlt : af + dlt - N1;
+        j = tl + (b - af) * stride;
+    };
+    // S_m[i, j] of pair q (zeros for m < f in the table), requested one pair ahead
+    auto load_s = [&](int q, double* sv) {
+        int f, b, j;
+        bool act;
+        pair_of(q, f, b, j, act);
+        const double* sg = st.SG + ((f * N1 + b) * D) * NQ + tl;
+#pragma unroll
+        for (int m = 0; m < D; ++m) sv[m] = sg[m * NQ];
+    };
+    auto eval = [&](int q, const double* sv) {
+        int f, b, j;
+        bool act;
+        pair_of(q, f, b, j, act);
+        if (!act) return;
+        double hj[NH], av[D], gb[NC], ubj[NC];
+#pragma unroll
+        for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
+#pragma unroll
+        for (int n = 0; n < D; ++n) av[n] = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            if (m < f) continue;
+#pragma unroll
+            for (int n = 0; n < D; ++n) av[n] += sv[m] * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
+        }
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
+            ubj[e] = 0.0;
+        }
+        // the pair's term enters r_q[i] as -c and r_q[j] as +c: one evaluation
+        // pulls (rbar_j - rbar_i) back to both states; node j's share goes
+        // through the row accumulator that phase D adds
+        A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
+    };
+    // with two threads per node, the half-team's pairs with compile-time pair indices
+    // (half is uniform over a wavefront), the S values of each pair requested one pair ahead
+    auto sweep = [&](auto hc) {
+        constexpr int H = decltype(hc)::value;
+        constexpr int Q0 = H * QH, Q1 = (H + 1) * QH < NPART ? (H + 1) * QH : NPART;
+        double sv[2][D];
+        load_s(Q0, sv[0]);
+#pragma unroll
+        for (int q = Q0; q < Q1; ++q) {
+            if (q + 1 < Q1) load_s(q + 1, sv[(q + 1 - Q0) & 1]);
+            eval(q, sv[(q - Q0) & 1]);
+        }
+    };
     if (half == 0 && tl < NQ) {
         const double* uqk = p.uq + (size_t)k * NQ * NC;
         const double* Lk = p.Lam + (size_t)k * NQ * D * D;
@@ -242,6 +315,17 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         for (int v = 0; v < NH; ++v) sh[v * NQ + tl] = hi[v];
 #pragma unroll
         for (int mn = 0; mn < D * D; ++mn) sL[mn * NQ + tl] = Lk[mn * NQ + tl];
+    }
+    if constexpr (EARLY) {
+        if (half == 0) {  // the wavefront's own LDS writes (hoisted states, metric, rbar, zeros)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (tl < NQ) {
+                sweep(VjpHalf<0>{});
+                sweep(VjpHalf<1>{});
+            }
+        }
     }
     // the facet nodes' interface adjoints (the longer chain) on the last half-team, so
     // that with two threads per node they run beside the volume hoisting
@@ -299,69 +383,12 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
     }
     if (tl < NQ) {  // the node's metric and rbar are read from LDS (fewer live registers)
-        const int a_last = tl % N1, a_mid = (tl / N1) % N1, a_first = tl / (N1 * N1);
-        auto pair_of = [&](int q, int& f, int& b, int& j, bool& act) {
-            f = q / PER_F;
-            const int dlt = q - f * PER_F + 1;
-            const int stride = D == 3 ? (f == 0 ? N1 * N1 : (f == 1 ? N1 : 1)) : (f == 0 ? N1 : 1);
-            const int af = D == 3 ? (f == 0 ? a_first : (f == 1 ? a_mid : a_last)) : (f == 0 ? a_mid : a_last);
-            act = !(EVEN && 2 * dlt == N1 && 2 * af >= N1);
-            b = af + dlt < N1 ? af + dlt : af + dlt - N1;
-            j = tl + (b - af) * stride;
-        };
-        // S_m[i, j] of pair q (zeros for m < f in the table), requested one pair ahead
-        auto load_s = [&](int q, double* sv) {
-            int f, b, j;
-            bool act;
-            pair_of(q, f, b, j, act);
-            const double* sg = st.SG + ((f * N1 + b) * D) * NQ + tl;
-#pragma unroll
-            for (int m = 0; m < D; ++m) sv[m] = sg[m * NQ];
-        };
-        auto eval = [&](int q, const double* sv) {
-            int f, b, j;
-            bool act;
-            pair_of(q, f, b, j, act);
-            if (!act) return;
-            double hj[NH], av[D], gb[NC], ubj[NC];
-#pragma unroll
-            for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
-#pragma unroll
-            for (int n = 0; n < D; ++n) av[n] = 0.0;
-#pragma unroll
-            for (int m = 0; m < D; ++m) {
-                if (m < f) continue;
-#pragma unroll
-                for (int n = 0; n < D; ++n) av[n] += sv[m] * (sL[(m + D * n) * NQ + tl] + sL[(m + D * n) * NQ + j]);
-            }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) {
-                gb[e] = srb[e * NQ + j] - srb[e * NQ + tl];
-                ubj[e] = 0.0;
-            }
-            // the pair's term enters r_q[i] as -c and r_q[j] as +c: one evaluation
-            // pulls (rbar_j - rbar_i) back to both states; node j's share goes
-            // through the row accumulator that phase D adds
-            A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
-#pragma unroll
-            for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
-        };
         if constexpr (!DET) {
             // with two threads per node, the half-team's pairs with compile-time pair indices
             // (half is uniform over a wavefront), the S values of each pair requested one
             // pair ahead
-            auto sweep = [&](auto hc) {
-                constexpr int H = decltype(hc)::value;
-                constexpr int Q0 = H * QH, Q1 = (H + 1) * QH < NPART ? (H + 1) * QH : NPART;
-                double sv[2][D];
-                load_s(Q0, sv[0]);
-#pragma unroll
-                for (int q = Q0; q < Q1; ++q) {
-                    if (q + 1 < Q1) load_s(q + 1, sv[(q + 1 - Q0) & 1]);
-                    eval(q, sv[(q - Q0) & 1]);
-                }
-            };
-            if constexpr (SPLIT > 1) {
+            if constexpr (EARLY) {  // swept in phase A
+            } else if constexpr (SPLIT > 1) {
                 if (half == 0)
                     sweep(VjpHalf<0>{});
                 else
@@ -534,7 +561,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
             rv[w] = w < tp.wrt ? tp.rt_val[w * NQ + tl] : 0.0;
         }
     }
-    if constexpr (SPLIT > 1) {
+    if constexpr (SPLIT > 1 && !EARLY) {
         __syncthreads();  // the pairs are done with the metric array, which takes the partial rows
         if (half == 1 && tl < NQ) {
 #pragma unroll
@@ -544,7 +571,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
     __syncthreads();
     // ---- phase D: cotangents of the entropy variables, w = c2e(u): volume and facet
     if (half == 0 && tl < NQ) {
-        if (SPLIT > 1) {
+        if (SPLIT > 1 && !EARLY) {
 #pragma unroll
             for (int e = 0; e < NC; ++e) ubq[e] += sub[e * NQ + tl];
         }
