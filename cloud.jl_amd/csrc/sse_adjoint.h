@@ -54,54 +54,34 @@ __device__ __forceinline__ void inv_logmean_grad(double x, double y, double& dx,
     }
 }
 
-// logmean(x, y) and its partial derivatives, from x, y and their logarithms
-// (no logarithm evaluated); the branch test is the forward one.
-__device__ __forceinline__ double logmean_vg(double x, double y, double lx, double ly, double ix, double iy,
-                                             double& dx, double& dy) {
-    // the forward's branch test f2 = ((x - y) / (x + y))^2 < 1e-4, without the division
-    const double dxy = x - y, sxy = x + y;
-    if (dxy * dxy < 1.0e-4 * (sxy * sxy)) {
-        const double r = 1.0 / sxy;
-        const double q = dxy * r;
-        const double f2 = q * q;
-        const double den = 210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0));
-        const double dden = 70.0 + f2 * (84.0 + f2 * 90.0);
-        const double g = 105.0 / den;
-        const double slope = -(x + y) * g * g * (1.0 / 105.0) * dden * 2.0 * q * 2.0 * r * r;
-        dx = g + slope * y;
-        dy = g - slope * x;
-        return (x + y) * g;
+// logmean(x, y) = N / Dn from x, y, their logarithms and reciprocals (no logarithm or
+// division evaluated), with the numerators of its partial derivatives:
+//   d logmean / dx = gx / Dn^2,  d logmean / dy = gy / Dn^2.
+// The branch is the forward's (logmean_frac: |y - x| < 1e-2 (x + y)):
+//   series: N = 105 s^7, Dn = 210 s^6 + 70 d^2 s^4 + 42 d^4 s^2 + 30 d^6 (s = x + y,
+//           d = y - x; the reference's series scaled by s^6), g = N_s Dn - N (Dn_s -+ Dn_d);
+//   log:    N = d, Dn = log y - log x, gx = -Dn + d / x, gy = Dn - d / y.
+// The inverse logarithmic mean is Dn / N, with derivatives -gx / N^2 and -gy / N^2.
+__device__ __forceinline__ void logmean_frac_vg(double x, double y, double lx, double ly, double ix, double iy,
+                                                double& N, double& Dn, double& gx, double& gy) {
+    const double s = x + y, d = y - x;
+    const double s2 = s * s, d2 = d * d;
+    if (d2 < 1.0e-4 * s2) {
+        const double s4 = s2 * s2, d4 = d2 * d2;
+        N = 105.0 * s * s4 * s2;
+        Dn = ((210.0 * s2 + 70.0 * d2) * s2 + 42.0 * d4) * s2 + 30.0 * d4 * d2;
+        const double Ns_s = 735.0 * s4 * s2;
+        const double Ds_s = ((1260.0 * s2 + 280.0 * d2) * s2 + 84.0 * d4) * s;
+        const double Ds_d = ((140.0 * s2 + 168.0 * d2) * s2 + 180.0 * d4) * d;
+        gx = Ns_s * Dn - N * (Ds_s - Ds_d);
+        gy = Ns_s * Dn - N * (Ds_s + Ds_d);
+    } else {
+        const double L = ly - lx;
+        N = d;
+        Dn = L;
+        gx = -L + d * ix;
+        gy = L - d * iy;
     }
-    const double L = ly - lx;
-    const double iL = 1.0 / L;
-    const double lm = (y - x) * iL;
-    dx = (-1.0 + (y - x) * ix * iL) * iL;
-    dy = (1.0 - (y - x) * iy * iL) * iL;
-    return lm;
-}
-
-// inv_logmean(x, y) and its partial derivatives, from x, y and their logarithms
-__device__ __forceinline__ double inv_logmean_vg(double x, double y, double lx, double ly, double ix, double iy,
-                                                 double& dx, double& dy) {
-    const double dxy = x - y, sxy = x + y;
-    if (dxy * dxy < 1.0e-4 * (sxy * sxy)) {
-        const double r = 1.0 / sxy;
-        const double q = dxy * r;
-        const double f2 = q * q;
-        const double den = 210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0));
-        const double dden = 70.0 + f2 * (84.0 + f2 * 90.0);
-        const double c = r * (1.0 / 105.0);
-        const double a = dden * 2.0 * q * c * 2.0 * r * r;  // times y (x) gives the q-chain term
-        const double b = -den * c * r;
-        dx = a * y + b;
-        dy = -a * x + b;
-        return den * c;
-    }
-    const double L = ly - lx;
-    const double r = 1.0 / (y - x);
-    dx = (-ix + L * r) * r;
-    dy = (iy - L * r) * r;
-    return L * r;
 }
 
 template <int D>
@@ -151,9 +131,18 @@ struct EulerAdj {
             c.avR += a[m] * vr;
         }
         const double pL = hL[H_P], pR = hR[H_P];
-        const double rho_avg = logmean_vg(hL[H_RHO], hR[H_RHO], hL[H_LR], hR[H_LR], hL[H_IR], hR[H_IR], c.drx, c.dry);
-        const double ilm = inv_logmean_vg(hL[H_B], hR[H_B], hL[H_LB], hR[H_LB], pL * hL[H_IR], pR * hR[H_IR], c.dbx,
-                                          c.dby);
+        // rho_avg = logmean(rho) = N1 / D1 and ilm = 1 / logmean(beta) = D2 / N2 with their
+        // derivatives, from one reciprocal of D1 N2 (as in the forward flux)
+        double N1, D1, gx1, gy1, N2, D2, gx2, gy2;
+        logmean_frac_vg(hL[H_RHO], hR[H_RHO], hL[H_LR], hR[H_LR], hL[H_IR], hR[H_IR], N1, D1, gx1, gy1);
+        logmean_frac_vg(hL[H_B], hR[H_B], hL[H_LB], hR[H_LB], pL * hL[H_IR], pR * hR[H_IR], N2, D2, gx2, gy2);
+        const double rr = rcp_refined(D1 * N2);
+        const double iD1 = N2 * rr, iN2 = D1 * rr;
+        const double rho_avg = N1 * iD1, ilm = D2 * iN2;
+        c.drx = gx1 * iD1 * iD1;
+        c.dry = gy1 * iD1 * iD1;
+        c.dbx = -gx2 * iN2 * iN2;
+        c.dby = -gy2 * iN2 * iN2;
         const double Cc = 0.5 * vv + P.inv_gm1 * ilm;
         c.f = rho_avg * av;
         double fb = Gb[0] + Gb[D + 1] * Cc;
