@@ -42,8 +42,13 @@ from collections import defaultdict
 SLOTS = {
     "nodal_values": re.compile(r"k_nodal_values"),
     "time_derivative": re.compile(r"k_time_derivative"),
+    "vjp_v1": re.compile(r"k_vjp_v1"),
+    "vjp_v2": re.compile(r"k_vjp_v2"),
+    "vjp_v3": re.compile(r"k_vjp_v3"),
     "copy_probe": re.compile(r"elementwise_kernel"),
 }
+# the slots the counter tables show, in this order
+TABLE_SLOTS = ("nodal_values", "time_derivative", "vjp_v1", "vjp_v2", "vjp_v3")
 
 
 def short_name(kernel_name: str) -> str:
@@ -173,7 +178,7 @@ def all_counters(directories):
 
 
 def format_counters(data) -> str:
-    slots = [s for s in ("nodal_values", "time_derivative") if s in data]
+    slots = [s for s in TABLE_SLOTS if s in data]
     names = sorted({c for s in slots for c in data[s]})
     lines = [f"{'counter (mean per dispatch)':34s}" + "".join(f"{s:>18s}" for s in slots)]
     for c in names:
@@ -243,7 +248,7 @@ def slot_means(trace_dir: str):
 
 
 def format_derived(d) -> str:
-    slots = [s for s in ("nodal_values", "time_derivative") if s in d]
+    slots = [s for s in TABLE_SLOTS if s in d]
     names = sorted({k for s in slots for k in d[s]})
     lines = [f"{'derived':40s}" + "".join(f"{s:>18s}" for s in slots)]
     for k in names:
