@@ -532,6 +532,46 @@ def test_rk_adjoint_recompute_beside_reverse_sweep_equals_one_stream(monkeypatch
     assert J1 == J0 and l1 == l0 and np.array_equal(g1, g0)
 
 
+@pytest.mark.parametrize("make", [lambda: density_wave_3d_config(M=2, p=3, ic_strength=0.2),
+                                  lambda: density_wave_3d_config(M=2, p=4, ic_strength=0.2),
+                                  lambda: density_wave_3d_config(M=1, p=5, ic_strength=0.2),
+                                  lambda: vortex_2d_config(M=4, p=4)],
+                         ids=["tet_p3", "tet_p4", "tet_p5", "tri_p4"])
+def test_kept_mass_matrices_match_the_transforms_and_the_oracle(monkeypatch, make):
+    # each element's kept weight-adjusted inverse V' diag(W/J_k) V (the default) against the
+    # sum-factorised V, W/J, V' (SSE_MASS_MATRICES=0): residual and VJP agree to round-off
+    # (the default mode's LDS atomics add ~1e-14 of their own), and the residual matches
+    # the oracle
+    from sse import semi_discrete_residual
+    from sse.adjoint import residual_vjp
+    from sse.solvers import Solver
+    from sse.time_integration import to_device, to_host
+    torch = _gpu()
+    law, sd, ic, form, ode = make()[:5]
+    u = _perturbed(ode)
+    rng = np.random.default_rng(5)
+    ybar = np.asfortranarray(rng.standard_normal(u.shape))
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SSE_MASS_MATRICES", flag)
+        s = Solver(law, sd, form)
+        try:
+            ud = to_device(u)
+            dd = torch.empty_like(ud)
+            semi_discrete_residual(dd, ud, s)
+            torch.cuda.synchronize()
+            r = to_host(dd)
+            ub, lam = residual_vjp(s, u, ybar)
+            out[flag] = (r, ub, lam)
+        finally:
+            s.close()
+    ref = O.residual(ode.p.arrays, u)
+    assert _rel(out["1"][0], ref) <= RES_TOL
+    assert _rel(out["1"][0], out["0"][0]) <= RES_TOL
+    assert _rel(out["1"][1], out["0"][1]) <= 1e-11
+    assert abs(out["1"][2] - out["0"][2]) <= 1e-11 * max(abs(out["0"][2]), 1e-300)
+
+
 def test_rk_steps_graph_replay_matches_eager_and_oracle(monkeypatch):
     # sse_rk_steps replays one captured step; a new state pointer or a new dt recaptures
     from sse.time_integration import rk_steps, to_device, to_host
