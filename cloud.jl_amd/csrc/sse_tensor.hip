@@ -148,19 +148,20 @@ k_nodal_values_tensor(TensorParams tp, const double* __restrict__ u) {
     // u_f = e2c(R w_q). A row's entries are taken four at a time, their loads issued
     // together (entries past the table width or the row's end are -1; a group whose first
     // entry is -1 ends the row)
+    constexpr int G = 8;  // entries per load group
     auto row_sum = [&](int j, int w_begin, int w_end, double* wb) {
-        for (int w0 = w_begin; w0 < w_end; w0 += 4) {
-            int ii[4];
-            double rv[4];
+        for (int w0 = w_begin; w0 < w_end; w0 += G) {
+            int ii[G];
+            double rv[G];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < G; ++q) {
                 const bool in = w0 + q < w_end;
                 ii[q] = in ? tp.r_idx[(w0 + q) * NF + j] : -1;
                 rv[q] = in ? tp.r_val[(w0 + q) * NF + j] : 0.0;
             }
             if (ii[0] < 0) break;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < G; ++q) {
                 if (ii[q] < 0) continue;
 #pragma unroll
                 for (int e = 0; e < NC; ++e) wb[e] += rv[q] * sa[e * NQ + ii[q]];
