@@ -158,37 +158,53 @@ struct EulerAdj {
         return c;
     }
 
-    // the side-specific rest: adds the pull-back into the conservative cotangent ub of
-    // the left (`left`) or the right state
-    __device__ static void ec_dot_adj_side_h(const LawParams& P, const double* hL, const double* hR, const double* a,
-                                             const double* Gb, const EcAdjShared& c, bool left, double* ub) {
+    // the side-specific rest in primitive variables: adds the pull-back into the (rho, v, p)
+    // cotangent ubp = [rho, v_0..v_{D-1}, p] of the left (`left`) or the right state
+    __device__ static void ec_dot_adj_side_prim(const LawParams& P, const double* hL, const double* hR,
+                                                const double* a, const double* Gb, const EcAdjShared& c, bool left,
+                                                double* ubp) {
         const double* hs = left ? hL : hR;
         const double* ho = left ? hR : hL;
         const double po = left ? hR[H_P] : hL[H_P];
         const double avo = left ? c.avR : c.avL;
-        double vb[D];
 #pragma unroll
         for (int m = 0; m < D; ++m)
-            vb[m] = 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * c.f + c.avb * a[m]) + 0.5 * c.Ccb * ho[H_V + m];
-        double pb = 0.5 * Gb[D + 1] * avo + 0.5 * c.pavgb;
+            ubp[1 + m] += 0.5 * Gb[D + 1] * po * a[m] + 0.5 * (Gb[m + 1] * c.f + c.avb * a[m]) + 0.5 * c.Ccb * ho[H_V + m];
         const double bb = P.inv_gm1 * c.Ccb * (left ? c.dbx : c.dby);
         const double ips = hs[H_IP];
-        double rb = c.rhoavgb * (left ? c.drx : c.dry) + bb * ips;
-        pb += -bb * hs[H_B] * ips;
-        // (rho, v, p) -> (rho, m, E), p = gm1 (E - 1/2 rho |v|^2), v = m / rho
+        ubp[D + 1] += 0.5 * Gb[D + 1] * avo + 0.5 * c.pavgb - bb * hs[H_B] * ips;
+        ubp[0] += c.rhoavgb * (left ? c.drx : c.dry) + bb * ips;
+    }
+
+    // (rho, v, p) cotangent ubp of a state with hoisted quantities hs -> conservative
+    // cotangent, added into ub: p = gm1 (E - 1/2 rho |v|^2), v = m / rho. Linear in ubp, so
+    // a node's cotangents from several terms can be summed first and converted once.
+    __device__ static void prim_cot_to_cons(const LawParams& P, const double* hs, const double* ubp, double* ub) {
         const double rho = hs[H_RHO], ir = hs[H_IR];
+        const double pb = ubp[D + 1];
         double s = 0.0, vdot = 0.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) s += hs[H_V + m] * hs[H_V + m];
         ub[D + 1] += P.gm1 * pb;
-        rb += -0.5 * P.gm1 * s * pb;
+        const double rb = ubp[0] - 0.5 * P.gm1 * s * pb;
 #pragma unroll
         for (int m = 0; m < D; ++m) {
-            const double vbt = vb[m] - P.gm1 * rho * hs[H_V + m] * pb;
+            const double vbt = ubp[1 + m] - P.gm1 * rho * hs[H_V + m] * pb;
             ub[m + 1] += vbt * ir;
             vdot += vbt * hs[H_V + m];
         }
         ub[0] += rb - vdot * ir;
+    }
+
+    // the side-specific rest: adds the pull-back into the conservative cotangent ub of
+    // the left (`left`) or the right state
+    __device__ static void ec_dot_adj_side_h(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                             const double* Gb, const EcAdjShared& c, bool left, double* ub) {
+        double ubp[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ubp[e] = 0.0;
+        ec_dot_adj_side_prim(P, hL, hR, a, Gb, c, left, ubp);
+        prim_cot_to_cons(P, left ? hL : hR, ubp, ub);
     }
 
     // ec_dot_adj_side from hoisted states: adds the pull-back of Gb through
@@ -208,6 +224,15 @@ struct EulerAdj {
         const EcAdjShared c = ec_dot_adj_shared(P, hL, hR, a, Gb, Vb);
         ec_dot_adj_side_h(P, hL, hR, a, Gb, c, true, ubL);
         ec_dot_adj_side_h(P, hL, hR, a, Gb, c, false, ubR);
+    }
+
+    // ec_dot_adj_h2 into (rho, v, p) cotangents (prim_cot_to_cons converts a node's sum)
+    __device__ static void ec_dot_adj_h2p(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                          const double* Gb, double* ubpL, double* ubpR) {
+        double Vb[D];
+        const EcAdjShared c = ec_dot_adj_shared(P, hL, hR, a, Gb, Vb);
+        ec_dot_adj_side_prim(P, hL, hR, a, Gb, c, true, ubpL);
+        ec_dot_adj_side_prim(P, hL, hR, a, Gb, c, false, ubpR);
     }
 
     // (rho, v, p) cotangents -> conservative cotangents, added into ub

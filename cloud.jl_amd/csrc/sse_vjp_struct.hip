@@ -283,7 +283,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         // the pair's term enters r_q[i] as -c and r_q[j] as +c: one evaluation
         // pulls (rbar_j - rbar_i) back to both states; node j's share goes
         // through the row accumulator that phase D adds
-        A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+        A::ec_dot_adj_h2p(p.law, hi, hj, av, gb, ubq, ubj);
 #pragma unroll
         for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + j], ubj[e]);
     };
@@ -444,7 +444,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
                     }
 #pragma unroll
                     for (int e = 0; e < NC; ++e) gb[e] = srb[e * NQ + jd] - srb[e * NQ + tl];
-                    A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, ubj);
+                    A::ec_dot_adj_h2p(p.law, hi, hj, av, gb, ubq, ubj);
                 }
             }
 #pragma unroll 1
@@ -485,7 +485,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
                     gb[e] = -(srb[e * NQ + tl] + fbar[(size_t)k * NF * NC + e * NF + j]);
                     dummy[e] = 0.0;
                 }
-                A::ec_dot_adj_h2(p.law, hi, hj, av, gb, ubq, dummy);
+                A::ec_dot_adj_h2p(p.law, hi, hj, av, gb, ubq, dummy);
             }
         }
     }
@@ -535,7 +535,7 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
                 gb[e] = -(srb[e * NQ + i] + fbj[e]);
                 uv[e] = 0.0;
             }
-            A::ec_dot_adj_h2(p.law, hv, hj, av, gb, uv, part);
+            A::ec_dot_adj_h2p(p.law, hv, hj, av, gb, uv, part);
             if constexpr (!DET) {
 #pragma unroll
                 for (int e = 0; e < NC; ++e) atomicAdd(&srow[e * NQ + i], uv[e]);
@@ -590,18 +590,21 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
 #pragma unroll
                 for (int e = 0; e < NC; ++e) ubf[e] += srow[e * MAXITEM + c];
             }
-#pragma unroll
-            for (int e = 0; e < NC; ++e) ubf[e] += swf[e * NF + tl];
         } else {
 #pragma unroll
-            for (int e = 0; e < NC; ++e) ubf[e] = swf[e * NF + tl] + sfac[e * NF + tl];
+            for (int e = 0; e < NC; ++e) ubf[e] = sfac[e * NF + tl];
         }
 #pragma unroll
         for (int v = 0; v < NH; ++v) hf[v] = shf[v * NF + tl];
+        // the C-pair sums are (rho, v, p) cotangents: to conservative, beside the interface part
+        double ubc[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) ubc[e] = swf[e * NF + tl];
+        A::prim_cot_to_cons(p.law, hf, ubf, ubc);
         double wb[NC];
 #pragma unroll
         for (int e = 0; e < NC; ++e) wb[e] = 0.0;
-        A::e2c_adj_prim(p.law, hf[A::H_RHO], hf + A::H_V, hf[A::H_P], ubf, wb);
+        A::e2c_adj_prim(p.law, hf[A::H_RHO], hf + A::H_V, hf[A::H_P], ubc, wb);
 #pragma unroll
         for (int e = 0; e < NC; ++e) swf[e * NF + tl] = wb[e];
     }
@@ -610,9 +613,15 @@ k_vjp_v2_tensor(TensorParams tp, StructTables st, double* __restrict__ rbar, con
         double wb[NC];
 #pragma unroll
         for (int v = 0; v < NH; ++v) hi[v] = sh[v * NQ + tl];
+        // the pair sums are (rho, v, p) cotangents: to conservative once per node
+        double ubc[NC];
 #pragma unroll
-        for (int e = 0; e < NC; ++e) wb[e] = 0.0;
-        A::e2c_adj_prim(p.law, hi[A::H_RHO], hi + A::H_V, hi[A::H_P], ubq, wb);
+        for (int e = 0; e < NC; ++e) {
+            ubc[e] = 0.0;
+            wb[e] = 0.0;
+        }
+        A::prim_cot_to_cons(p.law, hi, ubq, ubc);
+        A::e2c_adj_prim(p.law, hi[A::H_RHO], hi + A::H_V, hi[A::H_P], ubc, wb);
         // w_q' cotangent += R' (w_f cotangent)
 #pragma unroll
         for (int w = 0; w < MAXR; ++w) {
