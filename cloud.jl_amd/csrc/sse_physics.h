@@ -210,7 +210,8 @@ __device__ __forceinline__ double inv_logmean_h(double x, double y, double lx, d
     return num / den;
 }
 
-// The logarithmic mean as a fraction num/den, branch-free:
+// The logarithmic mean as a fraction num/den (a branch, so that a wavefront none of whose
+// pairs needs the series skips its polynomial):
 //   |y - x| < 1e-2 (x + y): the reference's series 105 s / (210 + 70 f2 + 42 f2^2 + 30 f2^3),
 //                           f2 = d^2 / s^2, with numerator and denominator scaled by s^6
 //                           (s = x + y, d = y - x), so no divide is needed for f2;
@@ -219,11 +220,13 @@ __device__ __forceinline__ double inv_logmean_h(double x, double y, double lx, d
 __device__ __forceinline__ void logmean_frac(double x, double y, double lx, double ly, double& num, double& den) {
     const double s = x + y, d = y - x;
     const double s2 = s * s, d2 = d * d;
-    const bool ser = d2 < 1.0e-4 * s2;
-    const double s6 = s2 * s2 * s2;
-    const double poly = ((210.0 * s2 + 70.0 * d2) * s2 + 42.0 * d2 * d2) * s2 + 30.0 * d2 * d2 * d2;
-    num = ser ? 105.0 * s * s6 : d;
-    den = ser ? poly : (ly - lx);
+    num = d;
+    den = ly - lx;
+    if (d2 < 1.0e-4 * s2) {
+        const double s6 = s2 * s2 * s2;
+        num = 105.0 * s * s6;
+        den = ((210.0 * s2 + 70.0 * d2) * s2 + 42.0 * d2 * d2) * s2 + 30.0 * d2 * d2 * d2;
+    }
 }
 
 // 1/x for a nonzero normal x: the hardware reciprocal estimate (v_rcp_f64) refined by
@@ -240,9 +243,13 @@ __device__ __forceinline__ double rcp_refined(double x) {
     return fma(r, e, r);
 }
 
+// ec_dot with the contraction vector given halved, ah = a / 2: the factors 1/2 of the
+// averages then fall on ah (c_0 = rho_avg (ah . (v_L + v_R)), a_m p_avg = ah_m (p_L + p_R),
+// (p_L (a . v_R) + p_R (a . v_L)) / 2 = p_L (ah . v_R) + p_R (ah . v_L)), five operations
+// fewer per flux; scaling by 1/2 is exact, so the products are ec_dot's
 template <int D>
-__device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, const double* hR, const double* a,
-                                       double* c) {
+__device__ __forceinline__ void ec_dot_h(const LawParams& P, const double* hL, const double* hR, const double* ah,
+                                         double* c) {
     // rho_avg = n1/d1 (logmean of rho), ilm = d2/n2 (inverse logmean of rho/p): one reciprocal
     double n1, d1, n2, d2;
     logmean_frac(hL[0], hR[0], hL[D + 2], hR[D + 2], n1, d1);
@@ -250,21 +257,31 @@ __device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, con
     const double rcp = rcp_refined(d1 * n2);
     const double rho_avg = n1 * n2 * rcp;
     const double ilm = d2 * d1 * rcp;
-    double vv = 0.0, avL = 0.0, avR = 0.0;
+    double vv = 0.0, avL = 0.0, avR = 0.0;  // avL = (a . v_L) / 2
 #pragma unroll
     for (int m = 0; m < D; ++m) {
         vv += hL[1 + m] * hR[1 + m];
-        avL += a[m] * hL[1 + m];
-        avR += a[m] * hR[1 + m];
+        avL += ah[m] * hL[1 + m];
+        avR += ah[m] * hR[1 + m];
     }
     const double pL = hL[D + 1], pR = hR[D + 1];
-    const double p_avg = 0.5 * (pL + pR);
+    const double ps = pL + pR;
     const double Cc = 0.5 * vv + P.inv_gm1 * ilm;
-    const double c0 = rho_avg * (0.5 * (avL + avR));
+    const double c0 = rho_avg * (avL + avR);
+    const double c0h = 0.5 * c0;
     c[0] = c0;
 #pragma unroll
-    for (int m = 0; m < D; ++m) c[1 + m] = c0 * (0.5 * (hL[1 + m] + hR[1 + m])) + a[m] * p_avg;
-    c[D + 1] = c0 * Cc + 0.5 * (pL * avR + pR * avL);
+    for (int m = 0; m < D; ++m) c[1 + m] = c0h * (hL[1 + m] + hR[1 + m]) + ah[m] * ps;
+    c[D + 1] = c0 * Cc + (pL * avR + pR * avL);
+}
+
+template <int D>
+__device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, const double* hR, const double* a,
+                                       double* c) {
+    double ah[D];
+#pragma unroll
+    for (int m = 0; m < D; ++m) ah[m] = 0.5 * a[m];
+    ec_dot_h<D>(P, hL, hR, ah, c);
 }
 
 // conservative_to_entropy! / entropy_to_conservative! (euler_navierstokes.jl:100-131)

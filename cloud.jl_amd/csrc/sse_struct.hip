@@ -240,7 +240,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 int f, b, j;
                 bool act;
                 pair_of(q, f, b, j, act);
-                const double* sg = st.SG + ((f * N1 + b) * D) * NQ + tl;
+                const double* sg = st.SGh + ((f * N1 + b) * D) * NQ + tl;
 #pragma unroll
                 for (int m = 0; m < D; ++m) sv[m] = sg[m * NQ];
             };
@@ -252,7 +252,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 double hj[NH], av[D], c[NC];
 #pragma unroll
                 for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
-                // a_n = sum_{m >= f} S_m[i,j] (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
+                // a_n / 2 = sum_{m >= f} (S_m[i,j] / 2) (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
 #pragma unroll
                 for (int n = 0; n < D; ++n) av[n] = 0.0;
 #pragma unroll
@@ -261,7 +261,7 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
 #pragma unroll
                     for (int n = 0; n < D; ++n) av[n] += sv[m] * (Li[m + D * n] + sL[(m + D * n) * NQ + j]);
                 }
-                ec_dot<D>(p.law, hi, hj, av, c);
+                ec_dot_h<D>(p.law, hi, hj, av, c);
                 // S antisymmetric, F symmetric: node j's row receives +c, through
                 // the row accumulator that phase D subtracts
 #pragma unroll
@@ -320,11 +320,11 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
 #pragma unroll
                         for (int m = 0; m < D; ++m) {
                             if (m < f) continue;
-                            const double sv = st.SG[((f * N1 + b) * D + m) * NQ + tl];
+                            const double sv = st.SGh[((f * N1 + b) * D + m) * NQ + tl];
 #pragma unroll
                             for (int n = 0; n < D; ++n) av[n] += sv * (Li[m + D * n] + sL[(m + D * n) * NQ + jd]);
                         }
-                        ec_dot<D>(p.law, hi, hj, av, c);
+                        ec_dot_h<D>(p.law, hi, hj, av, c);
 #pragma unroll
                         for (int e = 0; e < NC; ++e) r[e] -= c[e];
                     }
@@ -360,9 +360,9 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                         double s = snJ[n * NF + j];
 #pragma unroll
                         for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + tl] * sfn[l + D * fc];
-                        av[n] = 0.5 * cv * s;
+                        av[n] = 0.25 * cv * s;  // half the contraction vector
                     }
-                    ec_dot<D>(p.law, hi, hj, av, c);
+                    ec_dot_h<D>(p.law, hi, hj, av, c);
 #pragma unroll
                     for (int e = 0; e < NC; ++e) r[e] -= c[e];
                 }
@@ -401,9 +401,9 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                     double s = nj[n];
 #pragma unroll
                     for (int l = 0; l < D; ++l) s += sL[(l + D * n) * NQ + i] * fnl[l];
-                    av[n] = 0.5 * cvw * s;
+                    av[n] = 0.25 * cvw * s;  // half the contraction vector
                 }
-                ec_dot<D>(p.law, hv, hj, av, c);
+                ec_dot_h<D>(p.law, hv, hj, av, c);
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
                     part[e] += c[e];
