@@ -770,7 +770,7 @@ int build_struct(sse_handle* h, const sse_desc* D) {
     S.det = D->deterministic ? 1 : 0;
     int rc;
     std::vector<double> SGh(SG.size());
-    for (size_t t = 0; t < SG.size(); ++t) SGh[t] = 0.5 * SG[t];
+    for (size_t t = 0; t < SG.size(); ++t) SGh[t] = -0.5 * SG[t];
     if ((rc = upload(h, SG, &S.SG)) || (rc = upload(h, SGh, &S.SGh)) || (rc = upload(h, ci_i, &S.ci_i)) || (rc = upload(h, ci_v, &S.ci_v)) ||
         (rc = upload(h, ci_j, &S.ci_j)) || (rc = upload(h, ci_first, &S.ci_first)) || (rc = upload(h, cr_j, &S.cr_j)) ||
         (rc = upload(h, cr_v, &S.cr_v)))

@@ -252,7 +252,8 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 double hj[NH], av[D], c[NC];
 #pragma unroll
                 for (int v = 0; v < NH; ++v) hj[v] = sh[v * NQ + j];
-                // a_n / 2 = sum_{m >= f} (S_m[i,j] / 2) (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for m < f)
+                // -a_n / 2 = sum_{m >= f} (-S_m[i,j] / 2) (Lambda_i + Lambda_j)[m,n]  (S_m = 0 for
+                // m < f): the flux is linear in a, so c comes out negated, as both rows take it
 #pragma unroll
                 for (int n = 0; n < D; ++n) av[n] = 0.0;
 #pragma unroll
@@ -266,8 +267,8 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
                 // the row accumulator that phase D subtracts
 #pragma unroll
                 for (int e = 0; e < NC; ++e) {
-                    r[e] -= c[e];
-                    atomicAdd(&srow[e * NQ + j], -c[e]);
+                    r[e] += c[e];
+                    atomicAdd(&srow[e * NQ + j], c[e]);
                 }
             };
             if constexpr (!DET) {
@@ -324,16 +325,16 @@ k_time_derivative_struct(TensorParams tp, StructTables st, double* __restrict__ 
 #pragma unroll
                             for (int n = 0; n < D; ++n) av[n] += sv * (Li[m + D * n] + sL[(m + D * n) * NQ + jd]);
                         }
-                        ec_dot_h<D>(p.law, hi, hj, av, c);
+                        ec_dot_h<D>(p.law, hi, hj, av, c);  // negated, as above
 #pragma unroll
-                        for (int e = 0; e < NC; ++e) r[e] -= c[e];
+                        for (int e = 0; e < NC; ++e) r[e] += c[e];
                     }
                 }
 #pragma unroll 1
                 for (int h = 0; h < SPLIT; ++h) {
                     if (has && act && half == h) {
 #pragma unroll
-                        for (int e = 0; e < NC; ++e) srow[e * NQ + jd] -= c[e];
+                        for (int e = 0; e < NC; ++e) srow[e * NQ + jd] += c[e];
                     }
                     __syncthreads();
                 }

@@ -65,7 +65,7 @@ SSE_TUNE(ELEM_TRI, 4, 1, 64, 2, 3, 64, 4)
 // Line-structured operator tables (host-built from the descriptor's S, C and R).
 struct StructTables {
     const double* SG;    // D x N1 x D x NQ: S_m[i, j(i, f, b)], index ((f * N1 + b) * D + m) * NQ + i
-    const double* SGh;   // SG / 2 (the forward flux takes half its contraction vector)
+    const double* SGh;   // -SG / 2: the forward's volume pairs contract with -a / 2
     const int* ci_i;     // CH x MAXITEM: volume nodes of each column chunk (-1 pad)
     const double* ci_v;  // CH x MAXITEM
     const int* ci_j;     // MAXITEM: facet node of each chunk (-1 = no chunk)
