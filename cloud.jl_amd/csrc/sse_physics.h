@@ -171,10 +171,25 @@ struct Physics<D, LAW_EULER> {
 //   c_{D+1} = C c_0 + (p_L (a . v_R) + p_R (a . v_L)) / 2
 // which is the tensor of euler_navierstokes.jl:171-195 contracted with a.
 // ---------------------------------------------------------------------------
+
+// 1/x for a nonzero normal x: the hardware reciprocal estimate (v_rcp_f64) refined by
+// two Newton-Raphson steps, e = 1 - x r, r <- r + r e, each of which squares the
+// relative error (the iteration of the IEEE division sequence, without its scaling
+// and rounding fix-up, which only matter for denormal, infinite or near-overflow
+// operands). Five instructions instead of the eleven of a correctly rounded divide;
+// within two ulps of 1/x (checked on the GPU against the divide).
+__device__ __forceinline__ double rcp_refined(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 template <int D>
 __device__ __forceinline__ void hoist(const LawParams& P, const double* u, double* h) {
     const double rho = u[0];
-    const double irho = 1.0 / rho;
+    const double irho = rcp_refined(rho);
     double s = 0.0;
 #pragma unroll
     for (int m = 0; m < D; ++m) {
@@ -183,7 +198,7 @@ __device__ __forceinline__ void hoist(const LawParams& P, const double* u, doubl
         s += v * v;
     }
     const double p = P.gm1 * (u[D + 1] - 0.5 * rho * s);
-    const double beta = rho / p;
+    const double beta = rho * rcp_refined(p);
     h[0] = rho;
     h[D + 1] = p;
     h[D + 2] = log(rho);
@@ -227,20 +242,6 @@ __device__ __forceinline__ void logmean_frac(double x, double y, double lx, doub
         num = 105.0 * s * s6;
         den = ((210.0 * s2 + 70.0 * d2) * s2 + 42.0 * d2 * d2) * s2 + 30.0 * d2 * d2 * d2;
     }
-}
-
-// 1/x for a nonzero normal x: the hardware reciprocal estimate (v_rcp_f64) refined by
-// two Newton-Raphson steps, e = 1 - x r, r <- r + r e, each of which squares the
-// relative error (the iteration of the IEEE division sequence, without its scaling
-// and rounding fix-up, which only matter for denormal, infinite or near-overflow
-// operands). Five instructions instead of the eleven of a correctly rounded divide;
-// within two ulps of 1/x (checked on the GPU against the divide).
-__device__ __forceinline__ double rcp_refined(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
 }
 
 // ec_dot with the contraction vector given halved, ah = a / 2: the factors 1/2 of the
@@ -323,7 +324,9 @@ __device__ __forceinline__ bool e2c_fast(const LawParams& P, const double* win, 
     return win[D + 1] < 0.0;
 }
 
-// wave_speed from hoisted states (euler_navierstokes.jl:133-150)
+// wave_speed from hoisted states (euler_navierstokes.jl:133-150). The larger sound speed
+// is sqrt(gamma p / rho) of the side with the smaller beta = rho / p: one divide and one
+// square root instead of two each (sqrt is monotone)
 template <int D>
 __device__ __forceinline__ double wave_speed_h(const LawParams& P, const double* hi, const double* ho,
                                                const double* n) {
@@ -333,9 +336,7 @@ __device__ __forceinline__ double wave_speed_h(const LawParams& P, const double*
         vni += hi[1 + m] * n[m];
         vno += ho[1 + m] * n[m];
     }
-    const double c_in = sqrt(P.gamma * hi[D + 1] / hi[0]);
-    const double c_out = sqrt(P.gamma * ho[D + 1] / ho[0]);
-    return fmax(fabs(vni), fabs(vno)) + fmax(c_in, c_out);
+    return fmax(fabs(vni), fabs(vno)) + sqrt(P.gamma / fmin(hi[D + 3], ho[D + 3]));
 }
 
 template <int D>
