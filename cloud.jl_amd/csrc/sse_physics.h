@@ -290,15 +290,16 @@ __device__ __forceinline__ void ec_dot(const LawParams& P, const double* hL, con
 //   log(p / rho^gamma)                 = log p - gamma log rho
 //   ((gamma-1)/(-w_E)^gamma)^(1/(gamma-1)) exp(-s/(gamma-1))
 //                                      = exp((log(gamma-1) - gamma log(-w_E) - s) / (gamma-1))
-// (two transcendental calls per node instead of five; same values up to rounding).
+// (two transcendental calls per node instead of five; same values up to rounding), and
+// the divides as refined reciprocals.
 template <int D>
 __device__ __forceinline__ bool c2e_fast(const LawParams& P, const double* u, double* w) {
     double s = 0.0;
 #pragma unroll
     for (int m = 0; m < D; ++m) s += u[m + 1] * u[m + 1];
-    const double k = (0.5 / u[0]) * s;
+    const double k = (0.5 * rcp_refined(u[0])) * s;
     const double p = P.gm1 * (u[D + 1] - k);
-    const double inv_p = 1.0 / p;
+    const double inv_p = rcp_refined(p);
     w[0] = P.inv_gm1 * (P.gamma - (log(p) - P.gamma * log(u[0]))) - k * inv_p;
 #pragma unroll
     for (int m = 0; m < D; ++m) w[m + 1] = u[m + 1] * inv_p;
@@ -314,7 +315,7 @@ __device__ __forceinline__ bool e2c_fast(const LawParams& P, const double* win, 
     double s2 = 0.0;
 #pragma unroll
     for (int m = 0; m < D; ++m) s2 += w[m + 1] * w[m + 1];
-    const double k = s2 / (2.0 * w[D + 1]);
+    const double k = s2 * rcp_refined(2.0 * w[D + 1]);
     const double s = P.gamma - w[0] + k;
     const double rho_e = exp(P.inv_gm1 * (P.log_gm1 - P.gamma * log(-w[D + 1]) - s));
     u[0] = -w[D + 1] * rho_e;
