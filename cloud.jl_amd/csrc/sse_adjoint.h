@@ -92,7 +92,7 @@ struct EulerAdj {
     enum { H_RHO = 0, H_V = 1, H_P = D + 1, H_LR = D + 2, H_B = D + 3, H_LB = D + 4, H_IR = D + 5, H_IP = D + 6 };
 
     __device__ static void hoist_full(const LawParams& P, const double* u, double* h) {
-        const double ir = 1.0 / u[0];
+        const double ir = rcp_refined(u[0]);
         double s = 0.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) {
@@ -100,7 +100,7 @@ struct EulerAdj {
             s += h[H_V + m] * u[m + 1];
         }
         const double p = P.gm1 * (u[D + 1] - 0.5 * s);
-        const double ip = 1.0 / p;
+        const double ip = rcp_refined(p);
         h[H_RHO] = u[0];
         h[H_P] = p;
         h[H_LR] = log(u[0]);
