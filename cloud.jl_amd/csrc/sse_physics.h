@@ -162,7 +162,7 @@ struct Physics<D, LAW_EULER> {
 // (NH = D + 5 values). A pair then needs no logarithm: log(y/x) of the
 // reference's logmean (ConservationLaws.jl:143) becomes log y - log x, which is
 // only evaluated where |(y-x)/(y+x)| >= 1e-2, so its relative error stays
-// below 1e-14. The series/log branch is taken branch-free (one divide each).
+// below 1e-14. Both log-means of a flux come as fractions that share one reciprocal.
 //
 // ec_dot returns c_e = sum_n a_n F(uL,uR)[e,n] for a contraction vector a,
 // without forming the (D+2) x D flux tensor:
@@ -204,25 +204,6 @@ __device__ __forceinline__ void hoist(const LawParams& P, const double* u, doubl
     h[D + 2] = log(rho);
     h[D + 3] = beta;
     h[D + 4] = log(beta);
-}
-
-// logmean(x, y) from hoisted logs, branch-free selection, one divide after f^2
-__device__ __forceinline__ double logmean_h(double x, double y, double lx, double ly) {
-    const double t = (x - y) / (x + y);
-    const double f2 = t * t;
-    const bool ser = f2 < 1.0e-4;
-    const double num = ser ? (x + y) * 105.0 : (y - x);
-    const double den = ser ? (210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0))) : (ly - lx);
-    return num / den;
-}
-
-__device__ __forceinline__ double inv_logmean_h(double x, double y, double lx, double ly) {
-    const double t = (x - y) / (x + y);
-    const double f2 = t * t;
-    const bool ser = f2 < 1.0e-4;
-    const double num = ser ? (210.0 + f2 * (70.0 + f2 * (42.0 + f2 * 30.0))) : (ly - lx);
-    const double den = ser ? (x + y) * 105.0 : (y - x);
-    return num / den;
 }
 
 // The logarithmic mean as a fraction num/den (a branch, so that a wavefront none of whose
